@@ -1,0 +1,182 @@
+/*
+ * ssx.h -- C ABI of the MI355X (gfx950) elliptic-solve engine for SSENSS.
+ *
+ * All compute runs in hand-written HIP kernels on one GPU; the host side is
+ * C.  Vectors are element-local "E-vectors" exactly as the reference stores
+ * a double**** field: one contiguous block of nel*n^3 doubles, node (e,k,j,i)
+ * at ((e*n+k)*n+j)*n+i, i fastest (reference utils.c:64-81).  Unless a
+ * function says "host", vector arguments are DEVICE pointers.
+ *
+ * Every call is ordered on the context's HIP stream (ssx_set_stream).
+ * Functions that return a number to the host (ssx_dot, the solvers)
+ * synchronise that stream before returning; the others only enqueue work.
+ *
+ * Status: every function returns SSX_OK (0) or a positive SSX_E* code;
+ * ssx_last_error() gives a message for the last failure on this thread.
+ *
+ * Reference interfaces replaced (file:line in the SSENSS src/ tree):
+ *   ssx_create          connect_elements + initialise_preconditioners,
+ *                       init.c:38-50, precon.c:200-209 (the upload hook)
+ *   ssx_ax              discrete_laplace_operator poisson.c:34-56,
+ *                       discrete_helmholtz_operator helmholtz.c:33-49
+ *   ssx_opeval          global_opeval solver.c:191-198
+ *   ssx_dss             direct_stiffness_summation global.c:241-257
+ *   ssx_dss_average     direct_stiffness_averaging global.c:259-268
+ *   ssx_dot             global_dot_product global.c:270-282
+ *   ssx_mask            mask_dirichlet_boundaries poisson.c:89-99
+ *   ssx_nullspace       subtract_nullspace solver.c:200-213
+ *   ssx_restrict        multigrid_restriction precon.c:283-291
+ *   ssx_prolong         multigrid_prolongation precon.c:293-300
+ *   ssx_jacobi          jacobi_preconditioner precon.c:311-324
+ *   ssx_mg_precon       multigrid_preconditioner precon.c:227-236
+ *   ssx_cg              conjugate_gradient solver.c:33-81
+ *   ssx_fcg             flexible_conjugate_gradient solver.c:83-157
+ *   ssx_solve_poisson_cg    solve_poisson_cg solver.c:159-173 (host arrays)
+ *   ssx_solve_helmholtz_cg  solve_helmholtz_cg solver.c:175-189 (host arrays)
+ */
+#ifndef SSX_H
+#define SSX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSX_OK        0
+#define SSX_EINVAL    1   /* bad argument                                 */
+#define SSX_ENOMEM    2   /* host or device allocation failed             */
+#define SSX_EHIP      3   /* a HIP runtime call failed                    */
+#define SSX_ENOTCONV  4   /* solver stopped at max_iter above eps         */
+#define SSX_EMESH     5   /* connectivity the engine cannot assemble      */
+#define SSX_ENODEV    6   /* no GPU visible                               */
+#define SSX_ENUMERIC  7   /* NaN or Inf residual                          */
+
+/* variable codes, reference poisson.h:24-27 (enum PRESSURE ...) */
+#define SSX_PRESSURE    0   /* no Dirichlet mask, null space removed      */
+#define SSX_X_VELOCITY  1   /* Dirichlet mask, no null space              */
+#define SSX_Y_VELOCITY  2
+#define SSX_Z_VELOCITY  3
+
+#define SSX_OP_POISSON    0
+#define SSX_OP_HELMHOLTZ  1
+
+#define SSX_PC_JACOBI  0
+#define SSX_PC_MG      1
+
+/* ssx_level_data selectors */
+#define SSX_DATA_GFAC      0   /* 6*nel*n^3: G*w*detJ, [e][xx,xy,xz,yy,yz,zz][node] */
+#define SSX_DATA_MASS      1   /* w*detJ                                   */
+#define SSX_DATA_DIAG_P    2   /* inverse assembled Poisson diagonal       */
+#define SSX_DATA_DIAG_H    3   /* inverse assembled Helmholtz diagonal     */
+#define SSX_DATA_SCALE     4   /* 1 / multiplicity                         */
+#define SSX_DATA_MASK      5   /* Dirichlet mask                           */
+#define SSX_DATA_INVMASS   6   /* 1 / DSS(w*detJ), level 0                 */
+#define SSX_DATA_X         7   /* coordinates (x, y, z)                    */
+#define SSX_DATA_Y         8
+#define SSX_DATA_Z         9
+
+typedef struct ssx_ctx ssx_ctx;
+
+/* Mesh as SSENSS holds it after read_grid (dataio.c:197-227) and
+ * set_boundary_conditions (geometry.c:230-237): GLL coordinates per element
+ * and the grid.h5 connectivity fields (dataio.c:45-62).  Neighbour ids are
+ * element numbers >= 0 or boundary codes < 0 (-1 no-slip, -2 free-slip). */
+typedef struct ssx_mesh {
+	int n;                   /* GLL points per direction (polynomial order + 1) */
+	int nel;
+	const double *x, *y, *z;        /* nel*n^3 each, host */
+	const int *neigh;               /* nel*6: W,E,S,N,B,T */
+	const int *edge_neigh;          /* nel*12 */
+	const int *edge_size;           /* nel*12 */
+	const int *corn_neigh;          /* nel*8 */
+	const int *corn_size;           /* nel*8 */
+	double dt, visc;                /* Helmholtz lambda = (11/6)/(dt*visc) */
+	int svvn;                       /* SVV cut-off mode, reference main.c:54 */
+	double svva;                    /* SVV amplitude, reference main.c:55 */
+} ssx_mesh;
+
+typedef struct ssx_stats {
+	int iterations;          /* loop bodies of the outer solver            */
+	int converged;           /* residual <= eps                            */
+	double residual;         /* final sqrt(<r,r>_w)                        */
+	long long fine_ax;       /* fine-level operator applications          */
+	long long coarse_iters;  /* coarsest-level CG iterations (all cycles)  */
+	long long smooth_iters;  /* smoother CG iterations (all levels)        */
+	long long vcycles;
+	double seconds;          /* wall time of the solve                     */
+} ssx_stats;
+
+const char *ssx_last_error(void);
+const char *ssx_version(void);
+
+/* ---- structured mesh generator (host only; no device needed) ----
+ * A box of mx*my*mz hexahedra on [lo,hi] with GLL nodes, connectivity in the
+ * conventions of the reference's misc/rectangular_grid.py:80-135 (element
+ * id = i + j*mx + k*mx*my; faces/edges/corners wrap periodically, then
+ * boundary faces with bc[s] < 0 get that code with edge/corner size 0).
+ * bc[6] per face W,E,S,N,B,T: 0 periodic, -1 no-slip, -2 free-slip.
+ * deform: 0 none; 1 x += a sin(pi u)sin(pi v)sin(pi w) applied to each
+ * coordinate, with (u,v,w) the unit-cube coordinates; 2 tanh wall
+ * stretching of y with factor a: y = tanh(a eta)/tanh(a), eta in [-1,1].
+ * Arrays are caller-allocated with the sizes of ssx_mesh. */
+int ssx_mesh_box(int n, int mx, int my, int mz, const double lo[3],
+		const double hi[3], const int bc[6], int deform, double amp,
+		double *x, double *y, double *z, int *neigh, int *edge_neigh,
+		int *edge_size, int *corn_neigh, int *corn_size);
+
+/* ---- context ---- */
+int ssx_create(ssx_ctx **ctx, const ssx_mesh *mesh);
+void ssx_destroy(ssx_ctx *ctx);
+/* run all work of ctx on this hipStream_t (NULL: the context's own stream) */
+int ssx_set_stream(ssx_ctx *ctx, void *hip_stream);
+void *ssx_get_stream(ssx_ctx *ctx);
+int ssx_synchronize(ssx_ctx *ctx);
+/* levels, points per direction of a level, element count, local nodes */
+int ssx_nlevels(const ssx_ctx *ctx);
+int ssx_level_n(const ssx_ctx *ctx, int level);
+int ssx_nel(const ssx_ctx *ctx);
+size_t ssx_level_ntot(const ssx_ctx *ctx, int level);
+double ssx_lambda(const ssx_ctx *ctx);
+/* copy level data (SSX_DATA_*) to a host array */
+int ssx_level_data(ssx_ctx *ctx, int level, int which, double *host);
+/* shared-node groups of a level by multiplicity 2, 4, 8 */
+int ssx_dss_groups(const ssx_ctx *ctx, int level, long long counts[3]);
+/* Jacobi diagonal choice: 0 Poisson, 1 Helmholtz, -1 by var (default) */
+int ssx_set_diag_choice(ssx_ctx *ctx, int choice);
+
+/* ---- operators on device E-vectors ---- */
+int ssx_ax(ssx_ctx *ctx, int level, int op, const double *u, double *au);
+int ssx_opeval(ssx_ctx *ctx, int level, int op, int var, const double *p,
+		double *ap);
+int ssx_dss(ssx_ctx *ctx, int level, double *u);
+int ssx_dss_average(ssx_ctx *ctx, int level, double *u);
+int ssx_dot(ssx_ctx *ctx, int level, const double *x, const double *y,
+		double *out_host);
+int ssx_mask(ssx_ctx *ctx, int level, int var, double *u);
+int ssx_nullspace(ssx_ctx *ctx, int level, int var, double *u);
+int ssx_restrict(ssx_ctx *ctx, int level, const double *fine, double *coarse);
+int ssx_prolong(ssx_ctx *ctx, int level, const double *coarse, double *fine);
+int ssx_jacobi(ssx_ctx *ctx, int level, int var, const double *x, double *px);
+
+/* ---- solvers on device E-vectors (f is masked in place, u is the initial
+ * guess and the result, as in the reference) ---- */
+int ssx_cg(ssx_ctx *ctx, int level, int op, int var, double *f, double *u,
+		double eps, int max_iter, ssx_stats *stats);
+int ssx_fcg(ssx_ctx *ctx, int op, int var, int precon, double *f, double *u,
+		double eps, int max_iter, ssx_stats *stats);
+int ssx_mg_precon(ssx_ctx *ctx, int op, int var, double eps, const double *x,
+		double *px);
+
+/* ---- reference entry points on HOST E-vectors: f (masked in place) and u
+ * (initial guess in, solution out), FCG + p-multigrid for Poisson and
+ * Jacobi PCG for Helmholtz, max_iter 5000 ---- */
+int ssx_solve_poisson_cg(ssx_ctx *ctx, double *f, double *u, int var,
+		double eps, ssx_stats *stats);
+int ssx_solve_helmholtz_cg(ssx_ctx *ctx, double *f, double *u, int var,
+		double eps, ssx_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,17 @@
+"""ssenss_amd -- MI355X-native elliptic-solve engine for SSENSS.
+
+The engine is the C-ABI library ``libssx.so`` (C host code + gfx950 HIP
+kernels, declared in ``include/ssx.h``).  This package only loads it with
+ctypes and gives tests and benchmarks a thin Python handle; no numerics run
+in Python and there is no fallback path: if the library is missing, every
+entry point raises.
+"""
+from .native import (  # noqa: F401
+    SsxError, lib, load, library_path, Stats,
+    PRESSURE, X_VELOCITY, Y_VELOCITY, Z_VELOCITY,
+    OP_POISSON, OP_HELMHOLTZ, PC_JACOBI, PC_MG,
+    DATA_GFAC, DATA_MASS, DATA_DIAG_P, DATA_DIAG_H, DATA_SCALE, DATA_MASK,
+    DATA_INVMASS, DATA_X, DATA_Y, DATA_Z, EXPORTED_SYMBOLS,
+)
+from .mesh import Mesh, box_mesh  # noqa: F401
+from .engine import Context  # noqa: F401

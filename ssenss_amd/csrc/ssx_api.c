@@ -1,0 +1,927 @@
+/*
+ * ssx_api.c -- C host side of the ssx engine: context set-up, upload, and
+ * the solver control flow (PCG, flexible CG, p-multigrid V-cycle).  All
+ * vector work is enqueued as gfx950 kernels on the context stream; scalars
+ * (alpha, beta, inner products) stay in device memory, and the host reads
+ * one residual per iteration to decide termination.
+ *
+ * The algorithms follow the reference exactly (citations: SSENSS src/):
+ *   conjugate_gradient           solver.c:33-81
+ *   flexible_conjugate_gradient  solver.c:83-157
+ *   global_opeval                solver.c:191-198
+ *   subtract_nullspace           solver.c:200-213
+ *   multigrid V-cycle            precon.c:227-309
+ *   set-up of levels/diagonals   precon.c:58-83, 121-209, 336-413,
+ *                                helmholtz.c:51-77, geometry.c:102-135
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <hip/hip_runtime_api.h>
+#include "ssx_internal.h"
+
+#define HIPC(call)                                                      \
+	do {                                                            \
+		hipError_t e_ = (call);                                 \
+		if (e_ != hipSuccess) {                                 \
+			ssx_set_error("%s: %s", #call, hipGetErrorString(e_)); \
+			return SSX_EHIP;                                \
+		}                                                       \
+	} while (0)
+#define TRY(call)                                                       \
+	do {                                                            \
+		int rc_ = (call);                                       \
+		if (rc_ != SSX_OK)                                      \
+			return rc_;                                     \
+	} while (0)
+
+const char *ssx_version(void)
+{
+	return "ssx 0.1 (gfx950)";
+}
+
+static double now_s(void)
+{
+	struct timespec ts;
+	clock_gettime(CLOCK_MONOTONIC, &ts);
+	return (double) ts.tv_sec + 1e-9 * (double) ts.tv_nsec;
+}
+
+static int dalloc(double **p, size_t n)
+{
+	if (hipMalloc((void **) p, (n ? n : 1) * sizeof(double)) != hipSuccess) {
+		*p = NULL;
+		ssx_set_error("hipMalloc of %zu doubles failed", n);
+		return SSX_ENOMEM;
+	}
+	return SSX_OK;
+}
+
+static int up(double **d, const double *h, size_t n)
+{
+	TRY(dalloc(d, n));
+	HIPC(hipMemcpy(*d, h, n * sizeof(double), hipMemcpyHostToDevice));
+	return SSX_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* host geometry of one level (geometry.c:102-135, deriv.c:68-79,       */
+/* tripalg.c:165-234, helmholtz.c:51-77)                               */
+/* ------------------------------------------------------------------ */
+static void level_geometry(const ssx_basis *b, int nel, const double *x,
+		const double *y, const double *z, double lambda, double *gf,
+		double *mass, double *dp, double *dh)
+{
+	const int n = b->n, n2 = n * n, n3 = n2 * n;
+	double *w3 = malloc(sizeof(double) * n3);
+	double *G = malloc(sizeof(double) * 6 * n3);
+	for (int k = 0; k < n; ++k)
+		for (int j = 0; j < n; ++j)
+			for (int i = 0; i < n; ++i)
+				w3[(k * n + j) * n + i] = b->rho[i] * b->rho[j] * b->rho[k];
+	const double *D = b->d;
+	for (int e = 0; e < nel; ++e) {
+		const size_t off = (size_t) e * n3;
+		const double *c[3] = { x + off, y + off, z + off };
+		for (int k = 0; k < n; ++k)
+		for (int j = 0; j < n; ++j)
+		for (int i = 0; i < n; ++i) {
+			const int q = (k * n + j) * n + i;
+			double J[3][3];
+			for (int a = 0; a < 3; ++a) {
+				double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+				for (int l = 0; l < n; ++l) {
+					s0 += D[i * n + l] * c[a][(k * n + j) * n + l];
+					s1 += D[j * n + l] * c[a][(k * n + l) * n + i];
+					s2 += D[k * n + l] * c[a][(l * n + j) * n + i];
+				}
+				J[a][0] = s0;
+				J[a][1] = s1;
+				J[a][2] = s2;
+			}
+			const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+			const double c01 = J[1][0] * J[2][2] - J[1][2] * J[2][0];
+			const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+			const double det = J[0][0] * c00 - J[0][1] * c01 + J[0][2] * c02;
+			double Ji[3][3];
+			Ji[0][0] = c00 / det;
+			Ji[0][1] = -(J[0][1] * J[2][2] - J[0][2] * J[2][1]) / det;
+			Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+			Ji[1][0] = -c01 / det;
+			Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+			Ji[1][2] = -(J[0][0] * J[1][2] - J[0][2] * J[1][0]) / det;
+			Ji[2][0] = c02 / det;
+			Ji[2][1] = -(J[0][0] * J[2][1] - J[0][1] * J[2][0]) / det;
+			Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+			/* G = Jinv Jinv^T (rows of the inverse Jacobian) */
+			double g6[6];
+			const int pa[6] = { 0, 0, 0, 1, 1, 2 }, pb[6] = { 0, 1, 2, 1, 2, 2 };
+			for (int s = 0; s < 6; ++s)
+				g6[s] = Ji[pa[s]][0] * Ji[pb[s]][0]
+					+ Ji[pa[s]][1] * Ji[pb[s]][1]
+					+ Ji[pa[s]][2] * Ji[pb[s]][2];
+			const double wj = w3[q] * det;
+			for (int s = 0; s < 6; ++s) {
+				G[s * n3 + q] = g6[s];
+				gf[(size_t) e * 6 * n3 + (size_t) s * n3 + q] = g6[s] * wj;
+			}
+			mass[off + q] = wj;
+		}
+		/* approximate Jacobi diagonal: no detJ, no cross terms
+		 * (helmholtz.c:51-68), lambda term for the Helmholtz one */
+		for (int k = 0; k < n; ++k)
+		for (int j = 0; j < n; ++j)
+		for (int i = 0; i < n; ++i) {
+			const int q = (k * n + j) * n + i;
+			double t = 0.0;
+			for (int p = 0; p < n; ++p) {
+				const int q1 = (k * n + j) * n + p, q2 = (k * n + p) * n + i,
+					q3 = (p * n + j) * n + i;
+				t += w3[q1] * (D[p * n + i] * D[p * n + i]) * G[0 * n3 + q1];
+				t += w3[q2] * (D[p * n + j] * D[p * n + j]) * G[3 * n3 + q2];
+				t += w3[q3] * (D[p * n + k] * D[p * n + k]) * G[5 * n3 + q3];
+			}
+			dp[off + q] = t;
+			dh[off + q] = t + lambda * mass[off + q];
+		}
+	}
+	free(G);
+	free(w3);
+}
+
+/* coordinates of a coarser level: the finer level's coordinate polynomial
+ * evaluated at the coarse GLL points (precon.c:144-152) */
+static void coarsen_coords(const ssx_basis *bf, const ssx_basis *bc, int nel,
+		const double *xf, double *xc)
+{
+	const int nf = bf->n, nc = bc->n;
+	double A[SSX_NMAX * SSX_NMAX];
+	ssx_interp(bf->xi, nf, bc->xi, nc, A);  /* A[p*nc+i] = l_p^f(xi_i^c) */
+	double *t1 = malloc(sizeof(double) * nf * nf * nc);
+	double *t2 = malloc(sizeof(double) * nf * nc * nc);
+	for (int e = 0; e < nel; ++e) {
+		const double *u = xf + (size_t) e * nf * nf * nf;
+		double *v = xc + (size_t) e * nc * nc * nc;
+		for (int kj = 0; kj < nf * nf; ++kj)
+			for (int i = 0; i < nc; ++i) {
+				double s = 0.0;
+				for (int p = 0; p < nf; ++p)
+					s += A[p * nc + i] * u[kj * nf + p];
+				t1[kj * nc + i] = s;
+			}
+		for (int k = 0; k < nf; ++k)
+			for (int j = 0; j < nc; ++j)
+				for (int i = 0; i < nc; ++i) {
+					double s = 0.0;
+					for (int l = 0; l < nf; ++l)
+						s += t1[(k * nf + l) * nc + i] * A[l * nc + j];
+					t2[(k * nc + j) * nc + i] = s;
+				}
+		for (int k = 0; k < nc; ++k)
+			for (int ji = 0; ji < nc * nc; ++ji) {
+				double s = 0.0;
+				for (int l = 0; l < nf; ++l)
+					s += t2[l * nc * nc + ji] * A[l * nc + k];
+				v[k * nc * nc + ji] = s;
+			}
+	}
+	free(t1);
+	free(t2);
+}
+
+static int setup_level(ssx_ctx *c, const ssx_mesh *m, int l, double *hx,
+		double *hy, double *hz)
+{
+	ssx_level *L = &c->lev[l];
+	const int n = L->n, n3 = L->n3;
+	const size_t nt = L->ntot;
+	int rc;
+	double *gf = malloc(sizeof(double) * 6 * nt);
+	double *mass = malloc(sizeof(double) * nt);
+	double *dp = malloc(sizeof(double) * nt);
+	double *dh = malloc(sizeof(double) * nt);
+	double *one = malloc(sizeof(double) * nt);
+	if (!gf || !mass || !dp || !dh || !one) {
+		free(gf); free(mass); free(dp); free(dh); free(one);
+		ssx_set_error("out of host memory (level %d)", l);
+		return SSX_ENOMEM;
+	}
+	level_geometry(&L->b, c->nel, hx, hy, hz, c->lambda, gf, mass, dp, dh);
+	rc = ssx_build_groups(m, n, L->grp, L->ngrp);
+	if (rc != SSX_OK)
+		goto out;
+	/* assembled diagonals and multiplicity (precon.c:336-381) */
+	ssx_host_dss(L->grp, L->ngrp, dp);
+	ssx_host_dss(L->grp, L->ngrp, dh);
+	for (size_t q = 0; q < nt; ++q) {
+		dp[q] = 1.0 / dp[q];
+		dh[q] = 1.0 / dh[q];
+		one[q] = 1.0;
+	}
+	ssx_host_dss(L->grp, L->ngrp, one);
+	double nun = 0.0;
+	for (size_t q = 0; q < nt; ++q) {
+		one[q] = 1.0 / one[q];
+		nun += one[q];
+	}
+	L->nunique = nun;
+	rc = up(&L->gf, gf, 6 * nt);
+	if (!rc) rc = up(&L->mass, mass, nt);
+	if (!rc) rc = up(&L->diag_p, dp, nt);
+	if (!rc) rc = up(&L->diag_h, dh, nt);
+	if (!rc) rc = up(&L->scale, one, nt);
+	if (rc)
+		goto out;
+	if (l == 0) {            /* inverse mass, precon.c:397-413 */
+		memcpy(one, mass, sizeof(double) * nt);
+		ssx_host_dss(L->grp, L->ngrp, one);
+		for (size_t q = 0; q < nt; ++q)
+			one[q] = 1.0 / one[q];
+		rc = up(&c->inv_mass, one, nt);
+		if (rc)
+			goto out;
+	}
+	/* Dirichlet mask (precon.c:383-395, poisson.c:101-143) */
+	for (int e = 0; e < c->nel; ++e) {
+		int bits = 0;
+		for (int s = 0; s < 6; ++s)
+			if (m->neigh[e * 6 + s] == -1)
+				bits |= 1 << s;
+		for (int k = 0; k < n; ++k)
+			for (int j = 0; j < n; ++j)
+				for (int i = 0; i < n; ++i) {
+					int z0 = ((bits & 1) && i == 0) || ((bits & 2) && i == n - 1)
+						|| ((bits & 4) && j == 0) || ((bits & 8) && j == n - 1)
+						|| ((bits & 16) && k == 0) || ((bits & 32) && k == n - 1);
+					one[(size_t) e * n3 + (k * n + j) * n + i] = z0 ? 0.0 : 1.0;
+				}
+	}
+	rc = up(&L->mask, one, nt);
+	if (rc)
+		goto out;
+	{
+		double dm[2 * SSX_NMAX * SSX_NMAX];
+		memcpy(dm, L->b.d, sizeof(double) * n * n);
+		memcpy(dm + n * n, L->b.dsvv, sizeof(double) * n * n);
+		rc = up(&L->dmat, dm, 2 * (size_t) n * n);
+		if (rc)
+			goto out;
+	}
+	for (int mi = 0; mi < 3; ++mi) {
+		const int mult = 2 << mi;
+		int *d;
+		size_t bytes = sizeof(int) * (size_t) (L->ngrp[mi] ? L->ngrp[mi] : 1)
+			* mult;
+		if (hipMalloc((void **) &d, bytes) != hipSuccess) {
+			rc = SSX_ENOMEM;
+			ssx_set_error("hipMalloc groups");
+			goto out;
+		}
+		if (L->ngrp[mi] && hipMemcpy(d, L->grp[mi], sizeof(int) * (size_t)
+				L->ngrp[mi] * mult, hipMemcpyHostToDevice) != hipSuccess) {
+			rc = SSX_EHIP;
+			ssx_set_error("upload groups");
+			goto out;
+		}
+		free(L->grp[mi]);
+		L->grp[mi] = d;
+	}
+	{
+		int *bcb = malloc(sizeof(int) * c->nel);
+		for (int e = 0; e < c->nel; ++e) {
+			int bits = 0;
+			for (int s = 0; s < 6; ++s)
+				if (m->neigh[e * 6 + s] == -1)
+					bits |= 1 << s;
+			bcb[e] = bits;
+		}
+		if (hipMalloc((void **) &L->bc, sizeof(int) * c->nel) != hipSuccess
+				|| hipMemcpy(L->bc, bcb, sizeof(int) * c->nel,
+					hipMemcpyHostToDevice) != hipSuccess) {
+			free(bcb);
+			rc = SSX_EHIP;
+			ssx_set_error("upload bc");
+			goto out;
+		}
+		free(bcb);
+	}
+	rc = dalloc(&L->r, nt);
+	if (!rc) rc = dalloc(&L->z, nt);
+	if (!rc) rc = dalloc(&L->p, nt);
+	if (!rc) rc = dalloc(&L->ap, nt);
+	if (!rc) rc = dalloc(&L->mg_r, nt);
+	if (!rc) rc = dalloc(&L->mg_fn, nt);
+	if (!rc) rc = dalloc(&L->mg_e, nt);
+out:
+	free(gf); free(mass); free(dp); free(dh); free(one);
+	return rc;
+}
+
+int ssx_create(ssx_ctx **out, const ssx_mesh *m)
+{
+	*out = NULL;
+	if (!m || m->n < SSX_NMIN || m->n > SSX_NMAX || m->nel < 1) {
+		ssx_set_error("ssx_create: n must be in [%d,%d] and nel >= 1",
+				SSX_NMIN, SSX_NMAX);
+		return SSX_EINVAL;
+	}
+	int ndev = 0;
+	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+		ssx_set_error("ssx_create: no GPU visible");
+		return SSX_ENODEV;
+	}
+	ssx_ctx *c = calloc(1, sizeof(*c));
+	if (!c)
+		return SSX_ENOMEM;
+	c->nel = m->nel;
+	c->n = m->n;
+	c->diag_choice = -1;
+	c->lambda = (11.0 / 6.0) / (m->dt * m->visc);
+	int nl = SSX_LMAX;
+	for (int i = 0; i < SSX_LMAX; ++i) {     /* precon.c:62-70 */
+		int v = (m->n - 1) / (1 << i) + 1;
+		c->nmg[i] = v > 2 ? v : 2;
+		if (c->nmg[i] == 2) {
+			nl = i + 1;
+			break;
+		}
+	}
+	c->nlev = nl;
+	hipStream_t st;
+	if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+		free(c);
+		ssx_set_error("hipStreamCreate failed");
+		return SSX_EHIP;
+	}
+	c->own_stream = st;
+	c->stream = st;
+	int rc = SSX_OK;
+	/* coordinates of the previous (finer) level; level 0 keeps its own */
+	double *px = NULL, *py = NULL, *pz = NULL;
+	for (int l = 0; l < nl && rc == SSX_OK; ++l) {
+		ssx_level *L = &c->lev[l];
+		L->n = c->nmg[l];
+		L->n3 = L->n * L->n * L->n;
+		L->ntot = (size_t) c->nel * L->n3;
+		ssx_basis_init(&L->b, L->n, m->svvn, m->svva);
+		double *nx = malloc(sizeof(double) * L->ntot);
+		double *ny = malloc(sizeof(double) * L->ntot);
+		double *nz = malloc(sizeof(double) * L->ntot);
+		if (!nx || !ny || !nz) {
+			free(nx); free(ny); free(nz);
+			ssx_set_error("out of host memory (coordinates)");
+			rc = SSX_ENOMEM;
+			break;
+		}
+		if (l == 0) {
+			memcpy(nx, m->x, sizeof(double) * L->ntot);
+			memcpy(ny, m->y, sizeof(double) * L->ntot);
+			memcpy(nz, m->z, sizeof(double) * L->ntot);
+		} else {
+			const ssx_basis *bf = &c->lev[l - 1].b;
+			coarsen_coords(bf, &L->b, c->nel, px, nx);
+			coarsen_coords(bf, &L->b, c->nel, py, ny);
+			coarsen_coords(bf, &L->b, c->nel, pz, nz);
+		}
+		if (l >= 2) {
+			free(px); free(py); free(pz);
+		}
+		px = nx; py = ny; pz = nz;
+		if (l == 0) {
+			L->h_x = nx; L->h_y = ny; L->h_z = nz;
+		}
+		rc = setup_level(c, m, l, nx, ny, nz);
+	}
+	if (px && px != c->lev[0].h_x) {
+		free(px); free(py); free(pz);
+	}
+	/* prolongation matrices */
+	for (int l = 0; l + 1 < nl && rc == SSX_OK; ++l) {
+		double P[SSX_NMAX * SSX_NMAX];
+		ssx_interp(c->lev[l + 1].b.xi, c->lev[l + 1].n, c->lev[l].b.xi,
+				c->lev[l].n, P);
+		rc = up(&c->lev[l].prol, P, (size_t) c->lev[l + 1].n * c->lev[l].n);
+	}
+	if (rc == SSX_OK) {
+		const size_t nt = c->lev[0].ntot;
+		rc = dalloc(&c->scal, SSX_NSCAL);
+		if (!rc) rc = dalloc(&c->part, (size_t) 8 * SSX_NPART);
+		if (!rc) rc = dalloc(&c->fcg_r, nt);
+		if (!rc) rc = dalloc(&c->fcg_z, nt);
+		if (!rc && hipHostMalloc((void **) &c->h_scal,
+				SSX_NSCAL * sizeof(double), 0) != hipSuccess) {
+			rc = SSX_ENOMEM;
+			ssx_set_error("hipHostMalloc");
+		}
+		if (!rc)
+			HIPC(hipMemset(c->scal, 0, SSX_NSCAL * sizeof(double)));
+	}
+	if (rc != SSX_OK) {
+		ssx_destroy(c);
+		return rc;
+	}
+	*out = c;
+	return SSX_OK;
+}
+
+void ssx_destroy(ssx_ctx *c)
+{
+	if (!c)
+		return;
+	hipStreamSynchronize((hipStream_t) c->stream);
+	for (int l = 0; l < c->nlev; ++l) {
+		ssx_level *L = &c->lev[l];
+		hipFree(L->gf); hipFree(L->mass); hipFree(L->diag_p);
+		hipFree(L->diag_h); hipFree(L->scale); hipFree(L->mask);
+		hipFree(L->dmat); hipFree(L->prol); hipFree(L->bc);
+		for (int mi = 0; mi < 3; ++mi)
+			hipFree(L->grp[mi]);
+		hipFree(L->r); hipFree(L->z); hipFree(L->p); hipFree(L->ap);
+		hipFree(L->mg_r); hipFree(L->mg_fn); hipFree(L->mg_e);
+		free(L->h_x); free(L->h_y); free(L->h_z);
+	}
+	hipFree(c->inv_mass);
+	hipFree(c->scal);
+	hipFree(c->part);
+	hipFree(c->fcg_r);
+	hipFree(c->fcg_z);
+	for (int i = 0; i < 11; ++i)
+		hipFree(c->fcg_p[i]);
+	for (int i = 0; i < 10; ++i)
+		hipFree(c->fcg_ap[i]);
+	hipFree(c->d_f);
+	hipFree(c->d_u);
+	if (c->h_scal)
+		hipHostFree(c->h_scal);
+	if (c->own_stream)
+		hipStreamDestroy((hipStream_t) c->own_stream);
+	free(c);
+}
+
+int ssx_set_stream(ssx_ctx *c, void *s)
+{
+	if (!c)
+		return SSX_EINVAL;
+	HIPC(hipStreamSynchronize((hipStream_t) c->stream));
+	c->stream = s ? s : c->own_stream;
+	return SSX_OK;
+}
+
+void *ssx_get_stream(ssx_ctx *c) { return c ? c->stream : NULL; }
+
+int ssx_synchronize(ssx_ctx *c)
+{
+	HIPC(hipStreamSynchronize((hipStream_t) c->stream));
+	return SSX_OK;
+}
+
+int ssx_nlevels(const ssx_ctx *c) { return c->nlev; }
+int ssx_nel(const ssx_ctx *c) { return c->nel; }
+double ssx_lambda(const ssx_ctx *c) { return c->lambda; }
+
+int ssx_level_n(const ssx_ctx *c, int l)
+{
+	return (l >= 0 && l < c->nlev) ? c->lev[l].n : -1;
+}
+
+size_t ssx_level_ntot(const ssx_ctx *c, int l)
+{
+	return (l >= 0 && l < c->nlev) ? c->lev[l].ntot : 0;
+}
+
+int ssx_set_diag_choice(ssx_ctx *c, int choice)
+{
+	if (choice < -1 || choice > 1)
+		return SSX_EINVAL;
+	c->diag_choice = choice;
+	return SSX_OK;
+}
+
+int ssx_dss_groups(const ssx_ctx *c, int l, long long counts[3])
+{
+	if (l < 0 || l >= c->nlev)
+		return SSX_EINVAL;
+	for (int i = 0; i < 3; ++i)
+		counts[i] = c->lev[l].ngrp[i];
+	return SSX_OK;
+}
+
+int ssx_level_data(ssx_ctx *c, int l, int which, double *host)
+{
+	if (l < 0 || l >= c->nlev || !host)
+		return SSX_EINVAL;
+	ssx_level *L = &c->lev[l];
+	const double *src = NULL;
+	size_t n = L->ntot;
+	switch (which) {
+	case SSX_DATA_GFAC: src = L->gf; n = 6 * L->ntot; break;
+	case SSX_DATA_MASS: src = L->mass; break;
+	case SSX_DATA_DIAG_P: src = L->diag_p; break;
+	case SSX_DATA_DIAG_H: src = L->diag_h; break;
+	case SSX_DATA_SCALE: src = L->scale; break;
+	case SSX_DATA_MASK: src = L->mask; break;
+	case SSX_DATA_INVMASS: src = l == 0 ? c->inv_mass : NULL; break;
+	case SSX_DATA_X: case SSX_DATA_Y: case SSX_DATA_Z:
+		if (l != 0)
+			return SSX_EINVAL;
+		memcpy(host, which == SSX_DATA_X ? L->h_x : which == SSX_DATA_Y
+				? L->h_y : L->h_z, sizeof(double) * n);
+		return SSX_OK;
+	default: return SSX_EINVAL;
+	}
+	if (!src)
+		return SSX_EINVAL;
+	HIPC(hipStreamSynchronize((hipStream_t) c->stream));
+	HIPC(hipMemcpy(host, src, n * sizeof(double), hipMemcpyDeviceToHost));
+	return SSX_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* operators                                                            */
+/* ------------------------------------------------------------------ */
+
+static int chk(const ssx_ctx *c, int l)
+{
+	if (!c || l < 0 || l >= c->nlev) {
+		ssx_set_error("bad context or level %d", l);
+		return SSX_EINVAL;
+	}
+	return SSX_OK;
+}
+
+int ssx_ax(ssx_ctx *c, int l, int op, const double *u, double *au)
+{
+	TRY(chk(c, l));
+	ssx_level *L = &c->lev[l];
+	return ssxk_ax(c->stream, L->n, c->nel, op, u, au, L->gf, L->mass,
+			L->dmat, c->lambda, L->bc, 0);
+}
+
+int ssx_dss(ssx_ctx *c, int l, double *u)
+{
+	TRY(chk(c, l));
+	return ssxk_dss(c->stream, c->lev[l].grp, c->lev[l].ngrp, u);
+}
+
+int ssx_opeval(ssx_ctx *c, int l, int op, int var, const double *p,
+		double *ap)
+{
+	TRY(chk(c, l));
+	ssx_level *L = &c->lev[l];
+	if (l == 0)
+		c->cnt[0]++;
+	/* element operator, mask (var != PRESSURE), DSS: solver.c:191-198 */
+	TRY(ssxk_ax(c->stream, L->n, c->nel, op, p, ap, L->gf, L->mass,
+			L->dmat, c->lambda, L->bc, var != SSX_PRESSURE));
+	return ssxk_dss(c->stream, L->grp, L->ngrp, ap);
+}
+
+int ssx_dss_average(ssx_ctx *c, int l, double *u)
+{
+	TRY(ssx_dss(c, l, u));
+	return ssxk_mul(c->stream, c->lev[l].ntot, u, c->lev[l].scale, u);
+}
+
+static int dot_to(ssx_ctx *c, int l, const double *x, const double *y,
+		int dst)
+{
+	ssx_level *L = &c->lev[l];
+	TRY(ssxk_dot(c->stream, L->ntot, x, L->scale, y, c->part, 0));
+	return ssxk_reduce(c->stream, c->part, 0, c->scal, dst);
+}
+
+static int read_scal(ssx_ctx *c, int slot, double *v)
+{
+	HIPC(hipMemcpyAsync(c->h_scal + slot, c->scal + slot, sizeof(double),
+			hipMemcpyDeviceToHost, (hipStream_t) c->stream));
+	HIPC(hipStreamSynchronize((hipStream_t) c->stream));
+	*v = c->h_scal[slot];
+	return SSX_OK;
+}
+
+int ssx_dot(ssx_ctx *c, int l, const double *x, const double *y, double *out)
+{
+	TRY(chk(c, l));
+	TRY(dot_to(c, l, x, y, SC_TMP));
+	return read_scal(c, SC_TMP, out);
+}
+
+int ssx_mask(ssx_ctx *c, int l, int var, double *u)
+{
+	TRY(chk(c, l));
+	if (var == SSX_PRESSURE)          /* poisson.c:91-92 */
+		return SSX_OK;
+	return ssxk_mul(c->stream, c->lev[l].ntot, u, c->lev[l].mask, u);
+}
+
+int ssx_nullspace(ssx_ctx *c, int l, int var, double *u)
+{
+	TRY(chk(c, l));
+	if (var != SSX_PRESSURE)          /* solver.c:202-203 */
+		return SSX_OK;
+	ssx_level *L = &c->lev[l];
+	TRY(ssxk_dot(c->stream, L->ntot, u, L->scale, NULL, c->part, 0));
+	TRY(ssxk_reduce(c->stream, c->part, 0, c->scal, SC_NULLC));
+	return ssxk_shift_s(c->stream, L->ntot, c->scal, SC_NULLC, L->nunique, u);
+}
+
+int ssx_restrict(ssx_ctx *c, int l, const double *fine, double *coarse)
+{
+	if (l < 1 || l >= c->nlev)
+		return SSX_EINVAL;
+	TRY(ssxk_restrict(c->stream, c->lev[l - 1].n, c->lev[l].n, c->nel,
+			c->lev[l - 1].prol, fine, coarse));
+	return ssx_dss_average(c, l, coarse);   /* precon.c:283-291 */
+}
+
+int ssx_prolong(ssx_ctx *c, int l, const double *coarse, double *fine)
+{
+	if (l < 0 || l + 1 >= c->nlev)
+		return SSX_EINVAL;
+	return ssxk_prolong(c->stream, c->lev[l].n, c->lev[l + 1].n, c->nel,
+			c->lev[l].prol, coarse, fine, 0);
+}
+
+static const double *jdiag(ssx_ctx *c, int l, int var)
+{
+	int ch = c->diag_choice;
+	if (ch < 0)
+		ch = var == SSX_PRESSURE ? 0 : 1;      /* precon.c:211-225 */
+	return ch == 0 ? c->lev[l].diag_p : c->lev[l].diag_h;
+}
+
+int ssx_jacobi(ssx_ctx *c, int l, int var, const double *x, double *px)
+{
+	TRY(chk(c, l));
+	return ssxk_mul(c->stream, c->lev[l].ntot, x, jdiag(c, l, var), px);
+}
+
+/* ------------------------------------------------------------------ */
+/* solvers                                                              */
+/* ------------------------------------------------------------------ */
+
+/* conjugate_gradient, solver.c:33-81, Jacobi preconditioner at level l */
+static int cg_level(ssx_ctx *c, int l, int op, int var, double *f, double *u,
+		double eps, int max_iter, int *iters, double *err_out)
+{
+	ssx_level *L = &c->lev[l];
+	const size_t nt = L->ntot;
+	double *r = L->r, *z = L->z, *p = L->p, *ap = L->ap;
+	void *st = c->stream;
+	const double *mask = var == SSX_PRESSURE ? NULL : L->mask;
+	TRY(ssx_mask(c, l, var, f));
+	TRY(ssx_opeval(c, l, op, var, u, r));
+	TRY(ssxk_axpby(st, nt, 1.0, f, NULL, -1.0, r));
+	TRY(ssx_nullspace(c, l, var, r));
+	TRY(ssx_jacobi(c, l, var, r, z));
+	TRY(ssx_nullspace(c, l, var, z));
+	TRY(ssxk_copy(st, p, z, nt));
+	TRY(dot_to(c, l, r, z, SC_RDZ));
+	int j = 0, it = 0;
+	double err = 0.0;
+	do {
+		TRY(ssx_opeval(c, l, op, var, p, ap));
+		TRY(dot_to(c, l, p, ap, SC_PAP));
+		TRY(ssxk_scalar(st, c->scal, 0, SC_ALPHA, SC_RDZ, SC_PAP, 0.0));
+		TRY(ssxk_axpy_s(st, nt, c->scal, SC_ALPHA, 1.0, p, u));
+		TRY(ssxk_axpy_s(st, nt, c->scal, SC_ALPHA, -1.0, ap, r));
+		TRY(ssx_jacobi(c, l, var, r, z));
+		TRY(ssx_nullspace(c, l, var, z));
+		TRY(ssxk_scalar(st, c->scal, 2, SC_RDZ_OLD, SC_RDZ, 0, 0.0));
+		TRY(dot_to(c, l, r, z, SC_RDZ));
+		TRY(ssxk_scalar(st, c->scal, 0, SC_BETA, SC_RDZ, SC_RDZ_OLD, 0.0));
+		TRY(ssxk_xpby_s(st, nt, z, c->scal, SC_BETA, p, mask));
+		TRY(dot_to(c, l, r, r, SC_RR));
+		double rr;
+		TRY(read_scal(c, SC_RR, &rr));
+		err = sqrt(rr);
+		++it;
+		if (isnan(err) || isinf(err)) {
+			ssx_set_error("CG level %d: residual is %g", l, err);
+			if (iters)
+				*iters = it;
+			*err_out = err;
+			return SSX_ENUMERIC;
+		}
+	} while (err > eps && ++j < max_iter);
+	if (iters)
+		*iters = it;
+	*err_out = err;
+	return SSX_OK;
+}
+
+static int mg_recurse(ssx_ctx *c, int l, int op, int var, double *f,
+		double *x, double eps)
+{
+	double err;
+	int it;
+	if (l == c->nlev - 1) {          /* precon.c:242-244 */
+		TRY(cg_level(c, l, op, var, f, x, eps, 500, &it, &err));
+		c->cnt[1] += it;
+		return SSX_OK;
+	}
+	ssx_level *L = &c->lev[l], *C = &c->lev[l + 1];
+	void *st = c->stream;
+	double *r = L->mg_r, *fn = C->mg_fn, *e = C->mg_e;
+	if (l == 0) {                    /* precon.c:302-309 */
+		TRY(ssx_opeval(c, l, op, var, x, r));
+		TRY(ssxk_axpby(st, L->ntot, 1.0, f, NULL, -1.0, r));
+	} else {
+		TRY(ssxk_copy(st, r, f, L->ntot));
+	}
+	TRY(ssx_restrict(c, l + 1, r, fn));
+	TRY(ssxk_fill(st, e, 0.0, C->ntot));
+	TRY(mg_recurse(c, l + 1, op, var, fn, e, eps));
+	/* x += J e (prolongation fused with the add, precon.c:262-263) */
+	TRY(ssxk_prolong(st, L->n, C->n, c->nel, L->prol, e, x, 1));
+	TRY(cg_level(c, l, op, var, f, x, eps, 15, &it, &err));
+	c->cnt[2] += it;
+	return SSX_OK;
+}
+
+int ssx_mg_precon(ssx_ctx *c, int op, int var, double eps, const double *x,
+		double *px)
+{
+	TRY(chk(c, 0));
+	c->cnt[3]++;
+	TRY(ssx_jacobi(c, 0, var, x, px));      /* precon.c:230 */
+	return mg_recurse(c, 0, op, var, (double *) x, px, eps);
+}
+
+int ssx_cg(ssx_ctx *c, int l, int op, int var, double *f, double *u,
+		double eps, int max_iter, ssx_stats *stats)
+{
+	TRY(chk(c, l));
+	long long c0[4];
+	memcpy(c0, c->cnt, sizeof(c0));
+	double t0 = now_s(), err;
+	int it = 0;
+	int rc = cg_level(c, l, op, var, f, u, eps, max_iter, &it, &err);
+	if (rc == SSX_OK)
+		rc = ssx_synchronize(c);
+	if (stats) {
+		memset(stats, 0, sizeof(*stats));
+		stats->iterations = it;
+		stats->residual = err;
+		stats->converged = err <= eps;
+		stats->fine_ax = c->cnt[0] - c0[0];
+		stats->seconds = now_s() - t0;
+	}
+	if (rc == SSX_OK && !(err <= eps))
+		rc = SSX_ENOTCONV;
+	return rc;
+}
+
+int ssx_fcg(ssx_ctx *c, int op, int var, int precon, double *f, double *u,
+		double eps, int max_iter, ssx_stats *stats)
+{
+	TRY(chk(c, 0));
+	const int mmax = precon == SSX_PC_MG ? 10 : 1;   /* solver.c:90-92 */
+	ssx_level *L = &c->lev[0];
+	const size_t nt = L->ntot;
+	void *st = c->stream;
+	for (int i = 0; i < mmax + 1; ++i)
+		if (!c->fcg_p[i])
+			TRY(dalloc(&c->fcg_p[i], nt));
+	for (int i = 0; i < mmax; ++i)
+		if (!c->fcg_ap[i])
+			TRY(dalloc(&c->fcg_ap[i], nt));
+	double *p[11], *ap[10];
+	for (int i = 0; i < 11; ++i)
+		p[i] = c->fcg_p[i];
+	for (int i = 0; i < 10; ++i)
+		ap[i] = c->fcg_ap[i];
+	double *r = c->fcg_r, *z = c->fcg_z;
+	const double *mask = var == SSX_PRESSURE ? NULL : L->mask;
+	long long c0[4];
+	memcpy(c0, c->cnt, sizeof(c0));
+	double t0 = now_s();
+	TRY(ssx_mask(c, 0, var, f));
+	TRY(ssx_opeval(c, 0, op, var, u, r));
+	TRY(ssxk_axpby(st, nt, 1.0, f, NULL, -1.0, r));
+	TRY(ssx_nullspace(c, 0, var, r));
+	long j = 0;
+	int it = 0;
+	double err = 0.0;
+	int rc = SSX_OK;
+	do {
+		int m = (int) j, mm = (int) (j % (mmax + 1));
+		if (mm < 1)
+			mm = 1;
+		if (mm < m)
+			m = mm;
+		if (precon == SSX_PC_MG)
+			TRY(ssx_mg_precon(c, op, var, eps, r, z));
+		else
+			TRY(ssx_jacobi(c, 0, var, r, z));
+		TRY(ssx_nullspace(c, 0, var, z));
+		for (int k = 0; k < m; ++k) {
+			TRY(dot_to(c, 0, z, ap[k], SC_ZAP0 + k));
+			TRY(ssxk_scalar(st, c->scal, 0, SC_BETA0 + k, SC_ZAP0 + k,
+					SC_PAP0 + k, 0.0));
+		}
+		double *tp = p[m];                 /* p[k] -> p[k+1], k < m */
+		for (int k = m; k > 0; --k)
+			p[k] = p[k - 1];
+		p[0] = tp;
+		TRY(ssxk_copy(st, p[0], z, nt));
+		for (int k = 1; k < m + 1; ++k)
+			TRY(ssxk_axpy_s(st, nt, c->scal, SC_BETA0 + k - 1, -1.0, p[k],
+					p[0]));
+		if (mask)
+			TRY(ssxk_mul(st, nt, p[0], mask, p[0]));
+		int m2 = (int) (j + 1), mm2 = (int) ((j + 1) % (mmax + 1));
+		if (mm2 < 1)
+			mm2 = 1;
+		if (mm2 < m2)
+			m2 = mm2;
+		double *ta = ap[m2 - 1];           /* ap[k] -> ap[k+1], k < m2-1 */
+		for (int k = m2 - 1; k > 0; --k)
+			ap[k] = ap[k - 1];
+		ap[0] = ta;
+		TRY(ssx_opeval(c, 0, op, var, p[0], ap[0]));
+		for (int k = m2 - 1; k > 0; --k)
+			TRY(ssxk_scalar(st, c->scal, 2, SC_PAP0 + k, SC_PAP0 + k - 1, 0,
+					0.0));
+		TRY(dot_to(c, 0, p[0], ap[0], SC_PAP0));
+		TRY(dot_to(c, 0, p[0], r, SC_PR));
+		TRY(ssxk_scalar(st, c->scal, 0, SC_ALPHA, SC_PR, SC_PAP0, 0.0));
+		TRY(ssxk_axpy_s(st, nt, c->scal, SC_ALPHA, 1.0, p[0], u));
+		TRY(ssxk_axpy_s(st, nt, c->scal, SC_ALPHA, -1.0, ap[0], r));
+		TRY(dot_to(c, 0, r, r, SC_RR));
+		double rr;
+		TRY(read_scal(c, SC_RR, &rr));
+		err = sqrt(rr);
+		++it;
+		if (isnan(err) || isinf(err)) {
+			rc = SSX_ENUMERIC;
+			ssx_set_error("FCG: residual is %g", err);
+			break;
+		}
+	} while (err > eps && ++j < max_iter);
+	/* keep the rotated history buffers owned by the context */
+	for (int i = 0; i < 11; ++i)
+		c->fcg_p[i] = p[i];
+	for (int i = 0; i < 10; ++i)
+		c->fcg_ap[i] = ap[i];
+	if (rc == SSX_OK)
+		rc = ssx_synchronize(c);
+	if (stats) {
+		memset(stats, 0, sizeof(*stats));
+		stats->iterations = it;
+		stats->residual = err;
+		stats->converged = err <= eps;
+		stats->fine_ax = c->cnt[0] - c0[0];
+		stats->coarse_iters = c->cnt[1] - c0[1];
+		stats->smooth_iters = c->cnt[2] - c0[2];
+		stats->vcycles = c->cnt[3] - c0[3];
+		stats->seconds = now_s() - t0;
+	}
+	if (rc == SSX_OK && !(err <= eps))
+		rc = SSX_ENOTCONV;
+	return rc;
+}
+
+/* host-array entry points, solver.c:159-189 */
+static int host_solve(ssx_ctx *c, double *f, double *u, int var, double eps,
+		int helm, ssx_stats *stats)
+{
+	const size_t nt = c->lev[0].ntot;
+	if (!c->d_f)
+		TRY(dalloc(&c->d_f, nt));
+	if (!c->d_u)
+		TRY(dalloc(&c->d_u, nt));
+	hipStream_t st = (hipStream_t) c->stream;
+	HIPC(hipMemcpyAsync(c->d_f, f, nt * sizeof(double),
+			hipMemcpyHostToDevice, st));
+	HIPC(hipMemcpyAsync(c->d_u, u, nt * sizeof(double),
+			hipMemcpyHostToDevice, st));
+	int rc = helm ? ssx_cg(c, 0, SSX_OP_HELMHOLTZ, var, c->d_f, c->d_u, eps,
+			5000, stats)
+		: ssx_fcg(c, SSX_OP_POISSON, var, SSX_PC_MG, c->d_f, c->d_u, eps,
+			5000, stats);
+	if (rc != SSX_OK && rc != SSX_ENOTCONV)
+		return rc;
+	HIPC(hipMemcpyAsync(f, c->d_f, nt * sizeof(double),
+			hipMemcpyDeviceToHost, st));
+	HIPC(hipMemcpyAsync(u, c->d_u, nt * sizeof(double),
+			hipMemcpyDeviceToHost, st));
+	HIPC(hipStreamSynchronize(st));
+	return rc;
+}
+
+int ssx_solve_poisson_cg(ssx_ctx *c, double *f, double *u, int var,
+		double eps, ssx_stats *stats)
+{
+	TRY(chk(c, 0));
+	return host_solve(c, f, u, var, eps, 0, stats);
+}
+
+int ssx_solve_helmholtz_cg(ssx_ctx *c, double *f, double *u, int var,
+		double eps, ssx_stats *stats)
+{
+	TRY(chk(c, 0));
+	return host_solve(c, f, u, var, eps, 1, stats);
+}

@@ -1,0 +1,132 @@
+/*
+ * ssx_internal.h -- shared declarations of the ssx host code (C) and the
+ * gfx950 kernel launchers (ssx_kernels.hip).
+ */
+#ifndef SSX_INTERNAL_H
+#define SSX_INTERNAL_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "ssx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSX_NMIN 2
+#define SSX_NMAX 16          /* GLL points per direction, kernels 2..16    */
+#define SSX_LMAX 10          /* reference structs.c:58 new_preconditioner(n,10) */
+#define SSX_NPART 1024       /* blocks of the reduction kernels            */
+#define SSX_NSCAL 64         /* device scalar slots                        */
+
+/* device scalar slots */
+enum {
+	SC_RDZ = 0, SC_RDZ_OLD, SC_PAP, SC_ALPHA, SC_BETA, SC_RR, SC_TMP,
+	SC_NULLC, SC_DONE, SC_PR, SC_ZAP0 /* .. SC_ZAP0+9 */ ,
+	SC_PAP0 = SC_ZAP0 + 10 /* .. SC_PAP0+9 */,
+	SC_BETA0 = SC_PAP0 + 10 /* .. SC_BETA0+9 */,
+	SC_END = SC_BETA0 + 10
+};
+
+/* host-side basis of one level */
+typedef struct {
+	int n;
+	double xi[SSX_NMAX], rho[SSX_NMAX];
+	double d[SSX_NMAX * SSX_NMAX];     /* d[i*n+l] = D_il = l_l'(xi_i)  */
+	double dsvv[SSX_NMAX * SSX_NMAX];  /* same layout                   */
+} ssx_basis;
+
+/* one multigrid level on the device */
+typedef struct {
+	int n, n3;
+	size_t ntot;                 /* nel*n^3 */
+	ssx_basis b;
+	/* device arrays */
+	double *gf;                  /* [e][6][n3] G*w*detJ                 */
+	double *mass;                /* [e][n3] w*detJ                       */
+	double *diag_p, *diag_h;     /* inverse assembled diagonals          */
+	double *scale;               /* 1/mult                               */
+	double *mask;                /* Dirichlet mask                       */
+	double *dmat;                /* [2][n*n]: D, Dsvv, layout [i*n+l]    */
+	double *prol;                /* [nc][n]: l_p^{l+1}(xi_i^l) at [p*n+i] */
+	int *grp[3];                 /* shared-node groups, m = 2,4,8: [m][G] */
+	long long ngrp[3];
+	int *bc;                     /* per element Dirichlet bits           */
+	double nunique;              /* <1,1>_w                              */
+	/* solver scratch */
+	double *r, *z, *p, *ap, *mg_r, *mg_fn, *mg_e;
+	/* host copies kept for ssx_level_data */
+	double *h_x, *h_y, *h_z;
+} ssx_level;
+
+struct ssx_ctx {
+	int nel, nlev, n;
+	int nmg[SSX_LMAX];
+	ssx_level lev[SSX_LMAX];
+	double lambda;
+	int diag_choice;
+	double *inv_mass;            /* device, level 0 */
+	void *stream;                /* hipStream_t in use */
+	void *own_stream;
+	double *scal;                /* device scalars [SSX_NSCAL]           */
+	double *part;                /* device partial sums [8][SSX_NPART]   */
+	double *h_scal;              /* pinned host mirror                   */
+	/* FCG history (level 0) */
+	double *fcg_r, *fcg_z, *fcg_p[11], *fcg_ap[10];
+	/* host staging for the host-array entry points */
+	double *d_f, *d_u;
+	long long cnt[4];
+};
+
+/* ---- host setup (ssx_setup.c) ---- */
+void ssx_basis_init(ssx_basis *b, int n, int svvn, double svva);
+void ssx_interp(const double *xfrom, int nfrom, const double *xto, int nto,
+		double *out);
+/* build the shared-node groups of one element size from the connectivity,
+ * reproducing the reference's add order; returns SSX_OK or SSX_EMESH.
+ * grp[m] receives malloc'ed [mult][G] arrays of local node indices. */
+int ssx_build_groups(const ssx_mesh *m, int n, int *grp[3], long long ng[3]);
+/* host DSS with the groups (same order as the device kernel) */
+void ssx_host_dss(int *grp[3], const long long ng[3], double *u);
+void ssx_set_error(const char *fmt, ...);
+
+/* ---- kernel launchers (ssx_kernels.hip); all return 0 or SSX_EHIP ---- */
+int ssxk_ax(void *stream, int n, int nel, int op, const double *u,
+		double *au, const double *gf, const double *mass,
+		const double *dmat, double lambda, const int *bc, int apply_mask);
+int ssxk_dss(void *stream, int *const grp[3], const long long ng[3],
+		double *u);
+int ssxk_restrict(void *stream, int nf, int nc, int nel, const double *prol,
+		const double *fine, double *coarse);
+int ssxk_prolong(void *stream, int nf, int nc, int nel, const double *prol,
+		const double *coarse, double *fine, int accumulate);
+/* y[i] = a*x[i]*w[i] + b*y[i]  (w may be NULL = 1) */
+int ssxk_axpby(void *stream, size_t n, double a, const double *x,
+		const double *w, double b, double *y);
+/* z = x * w */
+int ssxk_mul(void *stream, size_t n, const double *x, const double *w,
+		double *z);
+/* per-block partials of sum(x*s*y) into part[slot*NPART + blk] */
+int ssxk_dot(void *stream, size_t n, const double *x, const double *s,
+		const double *y, double *part, int slot);
+/* scal[dst] = sum of part[slot*NPART ...] (fixed order) */
+int ssxk_reduce(void *stream, double *part, int slot, double *scal, int dst);
+/* scalar program: see ssx_kernels.hip */
+int ssxk_scalar(void *stream, double *scal, int op, int a, int b, int c,
+		double v);
+/* y += scal[ia]*sgn*x  */
+int ssxk_axpy_s(void *stream, size_t n, const double *scal, int ia,
+		double sgn, const double *x, double *y);
+/* y = x + scal[ib]*y, then mask (mask may be NULL) */
+int ssxk_xpby_s(void *stream, size_t n, const double *x, const double *scal,
+		int ib, double *y, const double *mask);
+/* y += c*1 with c = -scal[ia]/nunique */
+int ssxk_shift_s(void *stream, size_t n, const double *scal, int ia,
+		double nunique, double *y);
+int ssxk_copy(void *stream, double *dst, const double *src, size_t n);
+int ssxk_fill(void *stream, double *dst, double v, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

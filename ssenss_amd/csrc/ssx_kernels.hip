@@ -1,0 +1,565 @@
+// ssx_kernels.hip -- gfx950 kernels of the ssx elliptic-solve engine.
+//
+// Layout: E-vectors u[e][k][j][i] (i fastest), FP64.  Geometric factors are
+// premultiplied G~_ab = G_ab * w * detJ, stored per element as six
+// contiguous n^3 blocks [e][xx,xy,xz,yy,yz,zz][node] so that every factor
+// streams from HBM in fully coalesced runs.
+//
+// Kernels:
+//   k_ax<N,OP>      Poisson / Helmholtz operator, one "team" of N*N lanes per
+//                   element (lane = (i,j) column, loop over the k planes),
+//                   planes of u and of the two in-plane fluxes staged in LDS,
+//                   the zeta contraction kept in registers.  HBM-bound:
+//                   64 B per local node (Poisson), 72 B (Helmholtz).
+//   k_dss<M>        direct stiffness summation over shared-node groups of M
+//                   copies; each copy is replaced by own + partners in the
+//                   reference's add order, so the result is bit-identical to
+//                   the reference's pointer-wired DSS.  No atomics.
+//   k_restrict / k_prolong   tensor-product order transfer per element.
+//   k_dot / k_reduce         deterministic two-pass weighted inner product.
+//   vector kernels           axpy family with device-resident scalars, so a
+//                            CG iteration never waits on the host.
+#include <hip/hip_runtime.h>
+#include "ssx_internal.h"
+
+#define LAUNCH_CHECK(name)                                                   \
+	do {                                                                 \
+		hipError_t e_ = hipGetLastError();                           \
+		if (e_ != hipSuccess) {                                      \
+			ssx_set_error("%s: %s", name, hipGetErrorString(e_)); \
+			return SSX_EHIP;                                     \
+		}                                                            \
+	} while (0)
+
+// ---------------------------------------------------------------------
+// Operator
+// ---------------------------------------------------------------------
+
+template <int N>
+struct AxShape {
+	static constexpr int N2 = N * N;
+	static constexpr int N3 = N * N * N;
+	static constexpr int P = N + 1;   // padded LDS row: conflict-free rows
+	static constexpr int TEAMS = (256 / N2) > 0 ? (256 / N2) : 1;
+	static constexpr int BLOCK = TEAMS * N2;
+};
+
+template <int N, int OP>
+__global__ void __launch_bounds__(AxShape<N>::BLOCK)
+k_ax(int nel, const double *__restrict__ u, double *__restrict__ au,
+		const double *__restrict__ gf, const double *__restrict__ mass,
+		const double *__restrict__ dmat, double lambda,
+		const int *__restrict__ bc, int apply_mask)
+{
+	using S = AxShape<N>;
+	constexpr int N2 = S::N2, N3 = S::N3, P = S::P, TEAMS = S::TEAMS;
+	__shared__ double s_d[2 * N2];
+	__shared__ double s_u[TEAMS][N * N * P];
+	__shared__ double s_a[TEAMS][N * P];
+	__shared__ double s_b[TEAMS][N * P];
+
+	const int team = threadIdx.x / N2;
+	const int tid = threadIdx.x - team * N2;
+	const int i = tid % N, j = tid / N;
+	const int e = blockIdx.x * TEAMS + team;
+	const bool act = e < nel;
+
+	for (int q = threadIdx.x; q < 2 * N2; q += blockDim.x)
+		s_d[q] = dmat[q];
+	// D_il at [i*N+l]; the SVV matrix enters the xi-forward and the
+	// eta/zeta-transpose products only (helmholtz.c:41-44, deriv.c:29-66)
+	const double *gDn = dmat;
+	const double *gDs = dmat + (OP == 1 ? N2 : 0);
+
+	const size_t eoff = (size_t) (act ? e : 0) * N3;
+	double *su = s_u[team], *sa = s_a[team], *sb = s_b[team];
+	double uc[N], acc[N];
+#pragma unroll
+	for (int k = 0; k < N; ++k) {
+		uc[k] = act ? u[eoff + k * N2 + tid] : 0.0;
+		su[(k * N + j) * P + i] = uc[k];
+		acc[k] = 0.0;
+	}
+	__syncthreads();
+	const double *Dn = s_d;
+	const double *Ds = s_d + (OP == 1 ? N2 : 0);
+	double dxf[N], dyf[N], dxt[N], dyt[N];
+#pragma unroll
+	for (int l = 0; l < N; ++l) {
+		dxf[l] = Ds[i * N + l];
+		dyf[l] = Dn[j * N + l];
+		dxt[l] = Dn[l * N + i];
+		dyt[l] = Ds[l * N + j];
+	}
+	const double *g = gf + (size_t) (act ? e : 0) * 6 * N3;
+
+#pragma unroll 1
+	for (int k = 0; k < N; ++k) {
+		double gx = 0.0, gy = 0.0, gz = 0.0;
+		const double *plane = su + k * N * P;
+#pragma unroll
+		for (int l = 0; l < N; ++l) {
+			gx += dxf[l] * plane[j * P + l];
+			gy += dyf[l] * plane[l * P + i];
+			gz += gDn[k * N + l] * uc[l];      // uniform: scalar loads
+		}
+		double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+		if (act) {
+			const int q = k * N2 + tid;
+			const double gxx = g[q], gxy = g[N3 + q], gxz = g[2 * N3 + q];
+			const double gyy = g[3 * N3 + q], gyz = g[4 * N3 + q];
+			const double gzz = g[5 * N3 + q];
+			t0 = gxx * gx + gxy * gy + gxz * gz;
+			t1 = gxy * gx + gyy * gy + gyz * gz;
+			t2 = gxz * gx + gyz * gy + gzz * gz;
+		}
+		sa[j * P + i] = t0;
+		sb[j * P + i] = t1;
+		__syncthreads();
+		double r = 0.0;
+#pragma unroll
+		for (int l = 0; l < N; ++l)
+			r += dxt[l] * sa[j * P + l] + dyt[l] * sb[l * P + i];
+#pragma unroll
+		for (int kk = 0; kk < N; ++kk)
+			acc[kk] += (kk == k ? r : 0.0) + gDs[k * N + kk] * t2;
+		__syncthreads();
+	}
+	if (!act)
+		return;
+	int bits = apply_mask ? bc[e] : 0;
+	const bool mi = ((bits & 1) && i == 0) || ((bits & 2) && i == N - 1)
+		|| ((bits & 4) && j == 0) || ((bits & 8) && j == N - 1);
+#pragma unroll
+	for (int k = 0; k < N; ++k) {
+		const int q = k * N2 + tid;
+		double v = acc[k];
+		if (OP == 1)
+			v += lambda * (mass[eoff + q] * uc[k]);
+		const bool m = mi || ((bits & 16) && k == 0)
+			|| ((bits & 32) && k == N - 1);
+		au[eoff + q] = m ? 0.0 : v;
+	}
+}
+
+template <int N>
+static int launch_ax(hipStream_t st, int nel, int op, const double *u,
+		double *au, const double *gf, const double *mass,
+		const double *dmat, double lambda, const int *bc, int apply_mask)
+{
+	using S = AxShape<N>;
+	const int grid = (nel + S::TEAMS - 1) / S::TEAMS;
+	if (op == SSX_OP_HELMHOLTZ)
+		hipLaunchKernelGGL((k_ax<N, 1>), dim3(grid), dim3(S::BLOCK), 0, st,
+				nel, u, au, gf, mass, dmat, lambda, bc, apply_mask);
+	else
+		hipLaunchKernelGGL((k_ax<N, 0>), dim3(grid), dim3(S::BLOCK), 0, st,
+				nel, u, au, gf, mass, dmat, lambda, bc, apply_mask);
+	LAUNCH_CHECK("k_ax");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_ax(void *stream, int n, int nel, int op, const double *u,
+		double *au, const double *gf, const double *mass,
+		const double *dmat, double lambda, const int *bc, int apply_mask)
+{
+	hipStream_t st = (hipStream_t) stream;
+	if (nel <= 0)
+		return SSX_OK;
+	switch (n) {
+#define CASE(NN) case NN: return launch_ax<NN>(st, nel, op, u, au, gf, mass, \
+			dmat, lambda, bc, apply_mask);
+	CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9)
+	CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+	default:
+		ssx_set_error("ssxk_ax: n=%d unsupported", n);
+		return SSX_EINVAL;
+	}
+}
+
+// ---------------------------------------------------------------------
+// Direct stiffness summation over shared-node groups
+// ---------------------------------------------------------------------
+
+// partner order of copy t: t ^ ord(k), k = 1..M-1 (faces x,y,z, then edges
+// along z, y, x, then the corner; global.c:252-256)
+__device__ __forceinline__ int dss_ord(int M, int k)
+{
+	return M == 8 ? ((k == 3) ? 4 : (k == 4) ? 3 : k) : k;
+}
+
+template <int M>
+__global__ void __launch_bounds__(256)
+k_dss(long long G, const int *__restrict__ idx, double *__restrict__ u)
+{
+	const long long g = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= G)
+		return;
+	int ix[M];
+	double v[M];
+#pragma unroll
+	for (int t = 0; t < M; ++t)
+		ix[t] = idx[(long long) t * G + g];
+#pragma unroll
+	for (int t = 0; t < M; ++t)
+		v[t] = u[ix[t]];
+#pragma unroll
+	for (int t = 0; t < M; ++t) {
+		double s = v[t];
+#pragma unroll
+		for (int k = 1; k < M; ++k)
+			s = s + v[t ^ dss_ord(M, k)];
+		u[ix[t]] = s;
+	}
+}
+
+extern "C" int ssxk_dss(void *stream, int *const grp[3], const long long ng[3],
+		double *u)
+{
+	hipStream_t st = (hipStream_t) stream;
+	if (ng[0] > 0) {
+		hipLaunchKernelGGL((k_dss<2>), dim3((ng[0] + 255) / 256), dim3(256),
+				0, st, ng[0], grp[0], u);
+		LAUNCH_CHECK("k_dss<2>");
+	}
+	if (ng[1] > 0) {
+		hipLaunchKernelGGL((k_dss<4>), dim3((ng[1] + 255) / 256), dim3(256),
+				0, st, ng[1], grp[1], u);
+		LAUNCH_CHECK("k_dss<4>");
+	}
+	if (ng[2] > 0) {
+		hipLaunchKernelGGL((k_dss<8>), dim3((ng[2] + 255) / 256), dim3(256),
+				0, st, ng[2], grp[2], u);
+		LAUNCH_CHECK("k_dss<8>");
+	}
+	return SSX_OK;
+}
+
+// ---------------------------------------------------------------------
+// Order transfer (interpolation.c:136-176), one element per block
+// prol[p*nf+i] = l_p^{coarse}(xi_i^{fine})
+// ---------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256)
+k_restrict(int nf, int nc, const double *__restrict__ prol,
+		const double *__restrict__ fine, double *__restrict__ coarse)
+{
+	extern __shared__ double sm[];
+	const int nf3 = nf * nf * nf, nc3 = nc * nc * nc;
+	double *sp = sm;                       // nc*nf
+	double *su = sp + nc * nf;             // nf^3
+	double *t1 = su + nf3;                 // nf*nf*nc
+	double *t2 = t1 + nf * nf * nc;        // nf*nc*nc
+	const int e = blockIdx.x;
+	for (int q = threadIdx.x; q < nc * nf; q += blockDim.x)
+		sp[q] = prol[q];
+	for (int q = threadIdx.x; q < nf3; q += blockDim.x)
+		su[q] = fine[(size_t) e * nf3 + q];
+	__syncthreads();
+	for (int q = threadIdx.x; q < nf * nf * nc; q += blockDim.x) {
+		const int p = q % nc, kj = q / nc;
+		double s = 0.0;
+		for (int c = 0; c < nf; ++c)
+			s += sp[p * nf + c] * su[kj * nf + c];
+		t1[q] = s;
+	}
+	__syncthreads();
+	for (int q = threadIdx.x; q < nf * nc * nc; q += blockDim.x) {
+		const int i = q % nc, jq = (q / nc) % nc, k = q / (nc * nc);
+		double s = 0.0;
+		for (int l = 0; l < nf; ++l)
+			s += t1[(k * nf + l) * nc + i] * sp[jq * nf + l];
+		t2[q] = s;
+	}
+	__syncthreads();
+	for (int q = threadIdx.x; q < nc3; q += blockDim.x) {
+		const int ji = q % (nc * nc), k = q / (nc * nc);
+		double s = 0.0;
+		for (int l = 0; l < nf; ++l)
+			s += t2[l * nc * nc + ji] * sp[k * nf + l];
+		coarse[(size_t) e * nc3 + q] = s;
+	}
+}
+
+__global__ void __launch_bounds__(256)
+k_prolong(int nf, int nc, const double *__restrict__ prol,
+		const double *__restrict__ coarse, double *__restrict__ fine,
+		int accumulate)
+{
+	extern __shared__ double sm[];
+	const int nf3 = nf * nf * nf, nc3 = nc * nc * nc;
+	double *sp = sm;                       // nc*nf
+	double *su = sp + nc * nf;             // nc^3
+	double *t1 = su + nc3;                 // nc*nc*nf
+	double *t2 = t1 + nc * nc * nf;        // nc*nf*nf
+	const int e = blockIdx.x;
+	for (int q = threadIdx.x; q < nc * nf; q += blockDim.x)
+		sp[q] = prol[q];
+	for (int q = threadIdx.x; q < nc3; q += blockDim.x)
+		su[q] = coarse[(size_t) e * nc3 + q];
+	__syncthreads();
+	for (int q = threadIdx.x; q < nc * nc * nf; q += blockDim.x) {
+		const int i = q % nf, kj = q / nf;
+		double s = 0.0;
+		for (int p = 0; p < nc; ++p)
+			s += sp[p * nf + i] * su[kj * nc + p];
+		t1[q] = s;
+	}
+	__syncthreads();
+	for (int q = threadIdx.x; q < nc * nf * nf; q += blockDim.x) {
+		const int i = q % nf, jq = (q / nf) % nf, k = q / (nf * nf);
+		double s = 0.0;
+		for (int l = 0; l < nc; ++l)
+			s += t1[(k * nc + l) * nf + i] * sp[l * nf + jq];
+		t2[q] = s;
+	}
+	__syncthreads();
+	for (int q = threadIdx.x; q < nf3; q += blockDim.x) {
+		const int ji = q % (nf * nf), k = q / (nf * nf);
+		double s = 0.0;
+		for (int l = 0; l < nc; ++l)
+			s += t2[l * nf * nf + ji] * sp[l * nf + k];
+		const size_t o = (size_t) e * nf3 + q;
+		fine[o] = accumulate ? s + fine[o] : s;
+	}
+}
+
+extern "C" int ssxk_restrict(void *stream, int nf, int nc, int nel,
+		const double *prol, const double *fine, double *coarse)
+{
+	const size_t sh = sizeof(double) * ((size_t) nc * nf + (size_t) nf * nf * nf
+			+ (size_t) nf * nf * nc + (size_t) nf * nc * nc);
+	hipLaunchKernelGGL(k_restrict, dim3(nel), dim3(256), sh,
+			(hipStream_t) stream, nf, nc, prol, fine, coarse);
+	LAUNCH_CHECK("k_restrict");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_prolong(void *stream, int nf, int nc, int nel,
+		const double *prol, const double *coarse, double *fine,
+		int accumulate)
+{
+	const size_t sh = sizeof(double) * ((size_t) nc * nf + (size_t) nc * nc * nc
+			+ (size_t) nc * nc * nf + (size_t) nc * nf * nf);
+	hipLaunchKernelGGL(k_prolong, dim3(nel), dim3(256), sh,
+			(hipStream_t) stream, nf, nc, prol, coarse, fine, accumulate);
+	LAUNCH_CHECK("k_prolong");
+	return SSX_OK;
+}
+
+// ---------------------------------------------------------------------
+// Reductions: fixed grid of SSX_NPART blocks, fixed tree -> deterministic
+// ---------------------------------------------------------------------
+
+__device__ __forceinline__ double block_sum256(double v, double *sh)
+{
+	sh[threadIdx.x] = v;
+	__syncthreads();
+#pragma unroll
+	for (int s = 128; s > 0; s >>= 1) {
+		if (threadIdx.x < s)
+			sh[threadIdx.x] += sh[threadIdx.x + s];
+		__syncthreads();
+	}
+	return sh[0];
+}
+
+__global__ void __launch_bounds__(256)
+k_dot(size_t n, const double *__restrict__ x, const double *__restrict__ s,
+		const double *__restrict__ y, double *__restrict__ part)
+{
+	__shared__ double sh[256];
+	double acc = 0.0;
+	const size_t stride = (size_t) gridDim.x * blockDim.x;
+	for (size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x; q < n;
+			q += stride) {
+		double a = x[q] * s[q];
+		acc += y ? a * y[q] : a;
+	}
+	double t = block_sum256(acc, sh);
+	if (threadIdx.x == 0)
+		part[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(256)
+k_reduce(const double *__restrict__ part, double *__restrict__ scal, int dst)
+{
+	__shared__ double sh[256];
+	double acc = 0.0;
+	for (int q = threadIdx.x; q < SSX_NPART; q += 256)
+		acc += part[q];
+	double t = block_sum256(acc, sh);
+	if (threadIdx.x == 0)
+		scal[dst] = t;
+}
+
+extern "C" int ssxk_dot(void *stream, size_t n, const double *x,
+		const double *s, const double *y, double *part, int slot)
+{
+	hipLaunchKernelGGL(k_dot, dim3(SSX_NPART), dim3(256), 0,
+			(hipStream_t) stream, n, x, s, y, part + (size_t) slot * SSX_NPART);
+	LAUNCH_CHECK("k_dot");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_reduce(void *stream, double *part, int slot, double *scal,
+		int dst)
+{
+	hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, (hipStream_t) stream,
+			part + (size_t) slot * SSX_NPART, scal, dst);
+	LAUNCH_CHECK("k_reduce");
+	return SSX_OK;
+}
+
+// scalar program, one lane:
+//   op 0: s[a] = s[b] / s[c]        op 1: s[a] = v
+//   op 2: s[a] = s[b]               op 3: s[a] = -s[b]
+__global__ void k_scalar(double *scal, int op, int a, int b, int c, double v)
+{
+	if (threadIdx.x != 0)
+		return;
+	switch (op) {
+	case 0: scal[a] = scal[b] / scal[c]; break;
+	case 1: scal[a] = v; break;
+	case 2: scal[a] = scal[b]; break;
+	case 3: scal[a] = -scal[b]; break;
+	}
+}
+
+extern "C" int ssxk_scalar(void *stream, double *scal, int op, int a, int b,
+		int c, double v)
+{
+	hipLaunchKernelGGL(k_scalar, dim3(1), dim3(64), 0, (hipStream_t) stream,
+			scal, op, a, b, c, v);
+	LAUNCH_CHECK("k_scalar");
+	return SSX_OK;
+}
+
+// ---------------------------------------------------------------------
+// Vector kernels
+// ---------------------------------------------------------------------
+
+static inline dim3 vgrid(size_t n)
+{
+	size_t b = (n + 255) / 256;
+	if (b > 16384)
+		b = 16384;
+	if (b == 0)
+		b = 1;
+	return dim3((unsigned) b);
+}
+
+#define GRID_STRIDE(q, n) \
+	for (size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x; q < (n); \
+			q += (size_t) gridDim.x * blockDim.x)
+
+__global__ void k_axpby(size_t n, double a, const double *__restrict__ x,
+		const double *__restrict__ w, double b, double *__restrict__ y)
+{
+	GRID_STRIDE(q, n) {
+		double xv = w ? x[q] * w[q] : x[q];
+		y[q] = a * xv + b * y[q];
+	}
+}
+
+extern "C" int ssxk_axpby(void *stream, size_t n, double a, const double *x,
+		const double *w, double b, double *y)
+{
+	hipLaunchKernelGGL(k_axpby, vgrid(n), dim3(256), 0, (hipStream_t) stream,
+			n, a, x, w, b, y);
+	LAUNCH_CHECK("k_axpby");
+	return SSX_OK;
+}
+
+__global__ void k_mul(size_t n, const double *__restrict__ x,
+		const double *__restrict__ w, double *__restrict__ z)
+{
+	GRID_STRIDE(q, n) z[q] = x[q] * w[q];
+}
+
+extern "C" int ssxk_mul(void *stream, size_t n, const double *x,
+		const double *w, double *z)
+{
+	hipLaunchKernelGGL(k_mul, vgrid(n), dim3(256), 0, (hipStream_t) stream,
+			n, x, w, z);
+	LAUNCH_CHECK("k_mul");
+	return SSX_OK;
+}
+
+__global__ void k_axpy_s(size_t n, const double *__restrict__ scal, int ia,
+		double sgn, const double *__restrict__ x, double *__restrict__ y)
+{
+	const double a = sgn * scal[ia];
+	GRID_STRIDE(q, n) y[q] = a * x[q] + y[q];
+}
+
+extern "C" int ssxk_axpy_s(void *stream, size_t n, const double *scal,
+		int ia, double sgn, const double *x, double *y)
+{
+	hipLaunchKernelGGL(k_axpy_s, vgrid(n), dim3(256), 0,
+			(hipStream_t) stream, n, scal, ia, sgn, x, y);
+	LAUNCH_CHECK("k_axpy_s");
+	return SSX_OK;
+}
+
+__global__ void k_xpby_s(size_t n, const double *__restrict__ x,
+		const double *__restrict__ scal, int ib, double *__restrict__ y,
+		const double *__restrict__ mask)
+{
+	const double b = scal[ib];
+	GRID_STRIDE(q, n) {
+		double v = x[q] + b * y[q];
+		y[q] = mask ? v * mask[q] : v;
+	}
+}
+
+extern "C" int ssxk_xpby_s(void *stream, size_t n, const double *x,
+		const double *scal, int ib, double *y, const double *mask)
+{
+	hipLaunchKernelGGL(k_xpby_s, vgrid(n), dim3(256), 0,
+			(hipStream_t) stream, n, x, scal, ib, y, mask);
+	LAUNCH_CHECK("k_xpby_s");
+	return SSX_OK;
+}
+
+__global__ void k_shift_s(size_t n, const double *__restrict__ scal, int ia,
+		double nunique, double *__restrict__ y)
+{
+	const double c = -scal[ia] / nunique;
+	GRID_STRIDE(q, n) y[q] = c * 1.0 + y[q];
+}
+
+extern "C" int ssxk_shift_s(void *stream, size_t n, const double *scal,
+		int ia, double nunique, double *y)
+{
+	hipLaunchKernelGGL(k_shift_s, vgrid(n), dim3(256), 0,
+			(hipStream_t) stream, n, scal, ia, nunique, y);
+	LAUNCH_CHECK("k_shift_s");
+	return SSX_OK;
+}
+
+__global__ void k_fill(size_t n, double v, double *__restrict__ y)
+{
+	GRID_STRIDE(q, n) y[q] = v;
+}
+
+extern "C" int ssxk_fill(void *stream, double *dst, double v, size_t n)
+{
+	hipLaunchKernelGGL(k_fill, vgrid(n), dim3(256), 0, (hipStream_t) stream,
+			n, v, dst);
+	LAUNCH_CHECK("k_fill");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_copy(void *stream, double *dst, const double *src,
+		size_t n)
+{
+	hipError_t e = hipMemcpyAsync(dst, src, n * sizeof(double),
+			hipMemcpyDeviceToDevice, (hipStream_t) stream);
+	if (e != hipSuccess) {
+		ssx_set_error("copy: %s", hipGetErrorString(e));
+		return SSX_EHIP;
+	}
+	return SSX_OK;
+}

@@ -1,0 +1,154 @@
+"""Python handle on an ssx context.  Device vectors are torch tensors on the
+GPU (torch provides the memory and the stream, nothing else); all numerics
+run in libssx's HIP kernels."""
+import ctypes as C
+
+import numpy as np
+
+from .native import Stats, check, load
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    return C.c_void_p(t.data_ptr())
+
+
+class Context:
+    def __init__(self, mesh, stream=None):
+        self._lib = load()
+        self.mesh = mesh
+        self._ms = mesh.struct()
+        h = C.c_void_p()
+        check(self._lib.ssx_create(C.byref(h), C.byref(self._ms)))
+        self.h = h
+        if stream is not None:
+            self.set_stream(stream)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.ssx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- info ---
+    @property
+    def nlevels(self):
+        return self._lib.ssx_nlevels(self.h)
+
+    def level_n(self, level):
+        return self._lib.ssx_level_n(self.h, level)
+
+    def ntot(self, level=0):
+        return self._lib.ssx_level_ntot(self.h, level)
+
+    @property
+    def lam(self):
+        return self._lib.ssx_lambda(self.h)
+
+    def set_stream(self, stream):
+        """stream: torch.cuda.Stream (or None for the context's own)"""
+        s = None if stream is None else C.c_void_p(stream.cuda_stream)
+        check(self._lib.ssx_set_stream(self.h, s))
+
+    def stream_handle(self):
+        return self._lib.ssx_get_stream(self.h)
+
+    def synchronize(self):
+        check(self._lib.ssx_synchronize(self.h))
+
+    def level_data(self, level, which):
+        n = self.ntot(level) * (6 if which == 0 else 1)
+        out = np.zeros(n)
+        check(self._lib.ssx_level_data(
+            self.h, level, which, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def dss_groups(self, level=0):
+        a = (C.c_longlong * 3)()
+        check(self._lib.ssx_dss_groups(self.h, level, a))
+        return tuple(a)
+
+    def set_diag_choice(self, choice):
+        check(self._lib.ssx_set_diag_choice(self.h, choice))
+
+    # --- operators (torch device tensors) ---
+    def ax(self, level, op, u, au):
+        check(self._lib.ssx_ax(self.h, level, op, _ptr(u), _ptr(au)))
+
+    def opeval(self, level, op, var, p, ap):
+        check(self._lib.ssx_opeval(self.h, level, op, var, _ptr(p), _ptr(ap)))
+
+    def dss(self, level, u):
+        check(self._lib.ssx_dss(self.h, level, _ptr(u)))
+
+    def dss_average(self, level, u):
+        check(self._lib.ssx_dss_average(self.h, level, _ptr(u)))
+
+    def dot(self, level, x, y):
+        out = C.c_double()
+        check(self._lib.ssx_dot(self.h, level, _ptr(x), _ptr(y),
+                                C.byref(out)))
+        return out.value
+
+    def mask(self, level, var, u):
+        check(self._lib.ssx_mask(self.h, level, var, _ptr(u)))
+
+    def nullspace(self, level, var, u):
+        check(self._lib.ssx_nullspace(self.h, level, var, _ptr(u)))
+
+    def restrict(self, level, fine, coarse):
+        check(self._lib.ssx_restrict(self.h, level, _ptr(fine), _ptr(coarse)))
+
+    def prolong(self, level, coarse, fine):
+        check(self._lib.ssx_prolong(self.h, level, _ptr(coarse), _ptr(fine)))
+
+    def jacobi(self, level, var, x, px):
+        check(self._lib.ssx_jacobi(self.h, level, var, _ptr(x), _ptr(px)))
+
+    def mg_precon(self, op, var, eps, x, px):
+        check(self._lib.ssx_mg_precon(self.h, op, var, eps, _ptr(x), _ptr(px)))
+
+    def cg(self, level, op, var, f, u, eps, max_iter, allow_notconv=False):
+        st = Stats()
+        rc = self._lib.ssx_cg(self.h, level, op, var, _ptr(f), _ptr(u), eps,
+                              max_iter, C.byref(st))
+        if not (allow_notconv and rc == 4):
+            check(rc)
+        return st
+
+    def fcg(self, op, var, precon, f, u, eps, max_iter, allow_notconv=False):
+        st = Stats()
+        rc = self._lib.ssx_fcg(self.h, op, var, precon, _ptr(f), _ptr(u), eps,
+                               max_iter, C.byref(st))
+        if not (allow_notconv and rc == 4):
+            check(rc)
+        return st
+
+    # --- host-array entry points (reference solve_*_cg) ---
+    def solve_poisson_cg(self, f, u, var, eps):
+        st = Stats()
+        dp = C.POINTER(C.c_double)
+        check(self._lib.ssx_solve_poisson_cg(self.h, f.ctypes.data_as(dp),
+                                             u.ctypes.data_as(dp), var, eps,
+                                             C.byref(st)))
+        return st
+
+    def solve_helmholtz_cg(self, f, u, var, eps):
+        st = Stats()
+        dp = C.POINTER(C.c_double)
+        check(self._lib.ssx_solve_helmholtz_cg(self.h, f.ctypes.data_as(dp),
+                                               u.ctypes.data_as(dp), var, eps,
+                                               C.byref(st)))
+        return st
