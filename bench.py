@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Benchmark: FP64 Poisson Ax GDOF/s and p-multigrid PCG time-to-residual on
+one MI355X, on the deformed-cube workload (32^3 hexahedra at polynomial
+order 7, i.e. 8 GLL points per direction, 16.8M local nodes, Dirichlet
+walls, x' = x + 0.1 sin(pi x) sin(pi y) sin(pi z) on each coordinate,
+seeded 16-mode Fourier source, p-MG orders 7->3->1, tolerance 1e-10
+relative in the multiplicity-weighted norm).
+
+A "step" is one application of the FP64 Poisson operator (ssx_ax) to the
+whole mesh; `value` is local DOFs per second over the timed steps, inputs
+resident in HBM.  The full FCG + p-MG solve is timed once after the steps
+and reported in the same JSON line.  HIP events are recorded on the stream
+the kernels run on (the library is bound to torch's current stream).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+BYTES_PER_NODE_POISSON = 64      # u 8 + Au 8 + six factors G*w*detJ 48
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles",
+                               "ax_poisson_n8_cube32_traffic.txt")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic():
+    """HBM bytes per launch of the Poisson Ax kernel measured with
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (calibrated), if profiled."""
+    try:
+        for line in open(PROFILE_TRAFFIC):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def smooth_source(x, y, z, seed=2024):
+    rng = np.random.default_rng(seed)
+    k = rng.integers(1, 9, size=(16, 3))
+    a = rng.standard_normal(16)
+    f = np.zeros_like(x)
+    for m in range(16):
+        f += a[m] * (np.sin(np.pi * k[m, 0] * x) * np.sin(np.pi * k[m, 1] * y)
+                     * np.sin(np.pi * k[m, 2] * z))
+    return f
+
+
+def cpu_baseline(mesh, seconds):
+    """The oracle (C restatement of the reference CPU path) on a bounded
+    sample: Poisson Ax on a 12^3-element piece of the same deformed cube
+    at the same order, repeated for about `seconds` of wall time."""
+    from oracle import pyoracle as P
+    from ssenss_amd.mesh import box_mesh
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    P.set_threads(threads)
+    sm = box_mesh(mesh.n, (12, 12, 12), deform=1, amp=0.1)
+    o = P.Oracle(sm)
+    u = np.random.default_rng(1).standard_normal(sm.ntot)
+    o.ax(0, P.OP_POISSON, u)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        o.ax(0, P.OP_POISSON, u)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    o.close()
+    return {"value": reps * sm.ntot / el / 1e9, "unit": "GDOF/s",
+            "cores": P.get_threads(), "kind": "port",
+            "sample": f"FP64 Poisson Ax, 12^3 elements at n={mesh.n} "
+                      f"({sm.ntot} local nodes) x {reps} applications, "
+                      f"{el:.1f} s, OpenMP over elements"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--elems", type=int, default=32)
+    ap.add_argument("--n", type=int, default=8)
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--max-iter", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this "
+              "engine runs all elements on a single MI355X and has no "
+              "multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import ssenss_amd as S
+    from ssenss_amd.mesh import box_mesh
+
+    m = args.elems
+    log(f"[bench] building {m}^3 mesh n={args.n}")
+    mesh = box_mesh(args.n, (m, m, m), deform=1, amp=0.1)
+    t0 = time.perf_counter()
+    ctx = S.Context(mesh)
+    stream = torch.cuda.current_stream()
+    ctx.set_stream(stream)
+    log(f"[bench] context set-up {time.perf_counter() - t0:.1f} s")
+    nt = mesh.ntot
+    rng = np.random.default_rng(0)
+    u = torch.from_numpy(rng.standard_normal(nt)).cuda()
+    au = torch.zeros_like(u)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / steps
+
+    t_ax = timed(lambda: ctx.ax(0, S.OP_POISSON, u, au), args.steps,
+                 args.warmup)
+    log(f"[bench] Ax {t_ax * 1e6:.1f} us")
+    t_helm = timed(lambda: ctx.ax(0, S.OP_HELMHOLTZ, u, au), 20, 3)
+    t_dss = timed(lambda: ctx.dss(0, au), 20, 3)
+    t_op = timed(lambda: ctx.opeval(0, S.OP_POISSON, S.X_VELOCITY, u, au),
+                 20, 3)
+    log(f"[bench] helmholtz {t_helm*1e6:.1f} us dss {t_dss*1e6:.1f} us "
+        f"opeval {t_op*1e6:.1f} us")
+
+    achieved = BYTES_PER_NODE_POISSON * nt / t_ax / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                "traffic": read_traffic(),
+                "kernel": f"k_ax<{args.n},0> (FP64 Poisson Ax)",
+                "bytes_per_launch": BYTES_PER_NODE_POISSON * nt}
+
+    solve = None
+    if not args.no_solve:
+        x = torch.from_numpy(mesh.x.ravel()).cuda()
+        f = smooth_source(mesh.x.ravel(), mesh.y.ravel(), mesh.z.ravel())
+        massv = torch.from_numpy(ctx.level_data(0, S.DATA_MASS)).cuda()
+        b = torch.from_numpy(f).cuda() * massv
+        ctx.dss(0, b)
+        ctx.mask(0, S.X_VELOCITY, b)
+        bnorm = np.sqrt(ctx.dot(0, b, b))
+        eps = 1e-10 * bnorm
+        sol = torch.zeros_like(b)
+        ctx.set_diag_choice(0)
+        torch.cuda.synchronize()
+        log("[bench] FCG + p-MG solve")
+        t0 = time.perf_counter()
+        st = ctx.fcg(S.OP_POISSON, S.X_VELOCITY, S.PC_MG, b, sol, eps,
+                     args.max_iter, allow_notconv=True)
+        torch.cuda.synchronize()
+        tsolve = time.perf_counter() - t0
+        solve = {"time_to_residual_s": tsolve, "iterations": st.iterations,
+                 "converged": bool(st.converged),
+                 "final_residual_rel": st.residual / bnorm,
+                 "tolerance_rel": 1e-10, "precision": "fp64",
+                 "fine_ax": st.fine_ax, "coarse_cg_iters": st.coarse_iters,
+                 "smoother_cg_iters": st.smooth_iters, "vcycles": st.vcycles}
+        log(f"[bench] solve {tsolve:.2f} s, {st.iterations} its")
+        del x
+
+    cpu = None if args.no_cpu else cpu_baseline(mesh, args.cpu_seconds)
+    out = {
+        "metric": "Ax GDOF/s + p-MG PCG time to 1e-10 residual, FP64 vs "
+                  "mixed, 1 MI355X",
+        "value": nt / t_ax / 1e9,
+        "unit": "GDOF/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_ax * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"deformed cube {m}^3 hexahedra, N={args.n-1}"
+                               f" ({args.n}^3 GLL nodes/element), Dirichlet "
+                               "Poisson, FP64 operator + FCG/p-MG solve",
+                   "elements": m ** 3, "points_per_dir": args.n,
+                   "local_dofs": nt, "unique_dofs": (m * (args.n - 1) + 1) ** 3,
+                   "mg_orders": [ctx.level_n(l) - 1 for l in
+                                 range(ctx.nlevels)]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "helmholtz_ax_gdofs": nt / t_helm / 1e9,
+        "dss_us": t_dss * 1e6,
+        "opeval_gdofs": nt / t_op / 1e9,
+        "solve_fp64": solve,
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

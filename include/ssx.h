@@ -125,6 +125,16 @@ int ssx_mesh_box(int n, int mx, int my, int mz, const double lo[3],
 		double *x, double *y, double *z, int *neigh, int *edge_neigh,
 		int *edge_size, int *corn_neigh, int *corn_size);
 
+/* ---- host-side assembly helpers (no device needed) ----
+ * ssx_dss_host: direct stiffness summation of a HOST E-vector whose elements
+ * have n points per direction (any level), with the shared-node groups and
+ * add order the device kernel uses (reference global.c:241-257).
+ * ssx_geometry_host: level-0 premultiplied factors G*w*detJ (6*nel*n^3,
+ * layout of SSX_DATA_GFAC) and w*detJ (nel*n^3) of a mesh (reference
+ * geometry.c:102-135, poisson.c:58-66). */
+int ssx_dss_host(const ssx_mesh *mesh, int n, double *u);
+int ssx_geometry_host(const ssx_mesh *mesh, double *gfac, double *mass);
+
 /* ---- context ---- */
 int ssx_create(ssx_ctx **ctx, const ssx_mesh *mesh);
 void ssx_destroy(ssx_ctx *ctx);

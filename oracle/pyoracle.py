@@ -68,6 +68,8 @@ def load():
         "orc_mg_precon": (None, [vp, i, i, d, dp, dp]),
         "orc_set_diag_choice": (None, [vp, i]),
         "orc_counters": (None, [vp, C.POINTER(C.c_longlong)]),
+        "orc_set_threads": (None, [i]),
+        "orc_get_threads": (i, []),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -75,6 +77,14 @@ def load():
         f.argtypes = args
     _lib = L
     return L
+
+
+def set_threads(n):
+    load().orc_set_threads(int(n))
+
+
+def get_threads():
+    return load().orc_get_threads()
 
 
 def _dp(a):

@@ -29,6 +29,9 @@
 #include <stdlib.h>
 #include <string.h>
 #include "ssx_oracle.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define NMAX ORC_NMAX
 #define LMAX ORC_LMAX
@@ -1210,4 +1213,22 @@ const double *orc_level_data(const orc_ctx *c, int level, int which)
 	case 17: return L->prol;
 	default: return NULL;
 	}
+}
+
+void orc_set_threads(int nthreads)
+{
+#ifdef _OPENMP
+	omp_set_num_threads(nthreads > 0 ? nthreads : 1);
+#else
+	(void) nthreads;
+#endif
+}
+
+int orc_get_threads(void)
+{
+#ifdef _OPENMP
+	return omp_get_max_threads();
+#else
+	return 1;
+#endif
 }

@@ -121,6 +121,9 @@ void orc_set_diag_choice(orc_ctx *c, int choice);
 /* Cumulative counts since creation: [0] fine-level opeval calls,
  * [1] coarse CG iterations, [2] smoother CG iterations, [3] V-cycles. */
 void orc_counters(const orc_ctx *c, long long *out4);
+/* OpenMP threads used by the element loops (1 = serial) */
+void orc_set_threads(int nthreads);
+int orc_get_threads(void);
 
 #ifdef __cplusplus
 }

@@ -884,6 +884,39 @@ int ssx_fcg(ssx_ctx *c, int op, int var, int precon, double *f, double *u,
 	return rc;
 }
 
+int ssx_dss_host(const ssx_mesh *m, int n, double *u)
+{
+	if (!m || !u || n < SSX_NMIN || n > SSX_NMAX || m->nel < 1)
+		return SSX_EINVAL;
+	int *grp[3];
+	long long ng[3];
+	TRY(ssx_build_groups(m, n, grp, ng));
+	ssx_host_dss(grp, ng, u);
+	for (int i = 0; i < 3; ++i)
+		free(grp[i]);
+	return SSX_OK;
+}
+
+int ssx_geometry_host(const ssx_mesh *m, double *gfac, double *mass)
+{
+	if (!m || !gfac || !mass || m->n < SSX_NMIN || m->n > SSX_NMAX
+			|| m->nel < 1)
+		return SSX_EINVAL;
+	const size_t nt = (size_t) m->nel * m->n * m->n * m->n;
+	double *dp = malloc(sizeof(double) * nt), *dh = malloc(sizeof(double) * nt);
+	if (!dp || !dh) {
+		free(dp);
+		free(dh);
+		return SSX_ENOMEM;
+	}
+	ssx_basis b;
+	ssx_basis_init(&b, m->n, m->svvn, m->svva);
+	level_geometry(&b, m->nel, m->x, m->y, m->z, 0.0, gfac, mass, dp, dh);
+	free(dp);
+	free(dh);
+	return SSX_OK;
+}
+
 /* host-array entry points, solver.c:159-189 */
 static int host_solve(ssx_ctx *c, double *f, double *u, int var, double eps,
 		int helm, ssx_stats *stats)

@@ -15,7 +15,8 @@ ERRORS = {1: "EINVAL", 2: "ENOMEM", 3: "EHIP", 4: "ENOTCONV", 5: "EMESH",
 
 # every function declared in include/ssx.h
 EXPORTED_SYMBOLS = [
-    "ssx_last_error", "ssx_version", "ssx_mesh_box", "ssx_create",
+    "ssx_last_error", "ssx_version", "ssx_mesh_box", "ssx_dss_host",
+    "ssx_geometry_host", "ssx_create",
     "ssx_destroy", "ssx_set_stream", "ssx_get_stream", "ssx_synchronize",
     "ssx_nlevels", "ssx_level_n", "ssx_nel", "ssx_level_ntot", "ssx_lambda",
     "ssx_level_data", "ssx_dss_groups", "ssx_set_diag_choice", "ssx_ax",
@@ -82,6 +83,8 @@ def load():
         "ssx_version": (C.c_char_p, []),
         "ssx_mesh_box": (i, [i, i, i, i, dp, dp, ip, i, d, dp, dp, dp, ip, ip,
                              ip, ip, ip]),
+        "ssx_dss_host": (i, [C.POINTER(MeshStruct), i, dp]),
+        "ssx_geometry_host": (i, [C.POINTER(MeshStruct), dp, dp]),
         "ssx_create": (i, [C.POINTER(vp), C.POINTER(MeshStruct)]),
         "ssx_destroy": (None, [vp]),
         "ssx_set_stream": (i, [vp, vp]),

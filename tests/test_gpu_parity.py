@@ -1,0 +1,259 @@
+"""GPU path (libssx HIP kernels, called through the C ABI) against the golden
+vectors of the compiled reference and against the oracle.
+
+Bars (also stated in each assert): FP64 Ax / assembled operator 1e-12
+relative; DSS bit-exact (integer and random fields: the add order is the
+reference's); multiplicity and mask arrays exact; CG/FCG iteration counts
+within one of the reference; solutions within the solve tolerance."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import ssenss_amd as S  # noqa: E402
+from ssenss_amd.mesh import box_mesh, NOSLIP, PERIODIC  # noqa: E402
+from conftest import golden, golden_mesh, rel, CASES  # noqa: E402
+from oracle import pyoracle as P  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def make_ctx(mesh):
+    ctx = S.Context(mesh)
+    ctx.set_stream(torch.cuda.current_stream())
+    return ctx
+
+
+@pytest.fixture(scope="module", params=CASES)
+def gcase(request):
+    Z = golden(request.param)
+    mesh = golden_mesh(Z)
+    print(f"[gpu] case {request.param} n={mesh.n} nel={mesh.nel}", flush=True)
+    ctx = make_ctx(mesh)
+    yield request.param, Z, mesh, ctx
+    ctx.close()
+
+
+def test_level_data(gcase):
+    name, Z, mesh, ctx = gcase
+    assert ctx.nlevels == len(Z["levels"])
+    for l in range(ctx.nlevels):
+        assert ctx.level_n(l) == Z["levels"][l]
+        L = Z[f"ref_level{l}"].reshape(4, -1)
+        assert rel(ctx.level_data(l, S.DATA_DIAG_P), L[0]) <= 1e-13
+        assert rel(ctx.level_data(l, S.DATA_DIAG_H), L[1]) <= 1e-13
+        assert np.array_equal(ctx.level_data(l, S.DATA_SCALE), L[2])
+        assert np.array_equal(ctx.level_data(l, S.DATA_MASK), L[3])
+    assert rel(ctx.level_data(0, S.DATA_INVMASS), Z["ref_invmass"]) <= 1e-13
+
+
+def test_ax_poisson_helmholtz(gcase):
+    name, Z, mesh, ctx = gcase
+    u = dev(Z["in_u"])
+    au = torch.zeros_like(u)
+    ctx.ax(0, S.OP_POISSON, u, au)
+    assert rel(host(au), Z["ref_ax_poisson"]) <= 1e-12       # FP64 bar
+    ctx.ax(0, S.OP_HELMHOLTZ, u, au)
+    assert rel(host(au), Z["ref_ax_helmholtz"]) <= 1e-12
+
+
+def test_opeval(gcase):
+    name, Z, mesh, ctx = gcase
+    u = dev(Z["in_u"])
+    au = torch.zeros_like(u)
+    ctx.opeval(0, S.OP_POISSON, S.PRESSURE, u, au)
+    assert rel(host(au), Z["ref_opeval_pressure"]) <= 1e-12
+    ctx.opeval(0, S.OP_HELMHOLTZ, S.X_VELOCITY, u, au)
+    assert rel(host(au), Z["ref_opeval_helm_vel"]) <= 1e-12
+
+
+def test_dss_bit_exact(gcase):
+    name, Z, mesh, ctx = gcase
+    nt = mesh.ntot
+    q = np.arange(nt, dtype=np.uint64)
+    iv = ((q * np.uint64(2654435761)) % np.uint64(1000003)).astype(np.float64)
+    t = dev(iv)
+    ctx.dss(0, t)
+    assert np.array_equal(host(t), Z["ref_dss_int"])
+    t = dev(Z["in_u"])
+    ctx.dss(0, t)
+    assert np.array_equal(host(t), Z["ref_dss_u"])
+    t = dev(Z["in_u"])
+    ctx.dss_average(0, t)
+    assert np.array_equal(host(t), Z["ref_dss_avg"])
+
+
+def test_dot_nullspace_transfers(gcase):
+    name, Z, mesh, ctx = gcase
+    u = dev(Z["in_u"])
+    one = torch.ones_like(u)
+    d = Z["ref_dots"]
+    assert abs(ctx.dot(0, u, u) / d[0] - 1) <= 1e-13
+    assert abs(ctx.dot(0, u, one) - d[1]) <= 1e-12 * np.sqrt(d[0] * d[2])
+    assert ctx.dot(0, one, one) == d[2]
+    t = dev(Z["in_u"])
+    ctx.nullspace(0, S.PRESSURE, t)
+    assert rel(host(t), Z["ref_nullspace"]) <= 1e-13
+    if ctx.nlevels > 1:
+        o = P.Oracle(mesh)
+        ref = o.dss_average(1, Z["ref_restrict_raw"])
+        c = torch.zeros(ctx.ntot(1), dtype=torch.float64, device="cuda")
+        ctx.restrict(1, u, c)
+        assert rel(host(c), ref) <= 1e-13
+        f = torch.zeros_like(u)
+        ctx.prolong(0, dev(Z["in_v1"]), f)
+        assert rel(host(f), Z["ref_prolong"]) <= 1e-13
+
+
+def test_mg_precon_application(gcase):
+    name, Z, mesh, ctx = gcase
+    eps = Z["ref_scalars"][0]
+    b = dev(Z["in_f"])
+    pb = torch.zeros_like(b)
+    ctx.set_diag_choice(-1)
+    ctx.mg_precon(S.OP_POISSON, S.PRESSURE, eps, b, pb)
+    assert rel(host(pb), Z["ref_mg_pressure"]) <= 1e-9
+
+
+def test_cg_helmholtz(gcase):
+    name, Z, mesh, ctx = gcase
+    sc = Z["ref_scalars"]
+    eps = sc[0]
+    f = dev(Z["in_f"])
+    u = torch.zeros_like(f)
+    ctx.set_diag_choice(-1)
+    st = ctx.cg(0, S.OP_HELMHOLTZ, S.X_VELOCITY, f, u, eps, 5000)
+    print(f"[gpu] {name} cg helmholtz iters {st.iterations} ref {int(sc[1])}")
+    assert abs(st.iterations - int(sc[1])) <= 1 and st.converged
+    assert rel(host(u), Z["ref_cg_helm"]) <= 1e-8
+
+
+def test_fcg_mg_pressure(gcase):
+    name, Z, mesh, ctx = gcase
+    sc = Z["ref_scalars"]
+    eps = sc[0]
+    f = dev(Z["in_f"])
+    u = torch.zeros_like(f)
+    ctx.set_diag_choice(-1)
+    st = ctx.fcg(S.OP_POISSON, S.PRESSURE, S.PC_MG, f, u, eps, 5000)
+    print(f"[gpu] {name} fcg pressure iters {st.iterations} ref {int(sc[3])}")
+    assert abs(st.iterations - int(sc[3])) <= 1 and st.converged
+    assert rel(host(u), Z["ref_fcg_pressure"]) <= 1e-8
+
+
+def test_fcg_mg_dirichlet_poisson(gcase):
+    name, Z, mesh, ctx = gcase
+    sc = Z["ref_scalars"]
+    eps = sc[0]
+    f = dev(Z["in_f"])
+    u = torch.zeros_like(f)
+    ctx.set_diag_choice(0)
+    try:
+        st = ctx.fcg(S.OP_POISSON, S.X_VELOCITY, S.PC_MG, f, u, eps, 5000)
+    finally:
+        ctx.set_diag_choice(-1)
+    print(f"[gpu] {name} fcg dirichlet iters {st.iterations} ref {int(sc[5])}")
+    assert abs(st.iterations - int(sc[5])) <= 1 and st.converged
+    assert rel(host(u), Z["ref_fcg_dirichlet"]) <= 1e-8
+
+
+def test_host_array_entry_points():
+    Z = golden("cube2_n8")
+    mesh = golden_mesh(Z)
+    sc = Z["ref_scalars"]
+    with S.Context(mesh) as ctx:
+        f = Z["in_f"].copy()
+        u = np.zeros(mesh.ntot)
+        st = ctx.solve_helmholtz_cg(f, u, S.X_VELOCITY, sc[0])
+        assert abs(st.iterations - int(sc[1])) <= 1
+        assert rel(u, Z["ref_cg_helm"]) <= 1e-8
+        f = Z["in_f"].copy()
+        u = np.zeros(mesh.ntot)
+        st = ctx.solve_poisson_cg(f, u, S.PRESSURE, sc[0])
+        assert abs(st.iterations - int(sc[3])) <= 1
+        assert rel(u, Z["ref_fcg_pressure"]) <= 1e-8
+
+
+@pytest.mark.parametrize("n", list(range(2, 17)))
+def test_ax_every_order_against_oracle(n):
+    """All compiled orders, ragged element counts, curved elements."""
+    mesh = box_mesh(n, (3, 2, 5), bc=(NOSLIP, NOSLIP, NOSLIP, NOSLIP,
+                                      PERIODIC, PERIODIC),
+                    deform=1, amp=0.07, dt=0.01, visc=0.5)
+    o = P.Oracle(mesh)
+    rng = np.random.default_rng(100 + n)
+    u = rng.standard_normal(mesh.ntot)
+    with make_ctx(mesh) as ctx:
+        du, au = dev(u), torch.zeros(mesh.ntot, dtype=torch.float64,
+                                     device="cuda")
+        for op in (S.OP_POISSON, S.OP_HELMHOLTZ):
+            ctx.ax(0, op, du, au)
+            assert rel(host(au), o.ax(0, op, u)) <= 1e-12, (n, op)
+        ctx.opeval(0, S.OP_POISSON, S.X_VELOCITY, du, au)
+        assert rel(host(au), o.opeval(0, S.OP_POISSON, S.X_VELOCITY, u)) \
+            <= 1e-12
+        t = dev(u)
+        ctx.dss(0, t)
+        assert np.array_equal(host(t), o.dss(0, u))
+
+
+def test_errors_are_reported():
+    Z = golden("box2_n5")
+    mesh = golden_mesh(Z)
+    with make_ctx(mesh) as ctx:
+        f = dev(Z["in_f"])
+        u = torch.zeros_like(f)
+        st = ctx.cg(0, S.OP_HELMHOLTZ, S.X_VELOCITY, f, u, 1e-30, 3,
+                    allow_notconv=True)
+        assert st.iterations == 3 and not st.converged
+        with pytest.raises(S.SsxError) as e:
+            ctx.cg(0, S.OP_HELMHOLTZ, S.X_VELOCITY, f, u, 1e-30, 3)
+        assert e.value.code == 4
+        with pytest.raises(S.SsxError):
+            ctx.ax(7, S.OP_POISSON, f, u)
+
+
+def test_full_size_properties():
+    """32^3 elements at n = 8 (16.8M local nodes): size-independent
+    properties at the benchmark size."""
+    mesh = box_mesh(8, (32, 32, 32), deform=1, amp=0.1)
+    print("[gpu] full size 32^3 n=8", flush=True)
+    with make_ctx(mesh) as ctx:
+        nt = mesh.ntot
+        one = torch.ones(nt, dtype=torch.float64, device="cuda")
+        assert ctx.dot(0, one, one) == 225 ** 3        # unique nodes
+        au = torch.zeros_like(one)
+        ctx.ax(0, S.OP_POISSON, one, au)
+        g = ctx.level_data(0, S.DATA_GFAC)
+        assert host(au.abs().max()) <= 1e-12 * np.abs(g).max() * 64
+        # integer-valued DSS equals the host assembly bit for bit
+        q = torch.arange(nt, dtype=torch.int64, device="cuda")
+        iv = ((q * 2654435761) % 1000003).to(torch.float64)
+        ref = iv.cpu().numpy().copy()
+        import ctypes as C
+        ms = mesh.struct()
+        S.native.check(S.load().ssx_dss_host(
+            C.byref(ms), 8, ref.ctypes.data_as(C.POINTER(C.c_double))))
+        ctx.dss(0, iv)
+        assert np.array_equal(host(iv), ref)
+        # symmetry of the assembled operator on continuous vectors
+        rng = np.random.default_rng(7)
+        x = dev(rng.standard_normal(nt))
+        y = dev(rng.standard_normal(nt))
+        ctx.dss_average(0, x)
+        ctx.dss_average(0, y)
+        ax = torch.zeros_like(x)
+        ay = torch.zeros_like(y)
+        ctx.opeval(0, S.OP_POISSON, S.X_VELOCITY, x, ax)
+        ctx.opeval(0, S.OP_POISSON, S.X_VELOCITY, y, ay)
+        a, b = ctx.dot(0, y, ax), ctx.dot(0, x, ay)
+        assert abs(a - b) <= 1e-10 * abs(a)
