@@ -96,6 +96,10 @@ def main():
     ap.add_argument("--max-iter", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="kernels only, for rocprofv3 counter runs: the "
+                    "Poisson Ax on the workload, then a 1 GiB device copy "
+                    "(known bytes, to calibrate FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this "
@@ -133,6 +137,17 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / 1e3 / steps
 
+    if args.profile:
+        for _ in range(args.steps):
+            ctx.ax(0, S.OP_POISSON, u, au)
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)                                   # 1 GiB each way
+        torch.cuda.synchronize()
+        log("[bench] profile mode done")
+        ctx.close()
+        return
     t_ax = timed(lambda: ctx.ax(0, S.OP_POISSON, u, au), args.steps,
                  args.warmup)
     log(f"[bench] Ax {t_ax * 1e6:.1f} us")

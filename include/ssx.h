@@ -138,8 +138,11 @@ int ssx_geometry_host(const ssx_mesh *mesh, double *gfac, double *mass);
 /* ---- context ---- */
 int ssx_create(ssx_ctx **ctx, const ssx_mesh *mesh);
 void ssx_destroy(ssx_ctx *ctx);
-/* run all work of ctx on this hipStream_t (NULL: the context's own stream) */
+/* Run all later work of ctx on this hipStream_t.  NULL is the HIP null
+ * (default) stream, e.g. torch's default stream; a context starts on a
+ * non-blocking stream of its own, which ssx_own_stream() returns. */
 int ssx_set_stream(ssx_ctx *ctx, void *hip_stream);
+void *ssx_own_stream(ssx_ctx *ctx);
 void *ssx_get_stream(ssx_ctx *ctx);
 int ssx_synchronize(ssx_ctx *ctx);
 /* levels, points per direction of a level, element count, local nodes */
@@ -154,6 +157,11 @@ int ssx_level_data(ssx_ctx *ctx, int level, int which, double *host);
 int ssx_dss_groups(const ssx_ctx *ctx, int level, long long counts[3]);
 /* Jacobi diagonal choice: 0 Poisson, 1 Helmholtz, -1 by var (default) */
 int ssx_set_diag_choice(ssx_ctx *ctx, int choice);
+/* options: SSX_OPT_FUSED_CG 1 (default) runs each CG iteration as six fused
+ * kernels with the stopping test on the device; 0 runs the plain
+ * one-kernel-per-vector-operation loop with a host check per iteration */
+#define SSX_OPT_FUSED_CG 1
+int ssx_set_option(ssx_ctx *ctx, int option, int value);
 
 /* ---- operators on device E-vectors ---- */
 int ssx_ax(ssx_ctx *ctx, int level, int op, const double *u, double *au);

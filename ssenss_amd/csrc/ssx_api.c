@@ -198,6 +198,7 @@ static int setup_level(ssx_ctx *c, const ssx_mesh *m, int l, double *hx,
 	const int n = L->n, n3 = L->n3;
 	const size_t nt = L->ntot;
 	int rc;
+	double *mult = NULL;
 	double *gf = malloc(sizeof(double) * 6 * nt);
 	double *mass = malloc(sizeof(double) * nt);
 	double *dp = malloc(sizeof(double) * nt);
@@ -227,6 +228,12 @@ static int setup_level(ssx_ctx *c, const ssx_mesh *m, int l, double *hx,
 		nun += one[q];
 	}
 	L->nunique = nun;
+	mult = malloc(sizeof(double) * nt);
+	if (!mult) {
+		rc = SSX_ENOMEM;
+		goto out;
+	}
+	memcpy(mult, one, sizeof(double) * nt);
 	rc = up(&L->gf, gf, 6 * nt);
 	if (!rc) rc = up(&L->mass, mass, nt);
 	if (!rc) rc = up(&L->diag_p, dp, nt);
@@ -289,13 +296,36 @@ static int setup_level(ssx_ctx *c, const ssx_mesh *m, int l, double *hx,
 		L->grp[mi] = d;
 	}
 	{
+		/* bits 0-5: Dirichlet faces; bits 8-13: faces with a neighbour */
 		int *bcb = malloc(sizeof(int) * c->nel);
 		for (int e = 0; e < c->nel; ++e) {
 			int bits = 0;
-			for (int s = 0; s < 6; ++s)
+			for (int s = 0; s < 6; ++s) {
 				if (m->neigh[e * 6 + s] == -1)
 					bits |= 1 << s;
+				if (m->neigh[e * 6 + s] >= 0)
+					bits |= 256 << s;
+			}
 			bcb[e] = bits;
+		}
+		/* the fused inner products take multiplicity 1 for nodes on no
+		 * face with a neighbour; check that against the assembled
+		 * multiplicity (scale still holds 1/mult in `mult`) */
+		L->fused_ok = 1;
+		for (int e = 0; e < c->nel && L->fused_ok; ++e) {
+			const int eb = bcb[e];
+			for (int k = 0; k < n && L->fused_ok; ++k)
+				for (int j = 0; j < n; ++j)
+					for (int i = 0; i < n; ++i) {
+						int sh = ((eb & 256) && i == 0) || ((eb & 512) && i == n - 1)
+							|| ((eb & 1024) && j == 0) || ((eb & 2048) && j == n - 1)
+							|| ((eb & 4096) && k == 0) || ((eb & 8192) && k == n - 1);
+						double w = mult[(size_t) e * n3 + (k * n + j) * n + i];
+						if ((w < 1.0) != sh) {
+							L->fused_ok = 0;
+							break;
+						}
+					}
 		}
 		if (hipMalloc((void **) &L->bc, sizeof(int) * c->nel) != hipSuccess
 				|| hipMemcpy(L->bc, bcb, sizeof(int) * c->nel,
@@ -315,7 +345,7 @@ static int setup_level(ssx_ctx *c, const ssx_mesh *m, int l, double *hx,
 	if (!rc) rc = dalloc(&L->mg_fn, nt);
 	if (!rc) rc = dalloc(&L->mg_e, nt);
 out:
-	free(gf); free(mass); free(dp); free(dh); free(one);
+	free(gf); free(mass); free(dp); free(dh); free(one); free(mult);
 	return rc;
 }
 
@@ -338,6 +368,7 @@ int ssx_create(ssx_ctx **out, const ssx_mesh *m)
 	c->nel = m->nel;
 	c->n = m->n;
 	c->diag_choice = -1;
+	c->fused = 1;
 	c->lambda = (11.0 / 6.0) / (m->dt * m->visc);
 	int nl = SSX_LMAX;
 	for (int i = 0; i < SSX_LMAX; ++i) {     /* precon.c:62-70 */
@@ -406,8 +437,26 @@ int ssx_create(ssx_ctx **out, const ssx_mesh *m)
 	}
 	if (rc == SSX_OK) {
 		const size_t nt = c->lev[0].ntot;
+		size_t np = (size_t) 8 * SSX_NPART;
+		for (int l = 0; l < nl; ++l) {
+			size_t q = (size_t) ssxk_ax_blocks(c->lev[l].n, c->nel)
+				+ (size_t) ssxk_dss_blocks(c->lev[l].ngrp);
+			if (q > np)
+				np = q;
+		}
+		c->npart = np;
 		rc = dalloc(&c->scal, SSX_NSCAL);
-		if (!rc) rc = dalloc(&c->part, (size_t) 8 * SSX_NPART);
+		if (!rc) rc = dalloc(&c->part, np);
+		if (!rc && hipMalloc((void **) &c->cg_flags, sizeof(int) * 2 * SSX_LMAX)
+				!= hipSuccess) {
+			rc = SSX_ENOMEM;
+			ssx_set_error("hipMalloc flags");
+		}
+		if (!rc && hipHostMalloc((void **) &c->h_flags,
+				sizeof(int) * 2 * SSX_LMAX, 0) != hipSuccess) {
+			rc = SSX_ENOMEM;
+			ssx_set_error("hipHostMalloc flags");
+		}
 		if (!rc) rc = dalloc(&c->fcg_r, nt);
 		if (!rc) rc = dalloc(&c->fcg_z, nt);
 		if (!rc && hipHostMalloc((void **) &c->h_scal,
@@ -443,6 +492,9 @@ void ssx_destroy(ssx_ctx *c)
 		free(L->h_x); free(L->h_y); free(L->h_z);
 	}
 	hipFree(c->inv_mass);
+	hipFree(c->cg_flags);
+	if (c->h_flags)
+		hipHostFree(c->h_flags);
 	hipFree(c->scal);
 	hipFree(c->part);
 	hipFree(c->fcg_r);
@@ -465,11 +517,12 @@ int ssx_set_stream(ssx_ctx *c, void *s)
 	if (!c)
 		return SSX_EINVAL;
 	HIPC(hipStreamSynchronize((hipStream_t) c->stream));
-	c->stream = s ? s : c->own_stream;
+	c->stream = s;
 	return SSX_OK;
 }
 
 void *ssx_get_stream(ssx_ctx *c) { return c ? c->stream : NULL; }
+void *ssx_own_stream(ssx_ctx *c) { return c ? c->own_stream : NULL; }
 
 int ssx_synchronize(ssx_ctx *c)
 {
@@ -489,6 +542,20 @@ int ssx_level_n(const ssx_ctx *c, int l)
 size_t ssx_level_ntot(const ssx_ctx *c, int l)
 {
 	return (l >= 0 && l < c->nlev) ? c->lev[l].ntot : 0;
+}
+
+int ssx_set_option(ssx_ctx *c, int option, int value)
+{
+	if (!c)
+		return SSX_EINVAL;
+	switch (option) {
+	case SSX_OPT_FUSED_CG:
+		c->fused = value != 0;
+		return SSX_OK;
+	default:
+		ssx_set_error("unknown option %d", option);
+		return SSX_EINVAL;
+	}
 }
 
 int ssx_set_diag_choice(ssx_ctx *c, int choice)
@@ -556,13 +623,14 @@ int ssx_ax(ssx_ctx *c, int l, int op, const double *u, double *au)
 	TRY(chk(c, l));
 	ssx_level *L = &c->lev[l];
 	return ssxk_ax(c->stream, L->n, c->nel, op, u, au, L->gf, L->mass,
-			L->dmat, c->lambda, L->bc, 0);
+			L->dmat, c->lambda, L->bc, 0, NULL, NULL);
 }
 
 int ssx_dss(ssx_ctx *c, int l, double *u)
 {
 	TRY(chk(c, l));
-	return ssxk_dss(c->stream, c->lev[l].grp, c->lev[l].ngrp, u);
+	return ssxk_dss_all(c->stream, c->lev[l].grp, c->lev[l].ngrp, u, NULL,
+			NULL, 0, NULL);
 }
 
 int ssx_opeval(ssx_ctx *c, int l, int op, int var, const double *p,
@@ -574,14 +642,17 @@ int ssx_opeval(ssx_ctx *c, int l, int op, int var, const double *p,
 		c->cnt[0]++;
 	/* element operator, mask (var != PRESSURE), DSS: solver.c:191-198 */
 	TRY(ssxk_ax(c->stream, L->n, c->nel, op, p, ap, L->gf, L->mass,
-			L->dmat, c->lambda, L->bc, var != SSX_PRESSURE));
-	return ssxk_dss(c->stream, L->grp, L->ngrp, ap);
+			L->dmat, c->lambda, L->bc, var != SSX_PRESSURE, NULL, NULL));
+	return ssxk_dss_all(c->stream, L->grp, L->ngrp, ap, NULL, NULL, 0, NULL);
 }
 
 int ssx_dss_average(ssx_ctx *c, int l, double *u)
 {
-	TRY(ssx_dss(c, l, u));
-	return ssxk_mul(c->stream, c->lev[l].ntot, u, c->lev[l].scale, u);
+	TRY(chk(c, l));
+	/* DSS then x 1/mult; 1/mult is exactly 1/M on a group of M copies and
+	 * 1 elsewhere, so the group kernel scales in place (global.c:259-268) */
+	return ssxk_dss_all(c->stream, c->lev[l].grp, c->lev[l].ngrp, u, NULL,
+			NULL, 1, NULL);
 }
 
 static int dot_to(ssx_ctx *c, int l, const double *x, const double *y,
@@ -662,9 +733,12 @@ int ssx_jacobi(ssx_ctx *c, int l, int var, const double *x, double *px)
 /* solvers                                                              */
 /* ------------------------------------------------------------------ */
 
-/* conjugate_gradient, solver.c:33-81, Jacobi preconditioner at level l */
-static int cg_level(ssx_ctx *c, int l, int op, int var, double *f, double *u,
-		double eps, int max_iter, int *iters, double *err_out)
+/* conjugate_gradient, solver.c:33-81, Jacobi preconditioner at level l:
+ * one launch per vector operation and one host read of the residual per
+ * iteration (kept as the plain path and for meshes where the fused inner
+ * products do not apply) */
+static int cg_level_plain(ssx_ctx *c, int l, int op, int var, double *f,
+		double *u, double eps, int max_iter, int *iters, double *err_out)
 {
 	ssx_level *L = &c->lev[l];
 	const size_t nt = L->ntot;
@@ -712,13 +786,104 @@ static int cg_level(ssx_ctx *c, int l, int op, int var, double *f, double *u,
 	return SSX_OK;
 }
 
+/*
+ * The same iteration with six launches: Ax with <p,Ap> over unshared nodes,
+ * DSS with the shared part, a reduce that forms alpha, the fused update
+ * (u, r, z and the <r,z>, <r,r> partials), a reduce that forms beta and the
+ * stopping decision, and the p update.  The stopping test of solver.c:73-74
+ * runs on the device: once it fires every later kernel of the solve returns
+ * at once, so the host enqueues iterations in chunks and synchronises once
+ * per chunk.  x_zero: the initial guess is known to be zero (A*0 = 0 is
+ * exact, so skipping the product changes nothing).
+ */
+static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
+		double *u, double eps, int max_iter, int x_zero, int *iters,
+		double *err_out)
+{
+	ssx_level *L = &c->lev[l];
+	const size_t nt = L->ntot;
+	double *r = L->r, *z = L->z, *p = L->p, *ap = L->ap;
+	void *st = c->stream;
+	const int pres = var == SSX_PRESSURE;
+	const double *mask = pres ? NULL : L->mask;
+	const double *dinv = jdiag(c, l, var);
+	const int o = SC_CG(l);
+	int *flags = c->cg_flags + 2 * l;
+	TRY(ssx_mask(c, l, var, f));
+	if (x_zero) {
+		TRY(ssxk_copy(st, r, f, nt));
+	} else {
+		TRY(ssx_opeval(c, l, op, var, u, r));
+		TRY(ssxk_axpby(st, nt, 1.0, f, NULL, -1.0, r));
+	}
+	TRY(ssx_nullspace(c, l, var, r));
+	TRY(ssxk_mul(st, nt, r, dinv, z));
+	TRY(ssx_nullspace(c, l, var, z));
+	TRY(ssxk_copy(st, p, z, nt));
+	TRY(dot_to(c, l, r, z, o + CG_RDZ));
+	TRY(ssxk_set_ints(st, flags, 0, 2));
+	const long long nba = ssxk_ax_blocks(L->n, c->nel);
+	const long long nbd = ssxk_dss_blocks(L->ngrp);
+	int launched = 0;
+	int chunk = c->cg_hint[l] > 0 ? c->cg_hint[l] + 1 : 8;
+	for (;;) {
+		if (chunk > max_iter - launched)
+			chunk = max_iter - launched;
+		for (int k = 0; k < chunk; ++k) {
+			TRY(ssxk_ax(st, L->n, c->nel, op, p, ap, L->gf, L->mass,
+					L->dmat, c->lambda, L->bc, !pres, flags, c->part));
+			TRY(ssxk_dss_all(st, L->grp, L->ngrp, ap, p, c->part + nba, 0,
+					flags));
+			TRY(ssxk_cg_reduce(st, 0, c->part, nba + nbd, c->scal, o, flags,
+					eps, max_iter, L->nunique, pres));
+			TRY(ssxk_cg_update(st, nt, c->scal, o + CG_ALPHA, p, ap, u, r, z,
+					dinv, L->scale, pres, c->part, flags));
+			TRY(ssxk_cg_reduce(st, 1, c->part, 0, c->scal, o, flags, eps,
+					max_iter, L->nunique, pres));
+			TRY(ssxk_cg_pupdate(st, nt, z, p, c->scal, o + CG_BETA,
+					pres ? o + CG_NULLC : -1, mask, flags));
+		}
+		launched += chunk;
+		hipStream_t hs = (hipStream_t) st;
+		HIPC(hipMemcpyAsync(c->h_flags, flags, 2 * sizeof(int),
+				hipMemcpyDeviceToHost, hs));
+		HIPC(hipMemcpyAsync(c->h_scal + o + CG_RR, c->scal + o + CG_RR,
+				sizeof(double), hipMemcpyDeviceToHost, hs));
+		HIPC(hipStreamSynchronize(hs));
+		if (c->h_flags[0] || launched >= max_iter)
+			break;
+		chunk = 4;
+	}
+	const int it = c->h_flags[1];
+	const double err = sqrt(c->h_scal[o + CG_RR]);
+	c->cg_hint[l] = it;
+	if (iters)
+		*iters = it;
+	*err_out = err;
+	if (isnan(err) || isinf(err)) {
+		ssx_set_error("CG level %d: residual is %g", l, err);
+		return SSX_ENUMERIC;
+	}
+	return SSX_OK;
+}
+
+static int cg_level(ssx_ctx *c, int l, int op, int var, double *f, double *u,
+		double eps, int max_iter, int x_zero, int *iters, double *err_out)
+{
+	if (c->fused && c->lev[l].fused_ok)
+		return cg_level_fused(c, l, op, var, f, u, eps, max_iter, x_zero,
+				iters, err_out);
+	return cg_level_plain(c, l, op, var, f, u, eps, max_iter, iters,
+			err_out);
+}
+
 static int mg_recurse(ssx_ctx *c, int l, int op, int var, double *f,
 		double *x, double eps)
 {
 	double err;
 	int it;
 	if (l == c->nlev - 1) {          /* precon.c:242-244 */
-		TRY(cg_level(c, l, op, var, f, x, eps, 500, &it, &err));
+		TRY(cg_level(c, l, op, var, f, x, eps, 500, l > 0, &it, &err));
 		c->cnt[1] += it;
 		return SSX_OK;
 	}
@@ -736,7 +901,7 @@ static int mg_recurse(ssx_ctx *c, int l, int op, int var, double *f,
 	TRY(mg_recurse(c, l + 1, op, var, fn, e, eps));
 	/* x += J e (prolongation fused with the add, precon.c:262-263) */
 	TRY(ssxk_prolong(st, L->n, C->n, c->nel, L->prol, e, x, 1));
-	TRY(cg_level(c, l, op, var, f, x, eps, 15, &it, &err));
+	TRY(cg_level(c, l, op, var, f, x, eps, 15, 0, &it, &err));
 	c->cnt[2] += it;
 	return SSX_OK;
 }
@@ -758,7 +923,7 @@ int ssx_cg(ssx_ctx *c, int l, int op, int var, double *f, double *u,
 	memcpy(c0, c->cnt, sizeof(c0));
 	double t0 = now_s(), err;
 	int it = 0;
-	int rc = cg_level(c, l, op, var, f, u, eps, max_iter, &it, &err);
+	int rc = cg_level(c, l, op, var, f, u, eps, max_iter, 0, &it, &err);
 	if (rc == SSX_OK)
 		rc = ssx_synchronize(c);
 	if (stats) {

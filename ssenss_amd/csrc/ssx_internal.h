@@ -17,7 +17,7 @@ extern "C" {
 #define SSX_NMAX 16          /* GLL points per direction, kernels 2..16    */
 #define SSX_LMAX 10          /* reference structs.c:58 new_preconditioner(n,10) */
 #define SSX_NPART 1024       /* blocks of the reduction kernels            */
-#define SSX_NSCAL 64         /* device scalar slots                        */
+#define SSX_NSCAL 256        /* device scalar slots                        */
 
 /* device scalar slots */
 enum {
@@ -27,6 +27,10 @@ enum {
 	SC_BETA0 = SC_PAP0 + 10 /* .. SC_BETA0+9 */,
 	SC_END = SC_BETA0 + 10
 };
+
+/* per-level scalar block of the fused CG (solver.c:33-81) */
+enum { CG_RDZ = 0, CG_PAP, CG_ALPHA, CG_BETA, CG_RR, CG_NULLC, CG_NSLOT = 8 };
+#define SC_CG(l) (64 + CG_NSLOT * (l))
 
 /* host-side basis of one level */
 typedef struct {
@@ -53,6 +57,8 @@ typedef struct {
 	long long ngrp[3];
 	int *bc;                     /* per element Dirichlet bits           */
 	double nunique;              /* <1,1>_w                              */
+	int fused_ok;                /* multiplicity > 1 exactly on faces with
+	                                a neighbour: fused inner products OK */
 	/* solver scratch */
 	double *r, *z, *p, *ap, *mg_r, *mg_fn, *mg_e;
 	/* host copies kept for ssx_level_data */
@@ -65,12 +71,17 @@ struct ssx_ctx {
 	ssx_level lev[SSX_LMAX];
 	double lambda;
 	int diag_choice;
+	int fused;                   /* SSX_OPT_FUSED_CG                     */
 	double *inv_mass;            /* device, level 0 */
 	void *stream;                /* hipStream_t in use */
 	void *own_stream;
 	double *scal;                /* device scalars [SSX_NSCAL]           */
 	double *part;                /* device partial sums [8][SSX_NPART]   */
 	double *h_scal;              /* pinned host mirror                   */
+	int *cg_flags;               /* device: [2*l] done, [2*l+1] count    */
+	int *h_flags;                /* pinned host mirror                   */
+	int cg_hint[SSX_LMAX];       /* last iteration count per level       */
+	size_t npart;                /* doubles in part                      */
 	/* FCG history (level 0) */
 	double *fcg_r, *fcg_z, *fcg_p[11], *fcg_ap[10];
 	/* host staging for the host-array entry points */
@@ -91,9 +102,31 @@ void ssx_host_dss(int *grp[3], const long long ng[3], double *u);
 void ssx_set_error(const char *fmt, ...);
 
 /* ---- kernel launchers (ssx_kernels.hip); all return 0 or SSX_EHIP ---- */
+/* bc[e]: bits 0-5 Dirichlet faces (W,E,S,N,B,T), bits 8-13 faces with a
+ * neighbour; done (nullable): skip when *done; part (nullable): per block
+ * <p,Ap> over nodes of multiplicity 1 */
 int ssxk_ax(void *stream, int n, int nel, int op, const double *u,
 		double *au, const double *gf, const double *mass,
-		const double *dmat, double lambda, const int *bc, int apply_mask);
+		const double *dmat, double lambda, const int *bc, int apply_mask,
+		const int *done, double *part);
+int ssxk_ax_blocks(int n, int nel);
+long long ssxk_dss_blocks(const long long ng[3]);
+/* DSS of all groups in one launch; average: scale by 1/M; p, part
+ * (nullable): per block shared-node part of <p, DSS(u)>_w */
+int ssxk_dss_all(void *stream, int *const grp[3], const long long ng[3],
+		double *u, const double *p, double *part, int average,
+		const int *done);
+int ssxk_cg_update(void *stream, size_t n, const double *scal, int ia,
+		const double *p, const double *ap, double *u, double *r, double *z,
+		const double *dinv, const double *scale, int pressure, double *part,
+		const int *done);
+int ssxk_cg_pupdate(void *stream, size_t n, const double *z, double *p,
+		const double *scal, int ib, int ic, const double *mask,
+		const int *done);
+int ssxk_cg_reduce(void *stream, int mode, const double *part, long long cnt,
+		double *scal, int o, int *flags, double eps, int maxit,
+		double nunique, int pressure);
+int ssxk_set_ints(void *stream, int *dst, int v, int n);
 int ssxk_dss(void *stream, int *const grp[3], const long long ng[3],
 		double *u);
 int ssxk_restrict(void *stream, int nf, int nc, int nel, const double *prol,

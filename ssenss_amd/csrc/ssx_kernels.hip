@@ -31,6 +31,21 @@
 		}                                                            \
 	} while (0)
 
+static inline dim3 vgrid(size_t n)
+{
+	size_t b = (n + 255) / 256;
+	if (b > 16384)
+		b = 16384;
+	if (b == 0)
+		b = 1;
+	return dim3((unsigned) b);
+}
+
+#define GRID_STRIDE(q, n) \
+	for (size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x; q < (n); \
+			q += (size_t) gridDim.x * blockDim.x)
+#define GRID_STRIDE_LOOP GRID_STRIDE
+
 // ---------------------------------------------------------------------
 // Operator
 // ---------------------------------------------------------------------
@@ -49,10 +64,13 @@ __global__ void __launch_bounds__(AxShape<N>::BLOCK)
 k_ax(int nel, const double *__restrict__ u, double *__restrict__ au,
 		const double *__restrict__ gf, const double *__restrict__ mass,
 		const double *__restrict__ dmat, double lambda,
-		const int *__restrict__ bc, int apply_mask)
+		const int *__restrict__ bc, int apply_mask,
+		const int *__restrict__ done, double *__restrict__ part)
 {
 	using S = AxShape<N>;
 	constexpr int N2 = S::N2, N3 = S::N3, P = S::P, TEAMS = S::TEAMS;
+	if (done && *done)          // solver already converged: no-op
+		return;
 	__shared__ double s_d[2 * N2];
 	__shared__ double s_u[TEAMS][N * N * P];
 	__shared__ double s_a[TEAMS][N * P];
@@ -125,50 +143,88 @@ k_ax(int nel, const double *__restrict__ u, double *__restrict__ au,
 			acc[kk] += (kk == k ? r : 0.0) + gDs[k * N + kk] * t2;
 		__syncthreads();
 	}
-	if (!act)
-		return;
-	int bits = apply_mask ? bc[e] : 0;
+	// bc[e]: bits 0-5 Dirichlet faces, bits 8-13 faces with a neighbour
+	const int eb = act ? bc[e] : 0;
+	const int bits = apply_mask ? eb : 0;
 	const bool mi = ((bits & 1) && i == 0) || ((bits & 2) && i == N - 1)
 		|| ((bits & 4) && j == 0) || ((bits & 8) && j == N - 1);
+	const bool si = ((eb & 256) && i == 0) || ((eb & 512) && i == N - 1)
+		|| ((eb & 1024) && j == 0) || ((eb & 2048) && j == N - 1);
+	double pd = 0.0;
+	if (act) {
 #pragma unroll
-	for (int k = 0; k < N; ++k) {
-		const int q = k * N2 + tid;
-		double v = acc[k];
-		if (OP == 1)
-			v += lambda * (mass[eoff + q] * uc[k]);
-		const bool m = mi || ((bits & 16) && k == 0)
-			|| ((bits & 32) && k == N - 1);
-		au[eoff + q] = m ? 0.0 : v;
+		for (int k = 0; k < N; ++k) {
+			const int q = k * N2 + tid;
+			double v = acc[k];
+			if (OP == 1)
+				v += lambda * (mass[eoff + q] * uc[k]);
+			const bool m = mi || ((bits & 16) && k == 0)
+				|| ((bits & 32) && k == N - 1);
+			v = m ? 0.0 : v;
+			au[eoff + q] = v;
+			// <p, Ap>_w over nodes no other element shares (weight 1);
+			// the shared ones are added by the DSS kernel after assembly
+			const bool sh = si || ((eb & 4096) && k == 0)
+				|| ((eb & 8192) && k == N - 1);
+			if (!sh)
+				pd += uc[k] * v;
+		}
+	}
+	if (part) {
+		__shared__ double s_red[256];
+		for (int q = threadIdx.x; q < 256; q += blockDim.x)
+			s_red[q] = 0.0;
+		__syncthreads();
+		s_red[threadIdx.x] = pd;
+		__syncthreads();
+#pragma unroll
+		for (int st = 128; st > 0; st >>= 1) {
+			if ((int) threadIdx.x < st)
+				s_red[threadIdx.x] += s_red[threadIdx.x + st];
+			__syncthreads();
+		}
+		if (threadIdx.x == 0)
+			part[blockIdx.x] = s_red[0];
 	}
 }
 
 template <int N>
 static int launch_ax(hipStream_t st, int nel, int op, const double *u,
 		double *au, const double *gf, const double *mass,
-		const double *dmat, double lambda, const int *bc, int apply_mask)
+		const double *dmat, double lambda, const int *bc, int apply_mask,
+		const int *done, double *part)
 {
 	using S = AxShape<N>;
 	const int grid = (nel + S::TEAMS - 1) / S::TEAMS;
 	if (op == SSX_OP_HELMHOLTZ)
 		hipLaunchKernelGGL((k_ax<N, 1>), dim3(grid), dim3(S::BLOCK), 0, st,
-				nel, u, au, gf, mass, dmat, lambda, bc, apply_mask);
+				nel, u, au, gf, mass, dmat, lambda, bc, apply_mask,
+				done, part);
 	else
 		hipLaunchKernelGGL((k_ax<N, 0>), dim3(grid), dim3(S::BLOCK), 0, st,
-				nel, u, au, gf, mass, dmat, lambda, bc, apply_mask);
+				nel, u, au, gf, mass, dmat, lambda, bc, apply_mask,
+				done, part);
 	LAUNCH_CHECK("k_ax");
 	return SSX_OK;
 }
 
+extern "C" int ssxk_ax_blocks(int n, int nel)
+{
+	const int teams = (256 / (n * n)) > 0 ? 256 / (n * n) : 1;
+	return (nel + teams - 1) / teams;
+}
+
 extern "C" int ssxk_ax(void *stream, int n, int nel, int op, const double *u,
 		double *au, const double *gf, const double *mass,
-		const double *dmat, double lambda, const int *bc, int apply_mask)
+		const double *dmat, double lambda, const int *bc, int apply_mask,
+		const int *done, double *part)
 {
 	hipStream_t st = (hipStream_t) stream;
 	if (nel <= 0)
 		return SSX_OK;
 	switch (n) {
 #define CASE(NN) case NN: return launch_ax<NN>(st, nel, op, u, au, gf, mass, \
-			dmat, lambda, bc, apply_mask);
+			dmat, lambda, bc, apply_mask, done, part);
 	CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9)
 	CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
 #undef CASE
@@ -233,6 +289,259 @@ extern "C" int ssxk_dss(void *stream, int *const grp[3], const long long ng[3],
 				0, st, ng[2], grp[2], u);
 		LAUNCH_CHECK("k_dss<8>");
 	}
+	return SSX_OK;
+}
+
+// One launch for all three multiplicities: blocks [0,b2) take the pairs,
+// [b2,b2+b4) the quadruples, the rest the octuples.  Optionally each sum is
+// scaled by 1/M (direct_stiffness_averaging, global.c:259-268: the
+// multiplicity of every copy in a group of M is exactly M), and the
+// shared-node part of <p, DSS(ap)>_w is accumulated per block.
+template <int M>
+__device__ __forceinline__ double dss_group(long long G, long long g,
+		const int *__restrict__ idx, double *__restrict__ u,
+		const double *__restrict__ p, int average)
+{
+	int ix[M];
+	double v[M];
+#pragma unroll
+	for (int t = 0; t < M; ++t)
+		ix[t] = idx[(long long) t * G + g];
+#pragma unroll
+	for (int t = 0; t < M; ++t)
+		v[t] = u[ix[t]];
+	const double w = 1.0 / (double) M;
+	double pd = 0.0;
+#pragma unroll
+	for (int t = 0; t < M; ++t) {
+		double s = v[t];
+#pragma unroll
+		for (int k = 1; k < M; ++k)
+			s = s + v[t ^ dss_ord(M, k)];
+		if (average)
+			s = s * w;
+		u[ix[t]] = s;
+		if (p)
+			pd += (p[ix[t]] * w) * s;
+	}
+	return pd;
+}
+
+__global__ void __launch_bounds__(256)
+k_dss_all(long long G2, long long G4, long long G8, const int *__restrict__ i2,
+		const int *__restrict__ i4, const int *__restrict__ i8,
+		double *__restrict__ u, const double *__restrict__ p,
+		double *__restrict__ part, int average, const int *__restrict__ done)
+{
+	if (done && *done)
+		return;
+	const long long b2 = (G2 + 255) / 256, b4 = (G4 + 255) / 256;
+	long long b = blockIdx.x;
+	double pd = 0.0;
+	if (b < b2) {
+		long long g = b * 256 + threadIdx.x;
+		if (g < G2)
+			pd = dss_group<2>(G2, g, i2, u, p, average);
+	} else if (b < b2 + b4) {
+		long long g = (b - b2) * 256 + threadIdx.x;
+		if (g < G4)
+			pd = dss_group<4>(G4, g, i4, u, p, average);
+	} else {
+		long long g = (b - b2 - b4) * 256 + threadIdx.x;
+		if (g < G8)
+			pd = dss_group<8>(G8, g, i8, u, p, average);
+	}
+	if (part) {
+		__shared__ double sh[256];
+		sh[threadIdx.x] = pd;
+		__syncthreads();
+#pragma unroll
+		for (int st = 128; st > 0; st >>= 1) {
+			if ((int) threadIdx.x < st)
+				sh[threadIdx.x] += sh[threadIdx.x + st];
+			__syncthreads();
+		}
+		if (threadIdx.x == 0)
+			part[blockIdx.x] = sh[0];
+	}
+}
+
+extern "C" long long ssxk_dss_blocks(const long long ng[3])
+{
+	return (ng[0] + 255) / 256 + (ng[1] + 255) / 256 + (ng[2] + 255) / 256;
+}
+
+extern "C" int ssxk_dss_all(void *stream, int *const grp[3],
+		const long long ng[3], double *u, const double *p, double *part,
+		int average, const int *done)
+{
+	long long nb = ssxk_dss_blocks(ng);
+	if (nb == 0)
+		return SSX_OK;
+	hipLaunchKernelGGL(k_dss_all, dim3((unsigned) nb), dim3(256), 0,
+			(hipStream_t) stream, ng[0], ng[1], ng[2], grp[0], grp[1],
+			grp[2], u, p, part, average, done);
+	LAUNCH_CHECK("k_dss_all");
+	return SSX_OK;
+}
+
+// --- fused CG iteration kernels (solver.c:58-74) -------------------------
+
+__device__ __forceinline__ void block_sum4(double *v, double (*sh)[256])
+{
+#pragma unroll
+	for (int c = 0; c < 4; ++c)
+		sh[c][threadIdx.x] = v[c];
+	__syncthreads();
+#pragma unroll
+	for (int st = 128; st > 0; st >>= 1) {
+		if ((int) threadIdx.x < st) {
+#pragma unroll
+			for (int c = 0; c < 4; ++c)
+				sh[c][threadIdx.x] += sh[c][threadIdx.x + st];
+		}
+		__syncthreads();
+	}
+}
+
+// u += a p; r -= a ap; z = dinv r; partial <r,z>_w, <r,r>_w and, for the
+// pressure null space, <z,1>_w and <r,1>_w
+__global__ void __launch_bounds__(256)
+k_cg_update(size_t n, const double *__restrict__ scal, int ia,
+		const double *__restrict__ p, const double *__restrict__ ap,
+		double *__restrict__ u, double *__restrict__ r,
+		double *__restrict__ z, const double *__restrict__ dinv,
+		const double *__restrict__ scale, int pressure,
+		double *__restrict__ part, const int *__restrict__ done)
+{
+	if (done && *done)
+		return;
+	__shared__ double sh[4][256];
+	const double a = scal[ia];
+	double acc[4] = { 0.0, 0.0, 0.0, 0.0 };
+	const size_t stride = (size_t) gridDim.x * blockDim.x;
+	for (size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x; q < n;
+			q += stride) {
+		u[q] = a * p[q] + u[q];
+		const double rv = -a * ap[q] + r[q];
+		r[q] = rv;
+		const double zv = rv * dinv[q];
+		z[q] = zv;
+		const double w = scale[q];
+		acc[0] += (rv * w) * zv;
+		acc[1] += (rv * w) * rv;
+		if (pressure) {
+			acc[2] += zv * w;
+			acc[3] += rv * w;
+		}
+	}
+	block_sum4(acc, sh);
+	if (threadIdx.x < 4)
+		part[(size_t) threadIdx.x * SSX_NPART + blockIdx.x] = sh[threadIdx.x][0];
+}
+
+// p = (z - c) + beta p, masked (c: pressure null-space constant)
+__global__ void __launch_bounds__(256)
+k_cg_pupdate(size_t n, const double *__restrict__ z,
+		double *__restrict__ p, const double *__restrict__ scal, int ib,
+		int ic, const double *__restrict__ mask,
+		const int *__restrict__ done)
+{
+	if (done && *done)
+		return;
+	const double b = scal[ib];
+	const double c = ic >= 0 ? -scal[ic] : 0.0;
+	GRID_STRIDE_LOOP(q, n) {
+		const double zv = ic >= 0 ? c * 1.0 + z[q] : z[q];
+		double v = b * p[q] + zv;
+		p[q] = mask ? v * mask[q] : v;
+	}
+}
+
+// single-block reductions + scalar logic of one CG iteration
+//   mode 0: s[o+PAP] = sum part[0..cnt); s[o+ALPHA] = s[o+RDZ] / pap
+//   mode 1: rz, rr (, zs, rs) from the 4 update partials; null-space
+//           constant c = zs/nunique; rdz = rz - c rs; beta = rdz / rdz_old;
+//           iteration count ++, done when sqrt(rr) <= eps or count >= maxit
+__global__ void __launch_bounds__(1024)
+k_cg_reduce(int mode, const double *__restrict__ part, long long cnt,
+		double *__restrict__ scal, int o, int *__restrict__ flags,
+		double eps, int maxit, double nunique, int pressure)
+{
+	if (flags[0])
+		return;
+	__shared__ double sh[4][1024];
+	const int nq = mode == 0 ? 1 : (pressure ? 4 : 2);
+	for (int c = 0; c < 4; ++c) {
+		double acc = 0.0;
+		if (c < nq) {
+			const double *pp = mode == 0 ? part
+				: part + (size_t) c * SSX_NPART;
+			const long long m = mode == 0 ? cnt : SSX_NPART;
+			for (long long q = threadIdx.x; q < m; q += 1024)
+				acc += pp[q];
+		}
+		sh[c][threadIdx.x] = acc;
+	}
+	__syncthreads();
+	for (int st = 512; st > 0; st >>= 1) {
+		if ((int) threadIdx.x < st)
+			for (int c = 0; c < 4; ++c)
+				sh[c][threadIdx.x] += sh[c][threadIdx.x + st];
+		__syncthreads();
+	}
+	if (threadIdx.x != 0)
+		return;
+	if (mode == 0) {
+		scal[o + CG_PAP] = sh[0][0];
+		scal[o + CG_ALPHA] = scal[o + CG_RDZ] / sh[0][0];
+		return;
+	}
+	double rz = sh[0][0], rr = sh[1][0], c = 0.0;
+	if (pressure) {
+		c = sh[2][0] / nunique;
+		rz = rz - c * sh[3][0];
+	}
+	scal[o + CG_NULLC] = c;
+	scal[o + CG_BETA] = rz / scal[o + CG_RDZ];
+	scal[o + CG_RDZ] = rz;
+	scal[o + CG_RR] = rr;
+	const int it = flags[1] + 1;
+	flags[1] = it;
+	if (sqrt(rr) <= eps || it >= maxit || !(rr == rr))
+		flags[0] = 1;
+}
+
+extern "C" int ssxk_cg_update(void *stream, size_t n, const double *scal,
+		int ia, const double *p, const double *ap, double *u, double *r,
+		double *z, const double *dinv, const double *scale, int pressure,
+		double *part, const int *done)
+{
+	hipLaunchKernelGGL(k_cg_update, dim3(SSX_NPART), dim3(256), 0,
+			(hipStream_t) stream, n, scal, ia, p, ap, u, r, z, dinv, scale,
+			pressure, part, done);
+	LAUNCH_CHECK("k_cg_update");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_cg_pupdate(void *stream, size_t n, const double *z,
+		double *p, const double *scal, int ib, int ic, const double *mask,
+		const int *done)
+{
+	hipLaunchKernelGGL(k_cg_pupdate, vgrid(n), dim3(256), 0,
+			(hipStream_t) stream, n, z, p, scal, ib, ic, mask, done);
+	LAUNCH_CHECK("k_cg_pupdate");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_cg_reduce(void *stream, int mode, const double *part,
+		long long cnt, double *scal, int o, int *flags, double eps,
+		int maxit, double nunique, int pressure)
+{
+	hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(1024), 0,
+			(hipStream_t) stream, mode, part, cnt, scal, o, flags, eps, maxit,
+			nunique, pressure);
+	LAUNCH_CHECK("k_cg_reduce");
 	return SSX_OK;
 }
 
@@ -440,19 +749,6 @@ extern "C" int ssxk_scalar(void *stream, double *scal, int op, int a, int b,
 // Vector kernels
 // ---------------------------------------------------------------------
 
-static inline dim3 vgrid(size_t n)
-{
-	size_t b = (n + 255) / 256;
-	if (b > 16384)
-		b = 16384;
-	if (b == 0)
-		b = 1;
-	return dim3((unsigned) b);
-}
-
-#define GRID_STRIDE(q, n) \
-	for (size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x; q < (n); \
-			q += (size_t) gridDim.x * blockDim.x)
 
 __global__ void k_axpby(size_t n, double a, const double *__restrict__ x,
 		const double *__restrict__ w, double b, double *__restrict__ y)
@@ -536,6 +832,20 @@ extern "C" int ssxk_shift_s(void *stream, size_t n, const double *scal,
 	hipLaunchKernelGGL(k_shift_s, vgrid(n), dim3(256), 0,
 			(hipStream_t) stream, n, scal, ia, nunique, y);
 	LAUNCH_CHECK("k_shift_s");
+	return SSX_OK;
+}
+
+__global__ void k_set_ints(int *dst, int v, int n)
+{
+	if ((int) threadIdx.x < n)
+		dst[threadIdx.x] = v;
+}
+
+extern "C" int ssxk_set_ints(void *stream, int *dst, int v, int n)
+{
+	hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, (hipStream_t) stream,
+			dst, v, n);
+	LAUNCH_CHECK("k_set_ints");
 	return SSX_OK;
 }
 

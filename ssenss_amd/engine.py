@@ -58,9 +58,14 @@ class Context:
         return self._lib.ssx_lambda(self.h)
 
     def set_stream(self, stream):
-        """stream: torch.cuda.Stream (or None for the context's own)"""
-        s = None if stream is None else C.c_void_p(stream.cuda_stream)
-        check(self._lib.ssx_set_stream(self.h, s))
+        """Order the library's work on a torch.cuda.Stream (torch's default
+        stream is the HIP null stream, handle 0); None returns to the
+        context's own non-blocking stream."""
+        if stream is None:
+            s = self._lib.ssx_own_stream(self.h)
+        else:
+            s = stream.cuda_stream
+        check(self._lib.ssx_set_stream(self.h, C.c_void_p(s) if s else None))
 
     def stream_handle(self):
         return self._lib.ssx_get_stream(self.h)
@@ -82,6 +87,9 @@ class Context:
 
     def set_diag_choice(self, choice):
         check(self._lib.ssx_set_diag_choice(self.h, choice))
+
+    def set_option(self, option, value):
+        check(self._lib.ssx_set_option(self.h, option, int(value)))
 
     # --- operators (torch device tensors) ---
     def ax(self, level, op, u, au):

@@ -7,6 +7,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 PRESSURE, X_VELOCITY, Y_VELOCITY, Z_VELOCITY = 0, 1, 2, 3
 OP_POISSON, OP_HELMHOLTZ = 0, 1
 PC_JACOBI, PC_MG = 0, 1
+OPT_FUSED_CG = 1
 (DATA_GFAC, DATA_MASS, DATA_DIAG_P, DATA_DIAG_H, DATA_SCALE, DATA_MASK,
  DATA_INVMASS, DATA_X, DATA_Y, DATA_Z) = range(10)
 
@@ -17,9 +18,10 @@ ERRORS = {1: "EINVAL", 2: "ENOMEM", 3: "EHIP", 4: "ENOTCONV", 5: "EMESH",
 EXPORTED_SYMBOLS = [
     "ssx_last_error", "ssx_version", "ssx_mesh_box", "ssx_dss_host",
     "ssx_geometry_host", "ssx_create",
-    "ssx_destroy", "ssx_set_stream", "ssx_get_stream", "ssx_synchronize",
+    "ssx_destroy", "ssx_set_stream", "ssx_own_stream", "ssx_get_stream", "ssx_synchronize",
     "ssx_nlevels", "ssx_level_n", "ssx_nel", "ssx_level_ntot", "ssx_lambda",
-    "ssx_level_data", "ssx_dss_groups", "ssx_set_diag_choice", "ssx_ax",
+    "ssx_level_data", "ssx_dss_groups", "ssx_set_diag_choice",
+    "ssx_set_option", "ssx_ax",
     "ssx_opeval", "ssx_dss", "ssx_dss_average", "ssx_dot", "ssx_mask",
     "ssx_nullspace", "ssx_restrict", "ssx_prolong", "ssx_jacobi", "ssx_cg",
     "ssx_fcg", "ssx_mg_precon", "ssx_solve_poisson_cg",
@@ -89,6 +91,7 @@ def load():
         "ssx_destroy": (None, [vp]),
         "ssx_set_stream": (i, [vp, vp]),
         "ssx_get_stream": (vp, [vp]),
+        "ssx_own_stream": (vp, [vp]),
         "ssx_synchronize": (i, [vp]),
         "ssx_nlevels": (i, [vp]),
         "ssx_level_n": (i, [vp, i]),
@@ -98,6 +101,7 @@ def load():
         "ssx_level_data": (i, [vp, i, i, dp]),
         "ssx_dss_groups": (i, [vp, i, C.POINTER(C.c_longlong)]),
         "ssx_set_diag_choice": (i, [vp, i]),
+        "ssx_set_option": (i, [vp, i, i]),
         "ssx_ax": (i, [vp, i, i, vp, vp]),
         "ssx_opeval": (i, [vp, i, i, i, vp, vp]),
         "ssx_dss": (i, [vp, i, vp]),
