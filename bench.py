@@ -165,33 +165,52 @@ def main():
                 "kernel": f"k_ax<{args.n},0> (FP64 Poisson Ax)",
                 "bytes_per_launch": BYTES_PER_NODE_POISSON * nt}
 
-    solve = None
+    solves = {}
     if not args.no_solve:
-        x = torch.from_numpy(mesh.x.ravel()).cuda()
         f = smooth_source(mesh.x.ravel(), mesh.y.ravel(), mesh.z.ravel())
         massv = torch.from_numpy(ctx.level_data(0, S.DATA_MASS)).cuda()
-        b = torch.from_numpy(f).cuda() * massv
-        ctx.dss(0, b)
-        ctx.mask(0, S.X_VELOCITY, b)
-        bnorm = np.sqrt(ctx.dot(0, b, b))
+        b0 = torch.from_numpy(f).cuda() * massv
+        ctx.dss(0, b0)
+        ctx.mask(0, S.X_VELOCITY, b0)
+        bnorm = np.sqrt(ctx.dot(0, b0, b0))
         eps = 1e-10 * bnorm
-        sol = torch.zeros_like(b)
-        ctx.set_diag_choice(0)
-        torch.cuda.synchronize()
-        log("[bench] FCG + p-MG solve")
-        t0 = time.perf_counter()
-        st = ctx.fcg(S.OP_POISSON, S.X_VELOCITY, S.PC_MG, b, sol, eps,
-                     args.max_iter, allow_notconv=True)
-        torch.cuda.synchronize()
-        tsolve = time.perf_counter() - t0
-        solve = {"time_to_residual_s": tsolve, "iterations": st.iterations,
-                 "converged": bool(st.converged),
-                 "final_residual_rel": st.residual / bnorm,
-                 "tolerance_rel": 1e-10, "precision": "fp64",
-                 "fine_ax": st.fine_ax, "coarse_cg_iters": st.coarse_iters,
-                 "smoother_cg_iters": st.smooth_iters, "vcycles": st.vcycles}
-        log(f"[bench] solve {tsolve:.2f} s, {st.iterations} its")
-        del x
+        ctx.set_diag_choice(0)       # Dirichlet Poisson: Poisson diagonal
+        ref_sol = None
+        for name, prec in (("fp64", S.PREC_FP64), ("fp32", S.PREC_FP32),
+                           ("bf16", S.PREC_BF16)):
+            ctx.set_option(S.OPT_SMOOTHER_PRECISION, prec)
+            if prec != S.PREC_FP64:   # build the reduced-precision copies
+                w = torch.zeros_like(b0)
+                ctx.mg_precon(S.OP_POISSON, S.X_VELOCITY, eps, b0.clone(), w)
+            b = b0.clone()
+            sol = torch.zeros_like(b)
+            torch.cuda.synchronize()
+            log(f"[bench] FCG + p-MG solve, smoother {name}")
+            t0 = time.perf_counter()
+            st = ctx.fcg(S.OP_POISSON, S.X_VELOCITY, S.PC_MG, b, sol, eps,
+                         args.max_iter, allow_notconv=True)
+            torch.cuda.synchronize()
+            tsolve = time.perf_counter() - t0
+            rec = {"time_to_residual_s": tsolve, "iterations": st.iterations,
+                   "converged": bool(st.converged),
+                   "final_residual_rel": st.residual / bnorm,
+                   "tolerance_rel": 1e-10,
+                   "smoother_precision": name,
+                   "outer_precision": "fp64",
+                   "fine_ax": st.fine_ax, "coarse_cg_iters": st.coarse_iters,
+                   "smoother_cg_iters": st.smooth_iters,
+                   "vcycles": st.vcycles}
+            if ref_sol is None:
+                ref_sol = sol
+            else:
+                d = sol - ref_sol
+                rec["error_vs_fp64_rel"] = float(np.sqrt(
+                    ctx.dot(0, d, d) / ctx.dot(0, ref_sol, ref_sol)))
+                rec["speedup_vs_fp64"] = (solves["fp64"]["time_to_residual_s"]
+                                          / tsolve)
+            solves[name] = rec
+            log(f"[bench] solve {name} {tsolve:.3f} s, {st.iterations} its")
+        ctx.set_option(S.OPT_SMOOTHER_PRECISION, S.PREC_FP64)
 
     cpu = None if args.no_cpu else cpu_baseline(mesh, args.cpu_seconds)
     out = {
@@ -215,7 +234,9 @@ def main():
         "helmholtz_ax_gdofs": nt / t_helm / 1e9,
         "dss_us": t_dss * 1e6,
         "opeval_gdofs": nt / t_op / 1e9,
-        "solve_fp64": solve,
+        "solve_fp64": solves.get("fp64"),
+        "solve_mixed_fp32": solves.get("fp32"),
+        "solve_mixed_bf16": solves.get("bf16"),
     }
     ctx.close()
     print(json.dumps(out), flush=True)

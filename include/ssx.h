@@ -161,6 +161,16 @@ int ssx_set_diag_choice(ssx_ctx *ctx, int choice);
  * kernels with the stopping test on the device; 0 runs the plain
  * one-kernel-per-vector-operation loop with a host check per iteration */
 #define SSX_OPT_FUSED_CG 1
+/* SSX_OPT_SMOOTHER_PRECISION: precision of the p-multigrid smoother and
+ * order transfer on all but the coarsest level: SSX_PREC_FP64 (default, the
+ * reference algorithm), SSX_PREC_FP32 (FP32 fields and factors) or
+ * SSX_PREC_BF16 (FP32 fields, bfloat16 geometric factors).  The outer FCG,
+ * its inner products, the residual update and the coarsest-level CG stay
+ * FP64; inner products always accumulate in FP64. */
+#define SSX_OPT_SMOOTHER_PRECISION 2
+#define SSX_PREC_FP64 0
+#define SSX_PREC_FP32 1
+#define SSX_PREC_BF16 2
 int ssx_set_option(ssx_ctx *ctx, int option, int value);
 
 /* ---- operators on device E-vectors ---- */

@@ -490,6 +490,12 @@ void ssx_destroy(ssx_ctx *c)
 		hipFree(L->r); hipFree(L->z); hipFree(L->p); hipFree(L->ap);
 		hipFree(L->mg_r); hipFree(L->mg_fn); hipFree(L->mg_e);
 		free(L->h_x); free(L->h_y); free(L->h_z);
+		hipFree(L->gf_lp); hipFree(L->mass_lp); hipFree(L->diagp_lp);
+		hipFree(L->diagh_lp); hipFree(L->scale_lp); hipFree(L->mask_lp);
+		hipFree(L->dmat_lp); hipFree(L->prol_lp);
+		hipFree(L->r_lp); hipFree(L->z_lp); hipFree(L->p_lp);
+		hipFree(L->ap_lp); hipFree(L->mgr_lp); hipFree(L->fn_lp);
+		hipFree(L->e_lp);
 	}
 	hipFree(c->inv_mass);
 	hipFree(c->cg_flags);
@@ -551,6 +557,14 @@ int ssx_set_option(ssx_ctx *c, int option, int value)
 	switch (option) {
 	case SSX_OPT_FUSED_CG:
 		c->fused = value != 0;
+		return SSX_OK;
+	case SSX_OPT_SMOOTHER_PRECISION:
+		if (value != SSX_PREC_FP64 && value != SSX_PREC_FP32
+				&& value != SSX_PREC_BF16) {
+			ssx_set_error("precision %d unsupported", value);
+			return SSX_EINVAL;
+		}
+		c->smooth_prec = value;
 		return SSX_OK;
 	default:
 		ssx_set_error("unknown option %d", option);
@@ -906,10 +920,216 @@ static int mg_recurse(ssx_ctx *c, int l, int op, int var, double *f,
 	return SSX_OK;
 }
 
+/* ------------------------------------------------------------------ */
+/* mixed precision: FP32 (or FP32 + BF16 factors) smoother and transfer */
+/* on every level but the coarsest, which stays FP64                    */
+/* ------------------------------------------------------------------ */
+
+static int falloc(float **p, size_t n)
+{
+	if (hipMalloc((void **) p, (n ? n : 1) * sizeof(float)) != hipSuccess) {
+		*p = NULL;
+		ssx_set_error("hipMalloc of %zu floats failed", n);
+		return SSX_ENOMEM;
+	}
+	return SSX_OK;
+}
+
+static int cvt_new(ssx_ctx *c, float **dst, const double *src, size_t n)
+{
+	if (!*dst)
+		TRY(falloc(dst, n));
+	return ssxk_cvt(c->stream, SSX_PREC_FP64, SSX_PREC_FP32, src, *dst, n);
+}
+
+static int build_lp(ssx_ctx *c, int prec)
+{
+	for (int l = 0; l < c->nlev; ++l)
+		if (!c->lev[l].fused_ok) {
+			ssx_set_error("level %d: reduced-precision smoother needs the "
+					"fused CG path", l);
+			return SSX_EMESH;
+		}
+	for (int l = 0; l < c->nlev; ++l) {
+		ssx_level *L = &c->lev[l];
+		const size_t nt = L->ntot;
+		if (!L->fn_lp)
+			TRY(falloc(&L->fn_lp, nt));
+		if (!L->e_lp)
+			TRY(falloc(&L->e_lp, nt));
+		if (l == c->nlev - 1)
+			continue;                       /* coarsest: FP64 */
+		if (L->lp_prec != prec) {
+			hipFree(L->gf_lp);
+			L->gf_lp = NULL;
+			const size_t bytes = 6 * nt * (prec == SSX_PREC_BF16 ? 2 : 4);
+			if (hipMalloc(&L->gf_lp, bytes) != hipSuccess) {
+				ssx_set_error("hipMalloc low-precision factors");
+				return SSX_ENOMEM;
+			}
+			TRY(ssxk_cvt(c->stream, SSX_PREC_FP64, prec, L->gf, L->gf_lp,
+					6 * nt));
+		}
+		TRY(cvt_new(c, &L->mass_lp, L->mass, nt));
+		TRY(cvt_new(c, &L->diagp_lp, L->diag_p, nt));
+		TRY(cvt_new(c, &L->diagh_lp, L->diag_h, nt));
+		TRY(cvt_new(c, &L->scale_lp, L->scale, nt));
+		TRY(cvt_new(c, &L->mask_lp, L->mask, nt));
+		TRY(cvt_new(c, &L->dmat_lp, L->dmat, 2 * (size_t) L->n * L->n));
+		TRY(cvt_new(c, &L->prol_lp, L->prol,
+				(size_t) c->lev[l + 1].n * L->n));
+		if (!L->r_lp) {
+			TRY(falloc(&L->r_lp, nt));
+			TRY(falloc(&L->z_lp, nt));
+			TRY(falloc(&L->p_lp, nt));
+			TRY(falloc(&L->ap_lp, nt));
+			TRY(falloc(&L->mgr_lp, nt));
+		}
+		L->lp_prec = prec;
+	}
+	return SSX_OK;
+}
+
+static const float *jdiag_lp(ssx_ctx *c, int l, int var)
+{
+	int ch = c->diag_choice;
+	if (ch < 0)
+		ch = var == SSX_PRESSURE ? 0 : 1;
+	return ch == 0 ? c->lev[l].diagp_lp : c->lev[l].diagh_lp;
+}
+
+static int opeval_lp(ssx_ctx *c, int l, int op, int var, const float *p,
+		float *ap)
+{
+	ssx_level *L = &c->lev[l];
+	if (l == 0)
+		c->cnt[0]++;
+	TRY(ssxk_ax_lp(c->stream, L->lp_prec, L->n, c->nel, op, p, ap, L->gf_lp,
+			L->mass_lp, L->dmat_lp, c->lambda, L->bc, var != SSX_PRESSURE,
+			NULL, NULL));
+	return ssxk_dss_all_lp(c->stream, L->grp, L->ngrp, ap, NULL, NULL, 0,
+			NULL);
+}
+
+static int nullspace_lp(ssx_ctx *c, int l, int var, float *u)
+{
+	if (var != SSX_PRESSURE)
+		return SSX_OK;
+	ssx_level *L = &c->lev[l];
+	TRY(ssxk_dot_lp(c->stream, L->ntot, u, L->scale_lp, NULL, c->part, 0));
+	TRY(ssxk_reduce(c->stream, c->part, 0, c->scal, SC_NULLC));
+	return ssxk_shift_s_lp(c->stream, L->ntot, c->scal, SC_NULLC,
+			L->nunique, u);
+}
+
+/* conjugate_gradient (solver.c:33-81) in FP32 storage: the fused six-kernel
+ * iteration of cg_level_fused with FP64 scalars and inner products */
+static int cg_level_lp(ssx_ctx *c, int l, int op, int var, float *f,
+		float *u, double eps, int max_iter, int *iters)
+{
+	ssx_level *L = &c->lev[l];
+	const size_t nt = L->ntot;
+	float *r = L->r_lp, *z = L->z_lp, *p = L->p_lp, *ap = L->ap_lp;
+	void *st = c->stream;
+	const int pres = var == SSX_PRESSURE;
+	const float *mask = pres ? NULL : L->mask_lp;
+	const float *dinv = jdiag_lp(c, l, var);
+	const int o = SC_CG(l);
+	int *flags = c->cg_flags + 2 * l;
+	if (mask)
+		TRY(ssxk_mul_lp(st, nt, f, mask, f));
+	TRY(opeval_lp(c, l, op, var, u, r));
+	TRY(ssxk_axpby_lp(st, nt, 1.0, f, NULL, -1.0, r));
+	TRY(nullspace_lp(c, l, var, r));
+	TRY(ssxk_mul_lp(st, nt, r, dinv, z));
+	TRY(nullspace_lp(c, l, var, z));
+	TRY(ssxk_copy_lp(st, p, z, nt));
+	TRY(ssxk_dot_lp(st, nt, r, L->scale_lp, z, c->part, 0));
+	TRY(ssxk_reduce(st, c->part, 0, c->scal, o + CG_RDZ));
+	TRY(ssxk_set_ints(st, flags, 0, 2));
+	const long long nba = ssxk_ax_blocks(L->n, c->nel);
+	for (int k = 0; k < max_iter; ++k) {
+		TRY(ssxk_ax_lp(st, L->lp_prec, L->n, c->nel, op, p, ap, L->gf_lp,
+				L->mass_lp, L->dmat_lp, c->lambda, L->bc, !pres, flags,
+				c->part));
+		TRY(ssxk_dss_all_lp(st, L->grp, L->ngrp, ap, p, c->part + nba, 0,
+				flags));
+		TRY(ssxk_cg_reduce(st, 0, c->part, nba + ssxk_dss_blocks(L->ngrp),
+				c->scal, o, flags, eps, max_iter, L->nunique, pres));
+		TRY(ssxk_cg_update_lp(st, nt, c->scal, o + CG_ALPHA, p, ap, u, r, z,
+				dinv, L->scale_lp, pres, c->part, flags));
+		TRY(ssxk_cg_reduce(st, 1, c->part, 0, c->scal, o, flags, eps,
+				max_iter, L->nunique, pres));
+		TRY(ssxk_cg_pupdate_lp(st, nt, z, p, c->scal, o + CG_BETA,
+				pres ? o + CG_NULLC : -1, mask, flags));
+	}
+	if (iters) {
+		hipStream_t hs = (hipStream_t) st;
+		HIPC(hipMemcpyAsync(c->h_flags, flags, 2 * sizeof(int),
+				hipMemcpyDeviceToHost, hs));
+		HIPC(hipStreamSynchronize(hs));
+		*iters = c->h_flags[1];
+	}
+	return SSX_OK;
+}
+
+/* V-cycle of precon.c:238-271 with the level-l problem held in
+ * lev[l].fn_lp (right-hand side) and lev[l].e_lp (solution) */
+static int mg_recurse_lp(ssx_ctx *c, int l, int op, int var, double eps)
+{
+	ssx_level *L = &c->lev[l];
+	void *st = c->stream;
+	if (l == c->nlev - 1) {          /* coarsest level in FP64 */
+		double err;
+		int it;
+		TRY(ssxk_cvt(st, SSX_PREC_FP32, SSX_PREC_FP64, L->fn_lp, L->mg_fn,
+				L->ntot));
+		TRY(ssxk_fill(st, L->mg_e, 0.0, L->ntot));
+		TRY(cg_level(c, l, op, var, L->mg_fn, L->mg_e, eps, 500, 1, &it,
+				&err));
+		c->cnt[1] += it;
+		return ssxk_cvt(st, SSX_PREC_FP64, SSX_PREC_FP32, L->mg_e, L->e_lp,
+				L->ntot);
+	}
+	ssx_level *C = &c->lev[l + 1];
+	float *r = L->mgr_lp;
+	if (l == 0) {
+		TRY(opeval_lp(c, l, op, var, L->e_lp, r));
+		TRY(ssxk_axpby_lp(st, L->ntot, 1.0, L->fn_lp, NULL, -1.0, r));
+	} else {
+		TRY(ssxk_copy_lp(st, r, L->fn_lp, L->ntot));
+	}
+	TRY(ssxk_restrict_lp(st, L->n, C->n, c->nel, L->prol_lp, r, C->fn_lp));
+	TRY(ssxk_dss_all_lp(st, C->grp, C->ngrp, C->fn_lp, NULL, NULL, 1, NULL));
+	TRY(ssxk_fill_lp(st, C->e_lp, 0.0, C->ntot));
+	TRY(mg_recurse_lp(c, l + 1, op, var, eps));
+	TRY(ssxk_prolong_lp(st, L->n, C->n, c->nel, L->prol_lp, C->e_lp,
+			L->e_lp, 1));
+	int it = 15;
+	TRY(cg_level_lp(c, l, op, var, L->fn_lp, L->e_lp, eps, 15, NULL));
+	c->cnt[2] += it;
+	return SSX_OK;
+}
+
+static int mg_precon_lp(ssx_ctx *c, int op, int var, double eps,
+		const double *x, double *px)
+{
+	TRY(build_lp(c, c->smooth_prec));
+	ssx_level *L = &c->lev[0];
+	void *st = c->stream;
+	c->cnt[3]++;
+	TRY(ssxk_cvt(st, SSX_PREC_FP64, SSX_PREC_FP32, x, L->fn_lp, L->ntot));
+	TRY(ssxk_mul_lp(st, L->ntot, L->fn_lp, jdiag_lp(c, 0, var), L->e_lp));
+	TRY(mg_recurse_lp(c, 0, op, var, eps));
+	return ssxk_cvt(st, SSX_PREC_FP32, SSX_PREC_FP64, L->e_lp, px, L->ntot);
+}
+
 int ssx_mg_precon(ssx_ctx *c, int op, int var, double eps, const double *x,
 		double *px)
 {
 	TRY(chk(c, 0));
+	if (c->smooth_prec != SSX_PREC_FP64)
+		return mg_precon_lp(c, op, var, eps, x, px);
 	c->cnt[3]++;
 	TRY(ssx_jacobi(c, 0, var, x, px));      /* precon.c:230 */
 	return mg_recurse(c, 0, op, var, (double *) x, px, eps);

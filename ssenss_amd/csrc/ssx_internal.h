@@ -63,6 +63,13 @@ typedef struct {
 	double *r, *z, *p, *ap, *mg_r, *mg_fn, *mg_e;
 	/* host copies kept for ssx_level_data */
 	double *h_x, *h_y, *h_z;
+	/* low-precision smoother copies (FP32 fields; factors FP32 or BF16),
+	 * built on first use of SSX_OPT_SMOOTHER_PRECISION != FP64 */
+	int lp_prec;                 /* precision of gf_lp, 0 = not built     */
+	void *gf_lp;
+	float *mass_lp, *diagp_lp, *diagh_lp, *scale_lp, *mask_lp, *dmat_lp;
+	float *prol_lp;
+	float *r_lp, *z_lp, *p_lp, *ap_lp, *mgr_lp, *fn_lp, *e_lp;
 } ssx_level;
 
 struct ssx_ctx {
@@ -72,6 +79,7 @@ struct ssx_ctx {
 	double lambda;
 	int diag_choice;
 	int fused;                   /* SSX_OPT_FUSED_CG                     */
+	int smooth_prec;             /* SSX_OPT_SMOOTHER_PRECISION           */
 	double *inv_mass;            /* device, level 0 */
 	void *stream;                /* hipStream_t in use */
 	void *own_stream;
@@ -127,8 +135,38 @@ int ssxk_cg_reduce(void *stream, int mode, const double *part, long long cnt,
 		double *scal, int o, int *flags, double eps, int maxit,
 		double nunique, int pressure);
 int ssxk_set_ints(void *stream, int *dst, int v, int n);
-int ssxk_dss(void *stream, int *const grp[3], const long long ng[3],
-		double *u);
+/* low-precision (FP32 fields) variants; prec selects the factor type of
+ * ssxk_ax_lp (SSX_PREC_FP32 or SSX_PREC_BF16) */
+int ssxk_ax_lp(void *stream, int prec, int n, int nel, int op, const void *u,
+		void *au, const void *gf, const void *mass, const void *dmat,
+		double lambda, const int *bc, int apply_mask, const int *done,
+		double *part);
+int ssxk_dss_all_lp(void *stream, int *const grp[3], const long long ng[3],
+		void *u, const void *p, double *part, int average, const int *done);
+int ssxk_cg_update_lp(void *stream, size_t n, const double *scal, int ia,
+		const void *p, const void *ap, void *u, void *r, void *z,
+		const void *dinv, const void *scale, int pressure, double *part,
+		const int *done);
+int ssxk_cg_pupdate_lp(void *stream, size_t n, const void *z, void *p,
+		const double *scal, int ib, int ic, const void *mask,
+		const int *done);
+int ssxk_restrict_lp(void *stream, int nf, int nc, int nel, const void *prol,
+		const void *fine, void *coarse);
+int ssxk_prolong_lp(void *stream, int nf, int nc, int nel, const void *prol,
+		const void *coarse, void *fine, int accumulate);
+int ssxk_axpby_lp(void *stream, size_t n, double a, const void *x,
+		const void *w, double b, void *y);
+int ssxk_mul_lp(void *stream, size_t n, const void *x, const void *w,
+		void *z);
+int ssxk_dot_lp(void *stream, size_t n, const void *x, const void *s,
+		const void *y, double *part, int slot);
+int ssxk_shift_s_lp(void *stream, size_t n, const double *scal, int ia,
+		double nunique, void *y);
+int ssxk_fill_lp(void *stream, void *dst, double v, size_t n);
+int ssxk_copy_lp(void *stream, void *dst, const void *src, size_t n);
+/* precision conversion (SSX_PREC_FP64 -> FP32 / BF16, FP32 -> FP64) */
+int ssxk_cvt(void *stream, int from, int to, const void *x, void *y,
+		size_t n);
 int ssxk_restrict(void *stream, int nf, int nc, int nel, const double *prol,
 		const double *fine, double *coarse);
 int ssxk_prolong(void *stream, int nf, int nc, int nel, const double *prol,

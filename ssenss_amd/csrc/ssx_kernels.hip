@@ -1,24 +1,29 @@
 // ssx_kernels.hip -- gfx950 kernels of the ssx elliptic-solve engine.
 //
-// Layout: E-vectors u[e][k][j][i] (i fastest), FP64.  Geometric factors are
+// Layout: E-vectors u[e][k][j][i] (i fastest).  Geometric factors are
 // premultiplied G~_ab = G_ab * w * detJ, stored per element as six
 // contiguous n^3 blocks [e][xx,xy,xz,yy,yz,zz][node] so that every factor
 // streams from HBM in fully coalesced runs.
 //
+// Precisions (template parameters T = field type, G = factor type):
+//   double/double  the FP64 path (outer solver, coarsest level, parity)
+//   float/float    FP32 multigrid smoother and order transfer
+//   float/bf16     FP32 fields with bfloat16 geometric factors
+// Inner products always accumulate in FP64.
+//
 // Kernels:
-//   k_ax<N,OP>      Poisson / Helmholtz operator, one "team" of N*N lanes per
+//   k_ax<N,OP,T,G>  Poisson / Helmholtz operator, one "team" of N*N lanes per
 //                   element (lane = (i,j) column, loop over the k planes),
-//                   planes of u and of the two in-plane fluxes staged in LDS,
+//                   the element and the two in-plane fluxes staged in LDS,
 //                   the zeta contraction kept in registers.  HBM-bound:
-//                   64 B per local node (Poisson), 72 B (Helmholtz).
-//   k_dss<M>        direct stiffness summation over shared-node groups of M
-//                   copies; each copy is replaced by own + partners in the
-//                   reference's add order, so the result is bit-identical to
-//                   the reference's pointer-wired DSS.  No atomics.
+//                   FP64 64 B per local node (Poisson), 72 B (Helmholtz).
+//   k_dss_all<T>    direct stiffness summation over shared-node groups of
+//                   M = 2, 4, 8 copies; each copy becomes own + partners in
+//                   the reference's add order (bit-identical result).
+//   k_cg_*          fused CG iteration with device-resident scalars and a
+//                   device-side stopping test.
 //   k_restrict / k_prolong   tensor-product order transfer per element.
 //   k_dot / k_reduce         deterministic two-pass weighted inner product.
-//   vector kernels           axpy family with device-resident scalars, so a
-//                            CG iteration never waits on the host.
 #include <hip/hip_runtime.h>
 #include "ssx_internal.h"
 
@@ -30,6 +35,8 @@
 			return SSX_EHIP;                                     \
 		}                                                            \
 	} while (0)
+
+typedef unsigned short bf16_t;
 
 static inline dim3 vgrid(size_t n)
 {
@@ -44,7 +51,14 @@ static inline dim3 vgrid(size_t n)
 #define GRID_STRIDE(q, n) \
 	for (size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x; q < (n); \
 			q += (size_t) gridDim.x * blockDim.x)
-#define GRID_STRIDE_LOOP GRID_STRIDE
+
+// factor loads: bf16 is the upper half of an IEEE float
+__device__ __forceinline__ double ldg(const double *p, size_t i) { return p[i]; }
+__device__ __forceinline__ float ldg(const float *p, size_t i) { return p[i]; }
+__device__ __forceinline__ float ldg(const bf16_t *p, size_t i)
+{
+	return __uint_as_float(((unsigned) p[i]) << 16);
+}
 
 // ---------------------------------------------------------------------
 // Operator
@@ -59,11 +73,11 @@ struct AxShape {
 	static constexpr int BLOCK = TEAMS * N2;
 };
 
-template <int N, int OP>
+template <int N, int OP, typename T, typename G>
 __global__ void __launch_bounds__(AxShape<N>::BLOCK)
-k_ax(int nel, const double *__restrict__ u, double *__restrict__ au,
-		const double *__restrict__ gf, const double *__restrict__ mass,
-		const double *__restrict__ dmat, double lambda,
+k_ax(int nel, const T *__restrict__ u, T *__restrict__ au,
+		const G *__restrict__ gf, const T *__restrict__ mass,
+		const T *__restrict__ dmat, T lambda,
 		const int *__restrict__ bc, int apply_mask,
 		const int *__restrict__ done, double *__restrict__ part)
 {
@@ -71,10 +85,10 @@ k_ax(int nel, const double *__restrict__ u, double *__restrict__ au,
 	constexpr int N2 = S::N2, N3 = S::N3, P = S::P, TEAMS = S::TEAMS;
 	if (done && *done)          // solver already converged: no-op
 		return;
-	__shared__ double s_d[2 * N2];
-	__shared__ double s_u[TEAMS][N * N * P];
-	__shared__ double s_a[TEAMS][N * P];
-	__shared__ double s_b[TEAMS][N * P];
+	__shared__ T s_d[2 * N2];
+	__shared__ T s_u[TEAMS][N * N * P];
+	__shared__ T s_a[TEAMS][N * P];
+	__shared__ T s_b[TEAMS][N * P];
 
 	const int team = threadIdx.x / N2;
 	const int tid = threadIdx.x - team * N2;
@@ -86,22 +100,22 @@ k_ax(int nel, const double *__restrict__ u, double *__restrict__ au,
 		s_d[q] = dmat[q];
 	// D_il at [i*N+l]; the SVV matrix enters the xi-forward and the
 	// eta/zeta-transpose products only (helmholtz.c:41-44, deriv.c:29-66)
-	const double *gDn = dmat;
-	const double *gDs = dmat + (OP == 1 ? N2 : 0);
+	const T *gDn = dmat;
+	const T *gDs = dmat + (OP == 1 ? N2 : 0);
 
 	const size_t eoff = (size_t) (act ? e : 0) * N3;
-	double *su = s_u[team], *sa = s_a[team], *sb = s_b[team];
-	double uc[N], acc[N];
+	T *su = s_u[team], *sa = s_a[team], *sb = s_b[team];
+	T uc[N], acc[N];
 #pragma unroll
 	for (int k = 0; k < N; ++k) {
-		uc[k] = act ? u[eoff + k * N2 + tid] : 0.0;
+		uc[k] = act ? u[eoff + k * N2 + tid] : T(0);
 		su[(k * N + j) * P + i] = uc[k];
-		acc[k] = 0.0;
+		acc[k] = T(0);
 	}
 	__syncthreads();
-	const double *Dn = s_d;
-	const double *Ds = s_d + (OP == 1 ? N2 : 0);
-	double dxf[N], dyf[N], dxt[N], dyt[N];
+	const T *Dn = s_d;
+	const T *Ds = s_d + (OP == 1 ? N2 : 0);
+	T dxf[N], dyf[N], dxt[N], dyt[N];
 #pragma unroll
 	for (int l = 0; l < N; ++l) {
 		dxf[l] = Ds[i * N + l];
@@ -109,24 +123,24 @@ k_ax(int nel, const double *__restrict__ u, double *__restrict__ au,
 		dxt[l] = Dn[l * N + i];
 		dyt[l] = Ds[l * N + j];
 	}
-	const double *g = gf + (size_t) (act ? e : 0) * 6 * N3;
+	const G *g = gf + (size_t) (act ? e : 0) * 6 * N3;
 
 #pragma unroll 1
 	for (int k = 0; k < N; ++k) {
-		double gx = 0.0, gy = 0.0, gz = 0.0;
-		const double *plane = su + k * N * P;
+		T gx = T(0), gy = T(0), gz = T(0);
+		const T *plane = su + k * N * P;
 #pragma unroll
 		for (int l = 0; l < N; ++l) {
 			gx += dxf[l] * plane[j * P + l];
 			gy += dyf[l] * plane[l * P + i];
 			gz += gDn[k * N + l] * uc[l];      // uniform: scalar loads
 		}
-		double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+		T t0 = T(0), t1 = T(0), t2 = T(0);
 		if (act) {
 			const int q = k * N2 + tid;
-			const double gxx = g[q], gxy = g[N3 + q], gxz = g[2 * N3 + q];
-			const double gyy = g[3 * N3 + q], gyz = g[4 * N3 + q];
-			const double gzz = g[5 * N3 + q];
+			const T gxx = ldg(g, q), gxy = ldg(g, N3 + q);
+			const T gxz = ldg(g, 2 * N3 + q), gyy = ldg(g, 3 * N3 + q);
+			const T gyz = ldg(g, 4 * N3 + q), gzz = ldg(g, 5 * N3 + q);
 			t0 = gxx * gx + gxy * gy + gxz * gz;
 			t1 = gxy * gx + gyy * gy + gyz * gz;
 			t2 = gxz * gx + gyz * gy + gzz * gz;
@@ -134,13 +148,13 @@ k_ax(int nel, const double *__restrict__ u, double *__restrict__ au,
 		sa[j * P + i] = t0;
 		sb[j * P + i] = t1;
 		__syncthreads();
-		double r = 0.0;
+		T r = T(0);
 #pragma unroll
 		for (int l = 0; l < N; ++l)
 			r += dxt[l] * sa[j * P + l] + dyt[l] * sb[l * P + i];
 #pragma unroll
 		for (int kk = 0; kk < N; ++kk)
-			acc[kk] += (kk == k ? r : 0.0) + gDs[k * N + kk] * t2;
+			acc[kk] += (kk == k ? r : T(0)) + gDs[k * N + kk] * t2;
 		__syncthreads();
 	}
 	// bc[e]: bits 0-5 Dirichlet faces, bits 8-13 faces with a neighbour
@@ -155,19 +169,19 @@ k_ax(int nel, const double *__restrict__ u, double *__restrict__ au,
 #pragma unroll
 		for (int k = 0; k < N; ++k) {
 			const int q = k * N2 + tid;
-			double v = acc[k];
+			T v = acc[k];
 			if (OP == 1)
 				v += lambda * (mass[eoff + q] * uc[k]);
 			const bool m = mi || ((bits & 16) && k == 0)
 				|| ((bits & 32) && k == N - 1);
-			v = m ? 0.0 : v;
+			v = m ? T(0) : v;
 			au[eoff + q] = v;
 			// <p, Ap>_w over nodes no other element shares (weight 1);
 			// the shared ones are added by the DSS kernel after assembly
 			const bool sh = si || ((eb & 4096) && k == 0)
 				|| ((eb & 8192) && k == N - 1);
 			if (!sh)
-				pd += uc[k] * v;
+				pd += (double) uc[k] * (double) v;
 		}
 	}
 	if (part) {
@@ -188,24 +202,44 @@ k_ax(int nel, const double *__restrict__ u, double *__restrict__ au,
 	}
 }
 
-template <int N>
-static int launch_ax(hipStream_t st, int nel, int op, const double *u,
-		double *au, const double *gf, const double *mass,
-		const double *dmat, double lambda, const int *bc, int apply_mask,
-		const int *done, double *part)
+template <int N, typename T, typename G>
+static int launch_ax(hipStream_t st, int nel, int op, const void *u,
+		void *au, const void *gf, const void *mass, const void *dmat,
+		double lambda, const int *bc, int apply_mask, const int *done,
+		double *part)
 {
 	using S = AxShape<N>;
 	const int grid = (nel + S::TEAMS - 1) / S::TEAMS;
 	if (op == SSX_OP_HELMHOLTZ)
-		hipLaunchKernelGGL((k_ax<N, 1>), dim3(grid), dim3(S::BLOCK), 0, st,
-				nel, u, au, gf, mass, dmat, lambda, bc, apply_mask,
-				done, part);
+		hipLaunchKernelGGL((k_ax<N, 1, T, G>), dim3(grid), dim3(S::BLOCK), 0,
+				st, nel, (const T *) u, (T *) au, (const G *) gf,
+				(const T *) mass, (const T *) dmat, (T) lambda, bc,
+				apply_mask, done, part);
 	else
-		hipLaunchKernelGGL((k_ax<N, 0>), dim3(grid), dim3(S::BLOCK), 0, st,
-				nel, u, au, gf, mass, dmat, lambda, bc, apply_mask,
-				done, part);
+		hipLaunchKernelGGL((k_ax<N, 0, T, G>), dim3(grid), dim3(S::BLOCK), 0,
+				st, nel, (const T *) u, (T *) au, (const G *) gf,
+				(const T *) mass, (const T *) dmat, (T) lambda, bc,
+				apply_mask, done, part);
 	LAUNCH_CHECK("k_ax");
 	return SSX_OK;
+}
+
+template <typename T, typename G>
+static int dispatch_ax(hipStream_t st, int n, int nel, int op, const void *u,
+		void *au, const void *gf, const void *mass, const void *dmat,
+		double lambda, const int *bc, int apply_mask, const int *done,
+		double *part)
+{
+	switch (n) {
+#define CASE(NN) case NN: return launch_ax<NN, T, G>(st, nel, op, u, au, gf, \
+			mass, dmat, lambda, bc, apply_mask, done, part);
+	CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9)
+	CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+	default:
+		ssx_set_error("ssxk_ax: n=%d unsupported", n);
+		return SSX_EINVAL;
+	}
 }
 
 extern "C" int ssxk_ax_blocks(int n, int nel)
@@ -219,19 +253,24 @@ extern "C" int ssxk_ax(void *stream, int n, int nel, int op, const double *u,
 		const double *dmat, double lambda, const int *bc, int apply_mask,
 		const int *done, double *part)
 {
-	hipStream_t st = (hipStream_t) stream;
 	if (nel <= 0)
 		return SSX_OK;
-	switch (n) {
-#define CASE(NN) case NN: return launch_ax<NN>(st, nel, op, u, au, gf, mass, \
-			dmat, lambda, bc, apply_mask, done, part);
-	CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9)
-	CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
-#undef CASE
-	default:
-		ssx_set_error("ssxk_ax: n=%d unsupported", n);
-		return SSX_EINVAL;
-	}
+	return dispatch_ax<double, double>((hipStream_t) stream, n, nel, op, u,
+			au, gf, mass, dmat, lambda, bc, apply_mask, done, part);
+}
+
+extern "C" int ssxk_ax_lp(void *stream, int prec, int n, int nel, int op,
+		const void *u, void *au, const void *gf, const void *mass,
+		const void *dmat, double lambda, const int *bc, int apply_mask,
+		const int *done, double *part)
+{
+	if (nel <= 0)
+		return SSX_OK;
+	if (prec == SSX_PREC_BF16)
+		return dispatch_ax<float, bf16_t>((hipStream_t) stream, n, nel, op, u,
+				au, gf, mass, dmat, lambda, bc, apply_mask, done, part);
+	return dispatch_ax<float, float>((hipStream_t) stream, n, nel, op, u, au,
+			gf, mass, dmat, lambda, bc, apply_mask, done, part);
 }
 
 // ---------------------------------------------------------------------
@@ -245,76 +284,28 @@ __device__ __forceinline__ int dss_ord(int M, int k)
 	return M == 8 ? ((k == 3) ? 4 : (k == 4) ? 3 : k) : k;
 }
 
-template <int M>
-__global__ void __launch_bounds__(256)
-k_dss(long long G, const int *__restrict__ idx, double *__restrict__ u)
-{
-	const long long g = (long long) blockIdx.x * blockDim.x + threadIdx.x;
-	if (g >= G)
-		return;
-	int ix[M];
-	double v[M];
-#pragma unroll
-	for (int t = 0; t < M; ++t)
-		ix[t] = idx[(long long) t * G + g];
-#pragma unroll
-	for (int t = 0; t < M; ++t)
-		v[t] = u[ix[t]];
-#pragma unroll
-	for (int t = 0; t < M; ++t) {
-		double s = v[t];
-#pragma unroll
-		for (int k = 1; k < M; ++k)
-			s = s + v[t ^ dss_ord(M, k)];
-		u[ix[t]] = s;
-	}
-}
-
-extern "C" int ssxk_dss(void *stream, int *const grp[3], const long long ng[3],
-		double *u)
-{
-	hipStream_t st = (hipStream_t) stream;
-	if (ng[0] > 0) {
-		hipLaunchKernelGGL((k_dss<2>), dim3((ng[0] + 255) / 256), dim3(256),
-				0, st, ng[0], grp[0], u);
-		LAUNCH_CHECK("k_dss<2>");
-	}
-	if (ng[1] > 0) {
-		hipLaunchKernelGGL((k_dss<4>), dim3((ng[1] + 255) / 256), dim3(256),
-				0, st, ng[1], grp[1], u);
-		LAUNCH_CHECK("k_dss<4>");
-	}
-	if (ng[2] > 0) {
-		hipLaunchKernelGGL((k_dss<8>), dim3((ng[2] + 255) / 256), dim3(256),
-				0, st, ng[2], grp[2], u);
-		LAUNCH_CHECK("k_dss<8>");
-	}
-	return SSX_OK;
-}
-
-// One launch for all three multiplicities: blocks [0,b2) take the pairs,
-// [b2,b2+b4) the quadruples, the rest the octuples.  Optionally each sum is
-// scaled by 1/M (direct_stiffness_averaging, global.c:259-268: the
-// multiplicity of every copy in a group of M is exactly M), and the
-// shared-node part of <p, DSS(ap)>_w is accumulated per block.
-template <int M>
+// One group of M copies.  Optionally each sum is scaled by 1/M
+// (direct_stiffness_averaging, global.c:259-268: every copy in a group of M
+// has multiplicity exactly M), and the shared-node part of <p, DSS(u)>_w is
+// returned.
+template <int M, typename T>
 __device__ __forceinline__ double dss_group(long long G, long long g,
-		const int *__restrict__ idx, double *__restrict__ u,
-		const double *__restrict__ p, int average)
+		const int *__restrict__ idx, T *__restrict__ u,
+		const T *__restrict__ p, int average)
 {
 	int ix[M];
-	double v[M];
+	T v[M];
 #pragma unroll
 	for (int t = 0; t < M; ++t)
 		ix[t] = idx[(long long) t * G + g];
 #pragma unroll
 	for (int t = 0; t < M; ++t)
 		v[t] = u[ix[t]];
-	const double w = 1.0 / (double) M;
+	const T w = T(1) / T(M);
 	double pd = 0.0;
 #pragma unroll
 	for (int t = 0; t < M; ++t) {
-		double s = v[t];
+		T s = v[t];
 #pragma unroll
 		for (int k = 1; k < M; ++k)
 			s = s + v[t ^ dss_ord(M, k)];
@@ -322,15 +313,18 @@ __device__ __forceinline__ double dss_group(long long G, long long g,
 			s = s * w;
 		u[ix[t]] = s;
 		if (p)
-			pd += (p[ix[t]] * w) * s;
+			pd += (double) (p[ix[t]] * w) * (double) s;
 	}
 	return pd;
 }
 
+// One launch for all three multiplicities: blocks [0,b2) take the pairs,
+// [b2,b2+b4) the quadruples, the rest the octuples.
+template <typename T>
 __global__ void __launch_bounds__(256)
 k_dss_all(long long G2, long long G4, long long G8, const int *__restrict__ i2,
 		const int *__restrict__ i4, const int *__restrict__ i8,
-		double *__restrict__ u, const double *__restrict__ p,
+		T *__restrict__ u, const T *__restrict__ p,
 		double *__restrict__ part, int average, const int *__restrict__ done)
 {
 	if (done && *done)
@@ -341,15 +335,15 @@ k_dss_all(long long G2, long long G4, long long G8, const int *__restrict__ i2,
 	if (b < b2) {
 		long long g = b * 256 + threadIdx.x;
 		if (g < G2)
-			pd = dss_group<2>(G2, g, i2, u, p, average);
+			pd = dss_group<2, T>(G2, g, i2, u, p, average);
 	} else if (b < b2 + b4) {
 		long long g = (b - b2) * 256 + threadIdx.x;
 		if (g < G4)
-			pd = dss_group<4>(G4, g, i4, u, p, average);
+			pd = dss_group<4, T>(G4, g, i4, u, p, average);
 	} else {
 		long long g = (b - b2 - b4) * 256 + threadIdx.x;
 		if (g < G8)
-			pd = dss_group<8>(G8, g, i8, u, p, average);
+			pd = dss_group<8, T>(G8, g, i8, u, p, average);
 	}
 	if (part) {
 		__shared__ double sh[256];
@@ -371,21 +365,39 @@ extern "C" long long ssxk_dss_blocks(const long long ng[3])
 	return (ng[0] + 255) / 256 + (ng[1] + 255) / 256 + (ng[2] + 255) / 256;
 }
 
-extern "C" int ssxk_dss_all(void *stream, int *const grp[3],
-		const long long ng[3], double *u, const double *p, double *part,
-		int average, const int *done)
+template <typename T>
+static int launch_dss(hipStream_t st, int *const grp[3], const long long ng[3],
+		void *u, const void *p, double *part, int average, const int *done)
 {
 	long long nb = ssxk_dss_blocks(ng);
 	if (nb == 0)
 		return SSX_OK;
-	hipLaunchKernelGGL(k_dss_all, dim3((unsigned) nb), dim3(256), 0,
-			(hipStream_t) stream, ng[0], ng[1], ng[2], grp[0], grp[1],
-			grp[2], u, p, part, average, done);
+	hipLaunchKernelGGL(k_dss_all<T>, dim3((unsigned) nb), dim3(256), 0, st,
+			ng[0], ng[1], ng[2], grp[0], grp[1], grp[2], (T *) u,
+			(const T *) p, part, average, done);
 	LAUNCH_CHECK("k_dss_all");
 	return SSX_OK;
 }
 
-// --- fused CG iteration kernels (solver.c:58-74) -------------------------
+extern "C" int ssxk_dss_all(void *stream, int *const grp[3],
+		const long long ng[3], double *u, const double *p, double *part,
+		int average, const int *done)
+{
+	return launch_dss<double>((hipStream_t) stream, grp, ng, u, p, part,
+			average, done);
+}
+
+extern "C" int ssxk_dss_all_lp(void *stream, int *const grp[3],
+		const long long ng[3], void *u, const void *p, double *part,
+		int average, const int *done)
+{
+	return launch_dss<float>((hipStream_t) stream, grp, ng, u, p, part,
+			average, done);
+}
+
+// ---------------------------------------------------------------------
+// Fused CG iteration kernels (solver.c:58-74)
+// ---------------------------------------------------------------------
 
 __device__ __forceinline__ void block_sum4(double *v, double (*sh)[256])
 {
@@ -405,34 +417,32 @@ __device__ __forceinline__ void block_sum4(double *v, double (*sh)[256])
 }
 
 // u += a p; r -= a ap; z = dinv r; partial <r,z>_w, <r,r>_w and, for the
-// pressure null space, <z,1>_w and <r,1>_w
+// pressure null space, <z,1>_w and <r,1>_w (FP64 accumulation)
+template <typename T>
 __global__ void __launch_bounds__(256)
 k_cg_update(size_t n, const double *__restrict__ scal, int ia,
-		const double *__restrict__ p, const double *__restrict__ ap,
-		double *__restrict__ u, double *__restrict__ r,
-		double *__restrict__ z, const double *__restrict__ dinv,
-		const double *__restrict__ scale, int pressure,
+		const T *__restrict__ p, const T *__restrict__ ap,
+		T *__restrict__ u, T *__restrict__ r, T *__restrict__ z,
+		const T *__restrict__ dinv, const T *__restrict__ scale, int pressure,
 		double *__restrict__ part, const int *__restrict__ done)
 {
 	if (done && *done)
 		return;
 	__shared__ double sh[4][256];
-	const double a = scal[ia];
+	const T a = (T) scal[ia];
 	double acc[4] = { 0.0, 0.0, 0.0, 0.0 };
-	const size_t stride = (size_t) gridDim.x * blockDim.x;
-	for (size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x; q < n;
-			q += stride) {
+	GRID_STRIDE(q, n) {
 		u[q] = a * p[q] + u[q];
-		const double rv = -a * ap[q] + r[q];
+		const T rv = -a * ap[q] + r[q];
 		r[q] = rv;
-		const double zv = rv * dinv[q];
+		const T zv = rv * dinv[q];
 		z[q] = zv;
-		const double w = scale[q];
-		acc[0] += (rv * w) * zv;
-		acc[1] += (rv * w) * rv;
+		const T w = scale[q];
+		acc[0] += (double) (rv * w) * (double) zv;
+		acc[1] += (double) (rv * w) * (double) rv;
 		if (pressure) {
-			acc[2] += zv * w;
-			acc[3] += rv * w;
+			acc[2] += (double) (zv * w);
+			acc[3] += (double) (rv * w);
 		}
 	}
 	block_sum4(acc, sh);
@@ -441,19 +451,19 @@ k_cg_update(size_t n, const double *__restrict__ scal, int ia,
 }
 
 // p = (z - c) + beta p, masked (c: pressure null-space constant)
+template <typename T>
 __global__ void __launch_bounds__(256)
-k_cg_pupdate(size_t n, const double *__restrict__ z,
-		double *__restrict__ p, const double *__restrict__ scal, int ib,
-		int ic, const double *__restrict__ mask,
-		const int *__restrict__ done)
+k_cg_pupdate(size_t n, const T *__restrict__ z, T *__restrict__ p,
+		const double *__restrict__ scal, int ib, int ic,
+		const T *__restrict__ mask, const int *__restrict__ done)
 {
 	if (done && *done)
 		return;
-	const double b = scal[ib];
-	const double c = ic >= 0 ? -scal[ic] : 0.0;
-	GRID_STRIDE_LOOP(q, n) {
-		const double zv = ic >= 0 ? c * 1.0 + z[q] : z[q];
-		double v = b * p[q] + zv;
+	const T b = (T) scal[ib];
+	const T c = ic >= 0 ? (T) -scal[ic] : T(0);
+	GRID_STRIDE(q, n) {
+		const T zv = ic >= 0 ? c * T(1) + z[q] : z[q];
+		T v = b * p[q] + zv;
 		p[q] = mask ? v * mask[q] : v;
 	}
 }
@@ -512,26 +522,63 @@ k_cg_reduce(int mode, const double *__restrict__ part, long long cnt,
 		flags[0] = 1;
 }
 
+template <typename T>
+static int launch_cg_update(hipStream_t st, size_t n, const double *scal,
+		int ia, const void *p, const void *ap, void *u, void *r, void *z,
+		const void *dinv, const void *scale, int pressure, double *part,
+		const int *done)
+{
+	hipLaunchKernelGGL(k_cg_update<T>, dim3(SSX_NPART), dim3(256), 0, st, n,
+			scal, ia, (const T *) p, (const T *) ap, (T *) u, (T *) r,
+			(T *) z, (const T *) dinv, (const T *) scale, pressure, part,
+			done);
+	LAUNCH_CHECK("k_cg_update");
+	return SSX_OK;
+}
+
+template <typename T>
+static int launch_cg_pupdate(hipStream_t st, size_t n, const void *z,
+		void *p, const double *scal, int ib, int ic, const void *mask,
+		const int *done)
+{
+	hipLaunchKernelGGL(k_cg_pupdate<T>, vgrid(n), dim3(256), 0, st, n,
+			(const T *) z, (T *) p, scal, ib, ic, (const T *) mask, done);
+	LAUNCH_CHECK("k_cg_pupdate");
+	return SSX_OK;
+}
+
 extern "C" int ssxk_cg_update(void *stream, size_t n, const double *scal,
 		int ia, const double *p, const double *ap, double *u, double *r,
 		double *z, const double *dinv, const double *scale, int pressure,
 		double *part, const int *done)
 {
-	hipLaunchKernelGGL(k_cg_update, dim3(SSX_NPART), dim3(256), 0,
-			(hipStream_t) stream, n, scal, ia, p, ap, u, r, z, dinv, scale,
-			pressure, part, done);
-	LAUNCH_CHECK("k_cg_update");
-	return SSX_OK;
+	return launch_cg_update<double>((hipStream_t) stream, n, scal, ia, p, ap,
+			u, r, z, dinv, scale, pressure, part, done);
+}
+
+extern "C" int ssxk_cg_update_lp(void *stream, size_t n, const double *scal,
+		int ia, const void *p, const void *ap, void *u, void *r, void *z,
+		const void *dinv, const void *scale, int pressure, double *part,
+		const int *done)
+{
+	return launch_cg_update<float>((hipStream_t) stream, n, scal, ia, p, ap,
+			u, r, z, dinv, scale, pressure, part, done);
 }
 
 extern "C" int ssxk_cg_pupdate(void *stream, size_t n, const double *z,
 		double *p, const double *scal, int ib, int ic, const double *mask,
 		const int *done)
 {
-	hipLaunchKernelGGL(k_cg_pupdate, vgrid(n), dim3(256), 0,
-			(hipStream_t) stream, n, z, p, scal, ib, ic, mask, done);
-	LAUNCH_CHECK("k_cg_pupdate");
-	return SSX_OK;
+	return launch_cg_pupdate<double>((hipStream_t) stream, n, z, p, scal, ib,
+			ic, mask, done);
+}
+
+extern "C" int ssxk_cg_pupdate_lp(void *stream, size_t n, const void *z,
+		void *p, const double *scal, int ib, int ic, const void *mask,
+		const int *done)
+{
+	return launch_cg_pupdate<float>((hipStream_t) stream, n, z, p, scal, ib,
+			ic, mask, done);
 }
 
 extern "C" int ssxk_cg_reduce(void *stream, int mode, const double *part,
@@ -550,16 +597,18 @@ extern "C" int ssxk_cg_reduce(void *stream, int mode, const double *part,
 // prol[p*nf+i] = l_p^{coarse}(xi_i^{fine})
 // ---------------------------------------------------------------------
 
+template <typename T>
 __global__ void __launch_bounds__(256)
-k_restrict(int nf, int nc, const double *__restrict__ prol,
-		const double *__restrict__ fine, double *__restrict__ coarse)
+k_restrict(int nf, int nc, const T *__restrict__ prol,
+		const T *__restrict__ fine, T *__restrict__ coarse)
 {
-	extern __shared__ double sm[];
+	extern __shared__ char smraw[];
+	T *sm = (T *) smraw;
 	const int nf3 = nf * nf * nf, nc3 = nc * nc * nc;
-	double *sp = sm;                       // nc*nf
-	double *su = sp + nc * nf;             // nf^3
-	double *t1 = su + nf3;                 // nf*nf*nc
-	double *t2 = t1 + nf * nf * nc;        // nf*nc*nc
+	T *sp = sm;                            // nc*nf
+	T *su = sp + nc * nf;                  // nf^3
+	T *t1 = su + nf3;                      // nf*nf*nc
+	T *t2 = t1 + nf * nf * nc;             // nf*nc*nc
 	const int e = blockIdx.x;
 	for (int q = threadIdx.x; q < nc * nf; q += blockDim.x)
 		sp[q] = prol[q];
@@ -568,7 +617,7 @@ k_restrict(int nf, int nc, const double *__restrict__ prol,
 	__syncthreads();
 	for (int q = threadIdx.x; q < nf * nf * nc; q += blockDim.x) {
 		const int p = q % nc, kj = q / nc;
-		double s = 0.0;
+		T s = T(0);
 		for (int c = 0; c < nf; ++c)
 			s += sp[p * nf + c] * su[kj * nf + c];
 		t1[q] = s;
@@ -576,7 +625,7 @@ k_restrict(int nf, int nc, const double *__restrict__ prol,
 	__syncthreads();
 	for (int q = threadIdx.x; q < nf * nc * nc; q += blockDim.x) {
 		const int i = q % nc, jq = (q / nc) % nc, k = q / (nc * nc);
-		double s = 0.0;
+		T s = T(0);
 		for (int l = 0; l < nf; ++l)
 			s += t1[(k * nf + l) * nc + i] * sp[jq * nf + l];
 		t2[q] = s;
@@ -584,24 +633,25 @@ k_restrict(int nf, int nc, const double *__restrict__ prol,
 	__syncthreads();
 	for (int q = threadIdx.x; q < nc3; q += blockDim.x) {
 		const int ji = q % (nc * nc), k = q / (nc * nc);
-		double s = 0.0;
+		T s = T(0);
 		for (int l = 0; l < nf; ++l)
 			s += t2[l * nc * nc + ji] * sp[k * nf + l];
 		coarse[(size_t) e * nc3 + q] = s;
 	}
 }
 
+template <typename T>
 __global__ void __launch_bounds__(256)
-k_prolong(int nf, int nc, const double *__restrict__ prol,
-		const double *__restrict__ coarse, double *__restrict__ fine,
-		int accumulate)
+k_prolong(int nf, int nc, const T *__restrict__ prol,
+		const T *__restrict__ coarse, T *__restrict__ fine, int accumulate)
 {
-	extern __shared__ double sm[];
+	extern __shared__ char smraw[];
+	T *sm = (T *) smraw;
 	const int nf3 = nf * nf * nf, nc3 = nc * nc * nc;
-	double *sp = sm;                       // nc*nf
-	double *su = sp + nc * nf;             // nc^3
-	double *t1 = su + nc3;                 // nc*nc*nf
-	double *t2 = t1 + nc * nc * nf;        // nc*nf*nf
+	T *sp = sm;                            // nc*nf
+	T *su = sp + nc * nf;                  // nc^3
+	T *t1 = su + nc3;                      // nc*nc*nf
+	T *t2 = t1 + nc * nc * nf;             // nc*nf*nf
 	const int e = blockIdx.x;
 	for (int q = threadIdx.x; q < nc * nf; q += blockDim.x)
 		sp[q] = prol[q];
@@ -610,7 +660,7 @@ k_prolong(int nf, int nc, const double *__restrict__ prol,
 	__syncthreads();
 	for (int q = threadIdx.x; q < nc * nc * nf; q += blockDim.x) {
 		const int i = q % nf, kj = q / nf;
-		double s = 0.0;
+		T s = T(0);
 		for (int p = 0; p < nc; ++p)
 			s += sp[p * nf + i] * su[kj * nc + p];
 		t1[q] = s;
@@ -618,7 +668,7 @@ k_prolong(int nf, int nc, const double *__restrict__ prol,
 	__syncthreads();
 	for (int q = threadIdx.x; q < nc * nf * nf; q += blockDim.x) {
 		const int i = q % nf, jq = (q / nf) % nf, k = q / (nf * nf);
-		double s = 0.0;
+		T s = T(0);
 		for (int l = 0; l < nc; ++l)
 			s += t1[(k * nc + l) * nf + i] * sp[l * nf + jq];
 		t2[q] = s;
@@ -626,7 +676,7 @@ k_prolong(int nf, int nc, const double *__restrict__ prol,
 	__syncthreads();
 	for (int q = threadIdx.x; q < nf3; q += blockDim.x) {
 		const int ji = q % (nf * nf), k = q / (nf * nf);
-		double s = 0.0;
+		T s = T(0);
 		for (int l = 0; l < nc; ++l)
 			s += t2[l * nf * nf + ji] * sp[l * nf + k];
 		const size_t o = (size_t) e * nf3 + q;
@@ -634,27 +684,57 @@ k_prolong(int nf, int nc, const double *__restrict__ prol,
 	}
 }
 
+template <typename T>
+static int launch_restrict(hipStream_t st, int nf, int nc, int nel,
+		const void *prol, const void *fine, void *coarse)
+{
+	const size_t sh = sizeof(T) * ((size_t) nc * nf + (size_t) nf * nf * nf
+			+ (size_t) nf * nf * nc + (size_t) nf * nc * nc);
+	hipLaunchKernelGGL(k_restrict<T>, dim3(nel), dim3(256), sh, st, nf, nc,
+			(const T *) prol, (const T *) fine, (T *) coarse);
+	LAUNCH_CHECK("k_restrict");
+	return SSX_OK;
+}
+
+template <typename T>
+static int launch_prolong(hipStream_t st, int nf, int nc, int nel,
+		const void *prol, const void *coarse, void *fine, int accumulate)
+{
+	const size_t sh = sizeof(T) * ((size_t) nc * nf + (size_t) nc * nc * nc
+			+ (size_t) nc * nc * nf + (size_t) nc * nf * nf);
+	hipLaunchKernelGGL(k_prolong<T>, dim3(nel), dim3(256), sh, st, nf, nc,
+			(const T *) prol, (const T *) coarse, (T *) fine, accumulate);
+	LAUNCH_CHECK("k_prolong");
+	return SSX_OK;
+}
+
 extern "C" int ssxk_restrict(void *stream, int nf, int nc, int nel,
 		const double *prol, const double *fine, double *coarse)
 {
-	const size_t sh = sizeof(double) * ((size_t) nc * nf + (size_t) nf * nf * nf
-			+ (size_t) nf * nf * nc + (size_t) nf * nc * nc);
-	hipLaunchKernelGGL(k_restrict, dim3(nel), dim3(256), sh,
-			(hipStream_t) stream, nf, nc, prol, fine, coarse);
-	LAUNCH_CHECK("k_restrict");
-	return SSX_OK;
+	return launch_restrict<double>((hipStream_t) stream, nf, nc, nel, prol,
+			fine, coarse);
 }
 
 extern "C" int ssxk_prolong(void *stream, int nf, int nc, int nel,
 		const double *prol, const double *coarse, double *fine,
 		int accumulate)
 {
-	const size_t sh = sizeof(double) * ((size_t) nc * nf + (size_t) nc * nc * nc
-			+ (size_t) nc * nc * nf + (size_t) nc * nf * nf);
-	hipLaunchKernelGGL(k_prolong, dim3(nel), dim3(256), sh,
-			(hipStream_t) stream, nf, nc, prol, coarse, fine, accumulate);
-	LAUNCH_CHECK("k_prolong");
-	return SSX_OK;
+	return launch_prolong<double>((hipStream_t) stream, nf, nc, nel, prol,
+			coarse, fine, accumulate);
+}
+
+extern "C" int ssxk_restrict_lp(void *stream, int nf, int nc, int nel,
+		const void *prol, const void *fine, void *coarse)
+{
+	return launch_restrict<float>((hipStream_t) stream, nf, nc, nel, prol,
+			fine, coarse);
+}
+
+extern "C" int ssxk_prolong_lp(void *stream, int nf, int nc, int nel,
+		const void *prol, const void *coarse, void *fine, int accumulate)
+{
+	return launch_prolong<float>((hipStream_t) stream, nf, nc, nel, prol,
+			coarse, fine, accumulate);
 }
 
 // ---------------------------------------------------------------------
@@ -674,17 +754,16 @@ __device__ __forceinline__ double block_sum256(double v, double *sh)
 	return sh[0];
 }
 
+template <typename T>
 __global__ void __launch_bounds__(256)
-k_dot(size_t n, const double *__restrict__ x, const double *__restrict__ s,
-		const double *__restrict__ y, double *__restrict__ part)
+k_dot(size_t n, const T *__restrict__ x, const T *__restrict__ s,
+		const T *__restrict__ y, double *__restrict__ part)
 {
 	__shared__ double sh[256];
 	double acc = 0.0;
-	const size_t stride = (size_t) gridDim.x * blockDim.x;
-	for (size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x; q < n;
-			q += stride) {
-		double a = x[q] * s[q];
-		acc += y ? a * y[q] : a;
+	GRID_STRIDE(q, n) {
+		T a = x[q] * s[q];
+		acc += y ? (double) a * (double) y[q] : (double) a;
 	}
 	double t = block_sum256(acc, sh);
 	if (threadIdx.x == 0)
@@ -706,8 +785,18 @@ k_reduce(const double *__restrict__ part, double *__restrict__ scal, int dst)
 extern "C" int ssxk_dot(void *stream, size_t n, const double *x,
 		const double *s, const double *y, double *part, int slot)
 {
-	hipLaunchKernelGGL(k_dot, dim3(SSX_NPART), dim3(256), 0,
+	hipLaunchKernelGGL(k_dot<double>, dim3(SSX_NPART), dim3(256), 0,
 			(hipStream_t) stream, n, x, s, y, part + (size_t) slot * SSX_NPART);
+	LAUNCH_CHECK("k_dot");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_dot_lp(void *stream, size_t n, const void *x,
+		const void *s, const void *y, double *part, int slot)
+{
+	hipLaunchKernelGGL(k_dot<float>, dim3(SSX_NPART), dim3(256), 0,
+			(hipStream_t) stream, n, (const float *) x, (const float *) s,
+			(const float *) y, part + (size_t) slot * SSX_NPART);
 	LAUNCH_CHECK("k_dot");
 	return SSX_OK;
 }
@@ -749,12 +838,12 @@ extern "C" int ssxk_scalar(void *stream, double *scal, int op, int a, int b,
 // Vector kernels
 // ---------------------------------------------------------------------
 
-
-__global__ void k_axpby(size_t n, double a, const double *__restrict__ x,
-		const double *__restrict__ w, double b, double *__restrict__ y)
+template <typename T>
+__global__ void k_axpby(size_t n, T a, const T *__restrict__ x,
+		const T *__restrict__ w, T b, T *__restrict__ y)
 {
 	GRID_STRIDE(q, n) {
-		double xv = w ? x[q] * w[q] : x[q];
+		T xv = w ? x[q] * w[q] : x[q];
 		y[q] = a * xv + b * y[q];
 	}
 }
@@ -762,14 +851,25 @@ __global__ void k_axpby(size_t n, double a, const double *__restrict__ x,
 extern "C" int ssxk_axpby(void *stream, size_t n, double a, const double *x,
 		const double *w, double b, double *y)
 {
-	hipLaunchKernelGGL(k_axpby, vgrid(n), dim3(256), 0, (hipStream_t) stream,
-			n, a, x, w, b, y);
+	hipLaunchKernelGGL(k_axpby<double>, vgrid(n), dim3(256), 0,
+			(hipStream_t) stream, n, a, x, w, b, y);
 	LAUNCH_CHECK("k_axpby");
 	return SSX_OK;
 }
 
-__global__ void k_mul(size_t n, const double *__restrict__ x,
-		const double *__restrict__ w, double *__restrict__ z)
+extern "C" int ssxk_axpby_lp(void *stream, size_t n, double a, const void *x,
+		const void *w, double b, void *y)
+{
+	hipLaunchKernelGGL(k_axpby<float>, vgrid(n), dim3(256), 0,
+			(hipStream_t) stream, n, (float) a, (const float *) x,
+			(const float *) w, (float) b, (float *) y);
+	LAUNCH_CHECK("k_axpby");
+	return SSX_OK;
+}
+
+template <typename T>
+__global__ void k_mul(size_t n, const T *__restrict__ x,
+		const T *__restrict__ w, T *__restrict__ z)
 {
 	GRID_STRIDE(q, n) z[q] = x[q] * w[q];
 }
@@ -777,8 +877,18 @@ __global__ void k_mul(size_t n, const double *__restrict__ x,
 extern "C" int ssxk_mul(void *stream, size_t n, const double *x,
 		const double *w, double *z)
 {
-	hipLaunchKernelGGL(k_mul, vgrid(n), dim3(256), 0, (hipStream_t) stream,
-			n, x, w, z);
+	hipLaunchKernelGGL(k_mul<double>, vgrid(n), dim3(256), 0,
+			(hipStream_t) stream, n, x, w, z);
+	LAUNCH_CHECK("k_mul");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_mul_lp(void *stream, size_t n, const void *x,
+		const void *w, void *z)
+{
+	hipLaunchKernelGGL(k_mul<float>, vgrid(n), dim3(256), 0,
+			(hipStream_t) stream, n, (const float *) x, (const float *) w,
+			(float *) z);
 	LAUNCH_CHECK("k_mul");
 	return SSX_OK;
 }
@@ -819,18 +929,28 @@ extern "C" int ssxk_xpby_s(void *stream, size_t n, const double *x,
 	return SSX_OK;
 }
 
+template <typename T>
 __global__ void k_shift_s(size_t n, const double *__restrict__ scal, int ia,
-		double nunique, double *__restrict__ y)
+		double nunique, T *__restrict__ y)
 {
-	const double c = -scal[ia] / nunique;
-	GRID_STRIDE(q, n) y[q] = c * 1.0 + y[q];
+	const T c = (T) (-scal[ia] / nunique);
+	GRID_STRIDE(q, n) y[q] = c * T(1) + y[q];
 }
 
 extern "C" int ssxk_shift_s(void *stream, size_t n, const double *scal,
 		int ia, double nunique, double *y)
 {
-	hipLaunchKernelGGL(k_shift_s, vgrid(n), dim3(256), 0,
+	hipLaunchKernelGGL(k_shift_s<double>, vgrid(n), dim3(256), 0,
 			(hipStream_t) stream, n, scal, ia, nunique, y);
+	LAUNCH_CHECK("k_shift_s");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_shift_s_lp(void *stream, size_t n, const double *scal,
+		int ia, double nunique, void *y)
+{
+	hipLaunchKernelGGL(k_shift_s<float>, vgrid(n), dim3(256), 0,
+			(hipStream_t) stream, n, scal, ia, nunique, (float *) y);
 	LAUNCH_CHECK("k_shift_s");
 	return SSX_OK;
 }
@@ -849,15 +969,24 @@ extern "C" int ssxk_set_ints(void *stream, int *dst, int v, int n)
 	return SSX_OK;
 }
 
-__global__ void k_fill(size_t n, double v, double *__restrict__ y)
+template <typename T>
+__global__ void k_fill(size_t n, T v, T *__restrict__ y)
 {
 	GRID_STRIDE(q, n) y[q] = v;
 }
 
 extern "C" int ssxk_fill(void *stream, double *dst, double v, size_t n)
 {
-	hipLaunchKernelGGL(k_fill, vgrid(n), dim3(256), 0, (hipStream_t) stream,
-			n, v, dst);
+	hipLaunchKernelGGL(k_fill<double>, vgrid(n), dim3(256), 0,
+			(hipStream_t) stream, n, v, dst);
+	LAUNCH_CHECK("k_fill");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_fill_lp(void *stream, void *dst, double v, size_t n)
+{
+	hipLaunchKernelGGL(k_fill<float>, vgrid(n), dim3(256), 0,
+			(hipStream_t) stream, n, (float) v, (float *) dst);
 	LAUNCH_CHECK("k_fill");
 	return SSX_OK;
 }
@@ -871,5 +1000,61 @@ extern "C" int ssxk_copy(void *stream, double *dst, const double *src,
 		ssx_set_error("copy: %s", hipGetErrorString(e));
 		return SSX_EHIP;
 	}
+	return SSX_OK;
+}
+
+extern "C" int ssxk_copy_lp(void *stream, void *dst, const void *src, size_t n)
+{
+	hipError_t e = hipMemcpyAsync(dst, src, n * sizeof(float),
+			hipMemcpyDeviceToDevice, (hipStream_t) stream);
+	if (e != hipSuccess) {
+		ssx_set_error("copy: %s", hipGetErrorString(e));
+		return SSX_EHIP;
+	}
+	return SSX_OK;
+}
+
+// precision conversion: FP64 -> FP32 (round to nearest), FP32 -> FP64
+// (exact), FP64 -> bfloat16 (via FP32, round to nearest even)
+__global__ void k_d2f(size_t n, const double *__restrict__ x,
+		float *__restrict__ y)
+{
+	GRID_STRIDE(q, n) y[q] = (float) x[q];
+}
+
+__global__ void k_f2d(size_t n, const float *__restrict__ x,
+		double *__restrict__ y)
+{
+	GRID_STRIDE(q, n) y[q] = (double) x[q];
+}
+
+__global__ void k_d2bf(size_t n, const double *__restrict__ x,
+		bf16_t *__restrict__ y)
+{
+	GRID_STRIDE(q, n) {
+		unsigned b = __float_as_uint((float) x[q]);
+		b += 0x7fffu + ((b >> 16) & 1u);
+		y[q] = (bf16_t) (b >> 16);
+	}
+}
+
+extern "C" int ssxk_cvt(void *stream, int from, int to, const void *x,
+		void *y, size_t n)
+{
+	hipStream_t st = (hipStream_t) stream;
+	if (from == SSX_PREC_FP64 && to == SSX_PREC_FP32)
+		hipLaunchKernelGGL(k_d2f, vgrid(n), dim3(256), 0, st, n,
+				(const double *) x, (float *) y);
+	else if (from == SSX_PREC_FP32 && to == SSX_PREC_FP64)
+		hipLaunchKernelGGL(k_f2d, vgrid(n), dim3(256), 0, st, n,
+				(const float *) x, (double *) y);
+	else if (from == SSX_PREC_FP64 && to == SSX_PREC_BF16)
+		hipLaunchKernelGGL(k_d2bf, vgrid(n), dim3(256), 0, st, n,
+				(const double *) x, (bf16_t *) y);
+	else {
+		ssx_set_error("ssxk_cvt: %d -> %d unsupported", from, to);
+		return SSX_EINVAL;
+	}
+	LAUNCH_CHECK("k_cvt");
 	return SSX_OK;
 }

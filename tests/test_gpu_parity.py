@@ -297,3 +297,33 @@ def test_fused_cg_matches_plain(name):
     assert rel(a[1], b[1]) <= 1e-8 and rel(a[3], b[3]) <= 1e-8
     assert rel(a[1], Z["ref_cg_helm"]) <= 1e-8
     assert rel(a[3], Z["ref_fcg_pressure"]) <= 1e-8
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+@pytest.mark.parametrize("name", ["cube2_n8", "periodic3_n6", "channel_n6",
+                                  "cube2_n12"])
+def test_mixed_precision_smoother(name, prec):
+    """FP32 (prec 1) / FP32 + BF16-factor (prec 2) multigrid smoother and
+    transfers under the FP64 flexible CG: reaches the same FP64 tolerance;
+    the solution agrees with the reference's FP64 solution to the level the
+    tolerance fixes (both residuals <= 1e-10 ||b||_w)."""
+    Z = golden(name)
+    mesh = golden_mesh(Z)
+    sc = Z["ref_scalars"]
+    eps = sc[0]
+    with make_ctx(mesh) as ctx:
+        ctx.set_option(S.OPT_SMOOTHER_PRECISION, prec)
+        for var, ref, it_ref in ((S.PRESSURE, "ref_fcg_pressure", sc[3]),
+                                 (S.X_VELOCITY, "ref_fcg_dirichlet", sc[5])):
+            ctx.set_diag_choice(-1 if var == S.PRESSURE else 0)
+            f = dev(Z["in_f"])
+            u = torch.zeros_like(f)
+            st = ctx.fcg(S.OP_POISSON, var, S.PC_MG, f, u, eps, 500)
+            err = rel(host(u), Z[ref])
+            print(f"[gpu] {name} prec {prec} var {var}: {st.iterations} its "
+                  f"(fp64 ref {int(it_ref)}), |x - x_ref|/|x_ref| = {err:.2e}")
+            assert st.converged and st.residual <= eps
+            assert st.iterations <= int(it_ref) + 3
+            assert err <= 1e-7
+        ctx.set_option(S.OPT_SMOOTHER_PRECISION, 0)
+        ctx.set_diag_choice(-1)
