@@ -33,6 +33,9 @@
  *   ssx_fcg             flexible_conjugate_gradient solver.c:83-157
  *   ssx_solve_poisson_cg    solve_poisson_cg solver.c:159-173 (host arrays)
  *   ssx_solve_helmholtz_cg  solve_helmholtz_cg solver.c:175-189 (host arrays)
+ *   ssx_particles_set_velocity  compute_reference_domain_velocity
+ *                       geometry.c:188-198
+ *   ssx_particles_step  integrate_particle_equations particles.c:182-218
  */
 #ifndef SSX_H
 #define SSX_H
@@ -195,6 +198,30 @@ int ssx_fcg(ssx_ctx *ctx, int op, int var, int precon, double *f, double *u,
 		double eps, int max_iter, ssx_stats *stats);
 int ssx_mg_precon(ssx_ctx *ctx, int op, int var, double eps, const double *x,
 		double *px);
+
+/* ---- inertial point particles (particles.c), device arrays ----
+ * State as struct particle (particles.h:25-41): owning element, previous
+ * element, reference coordinates (x,y,z) in [-1,1]^3 of that element,
+ * velocity (u,v,w) in reference coordinates, time of the last update t,
+ * response time tau.  reinit (may be NULL) is set to 1 for a particle the
+ * reference would re-seed at random (more than 10 relocation passes, or no
+ * way back into an element): it is restarted at the element centre. */
+typedef struct ssx_particles {
+	long n;
+	int *el, *old_el;
+	double *x, *y, *z, *u, *v, *w, *t, *tau;
+	int *reinit;
+} ssx_particles;
+/* compute_reference_domain_velocity (geometry.c:188-198): shift the
+ * reference-space velocity history one level and set ur = J^-1 (u,v,w) from
+ * physical device E-vectors (level 0) */
+int ssx_particles_set_velocity(ssx_ctx *ctx, const double *u,
+		const double *v, const double *w);
+/* integrate_particle_equations (particles.c:182-218): advance every
+ * particle from its t to t_now (params->t) with history step dt
+ * (params->dt); *max_passes receives the largest relocation pass count */
+int ssx_particles_step(ssx_ctx *ctx, const ssx_particles *p, double t_now,
+		double dt, int *max_passes);
 
 /* ---- reference entry points on HOST E-vectors: f (masked in place) and u
  * (initial guess in, solution out), FCG + p-multigrid for Poisson and

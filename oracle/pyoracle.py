@@ -249,3 +249,70 @@ class Oracle:
         a = (C.c_longlong * 4)()
         self.L.orc_counters(self.h, a)
         return tuple(a)
+
+
+class Particle(C.Structure):
+    """orc_particle (struct particle of the reference, single rank)"""
+    _fields_ = [("id", C.c_long), ("el", C.c_int), ("old_el", C.c_int),
+                ("status", C.c_int), ("x", C.c_double), ("y", C.c_double),
+                ("z", C.c_double), ("u", C.c_double), ("v", C.c_double),
+                ("w", C.c_double), ("t", C.c_double), ("tau", C.c_double),
+                ("reinit", C.c_int)]
+
+
+def _particle_sigs(L):
+    vp, dp = C.c_void_p, C.POINTER(C.c_double)
+    L.orc_interp_point.restype = C.c_double
+    L.orc_interp_point.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double,
+                                   C.c_double]
+    L.orc_particles_set_velocity.restype = None
+    L.orc_particles_set_velocity.argtypes = [vp, dp, dp, dp]
+    L.orc_particles_step.restype = C.c_int
+    L.orc_particles_step.argtypes = [vp, C.POINTER(Particle), C.c_long,
+                                     C.c_double, C.c_double]
+    L.orc_particle_position.restype = None
+    L.orc_particle_position.argtypes = [vp, C.POINTER(Particle), dp]
+
+
+def interp_point(n, xi, f, x, y, z):
+    L = load()
+    _particle_sigs(L)
+    xi = np.ascontiguousarray(xi, np.float64)
+    f = np.ascontiguousarray(f, np.float64)
+    return L.orc_interp_point(n, _dp(xi), _dp(f), x, y, z)
+
+
+def particles_from_records(rec):
+    """rec: (n, 9) el, x, y, z, u, v, w, t, tau"""
+    arr = (Particle * len(rec))()
+    for i, r in enumerate(rec):
+        p = arr[i]
+        p.id, p.el, p.old_el, p.status = i, int(r[0]), int(r[0]), 0
+        p.x, p.y, p.z, p.u, p.v, p.w, p.t, p.tau = map(float, r[1:9])
+        p.reinit = 0
+    return arr
+
+
+def particles_to_records(arr):
+    """(n, 8): el, x, y, z, u, v, w, t  and the re-initialised flags"""
+    out = np.array([[p.el, p.x, p.y, p.z, p.u, p.v, p.w, p.t] for p in arr])
+    return out, np.array([p.reinit for p in arr])
+
+
+def _oracle_particles(self, vel, scales, rec, t_now, dt):
+    """Three set_velocity calls (oldest first), then one step."""
+    L = self.L
+    _particle_sigs(L)
+    nt = self.ntot()
+    for s in scales:
+        u = np.ascontiguousarray(s * vel[:nt])
+        v = np.ascontiguousarray(s * vel[nt:2 * nt])
+        w = np.ascontiguousarray(s * vel[2 * nt:])
+        L.orc_particles_set_velocity(self.h, _dp(u), _dp(v), _dp(w))
+    arr = particles_from_records(rec)
+    passes = L.orc_particles_step(self.h, arr, len(rec), t_now, dt)
+    out, re = particles_to_records(arr)
+    return out, re, passes
+
+
+Oracle.particles = _oracle_particles

@@ -37,6 +37,8 @@
 #include "helmholtz.h"
 #include "global.h"
 #include "solver.h"
+#include "particles.h"
+#include <unistd.h>
 
 static void die(const char *m)
 {
@@ -373,6 +375,77 @@ static void do_mesh(const char *dir)
 	}
 	res[nres++] = bnorm;
 	wr(dir, "ref_scalars.bin", res, nres);
+
+	/* particles (particles.c:182-218) with injected positions: the
+	 * reference seeds them from the wall clock (particles.c:152) */
+	snprintf(path, sizeof(path), "%s/in_particles.bin", dir);
+	if (access(path, R_OK) == 0) {
+		/* in_particles.bin: int np; double t_now, dt, s[3];
+		 * then per particle el, x, y, z, u, v, w, t, tau (doubles) */
+		FILE *pf = fopen(path, "rb");
+		int np;
+		double hdr2[5];
+		if (fread(&np, sizeof(int), 1, pf) != 1
+				|| fread(hdr2, sizeof(double), 5, pf) != 5)
+			die("particles header");
+		double *rec = rd(pf, sizeof(double), (size_t) np * 9);
+		fclose(pf);
+		double *vel = rdvec(dir, "in_vel.bin", 3 * nt);
+		/* three velocity levels, oldest first, each shifted into the
+		 * history by compute_reference_domain_velocity (main.c:84) */
+		for (int lev = 0; lev < 3; ++lev) {
+			for (int e = 0; e < nel; ++e)
+				for (int a = 0; a < 3; ++a) {
+					double *dst = ***sel[e]->u->ind[a];
+					const double *src = vel + a * nt + (size_t) e * n3;
+					for (int q = 0; q < n3; ++q)
+						dst[q] = hdr2[2 + lev] * src[q];
+				}
+			compute_reference_domain_velocity(sel);
+		}
+		params->t = hdr2[0];
+		params->dt = hdr2[1];
+		struct plist pl;
+		pl.n = np;
+		pl.ntot = np;
+		pl.nbuff = np;
+		pl.ntypes = 1;
+		pl.tau = NULL;
+		pl.nproc = 1;
+		pl.mpi_comm = MPI_COMM_WORLD;
+		pl.list = calloc(np, sizeof(struct particle));
+		for (int i = 0; i < np; ++i) {
+			const double *r = rec + (size_t) i * 9;
+			struct particle *q = pl.list + i;
+			q->id = i;
+			q->type = 0;
+			q->el = (int) r[0];
+			q->old_el = q->el;
+			q->x = r[1]; q->y = r[2]; q->z = r[3];
+			q->u = r[4]; q->v = r[5]; q->w = r[6];
+			q->t = r[7];
+			q->tau = r[8];
+			q->proc = 0;
+			q->old_proc = 0;
+			q->status = 0;
+		}
+		integrate_particle_equations(&pl, sel);
+		/* relocate_particles reorders the list: write back by id */
+		double *out = calloc((size_t) np * 8, sizeof(double));
+		for (int i = 0; i < np; ++i) {
+			const struct particle *q = pl.list + i;
+			double *o = out + (size_t) q->id * 8;
+			o[0] = q->el;
+			o[1] = q->x; o[2] = q->y; o[3] = q->z;
+			o[4] = q->u; o[5] = q->v; o[6] = q->w;
+			o[7] = q->t;
+		}
+		wr(dir, "ref_particles.bin", out, (size_t) np * 8);
+		free(out);
+		free(pl.list);
+		free(rec);
+		free(vel);
+	}
 	free(zero);
 	free(sol);
 	free(u);

@@ -65,6 +65,11 @@ struct orc_ctx {
 	int nmg[LMAX];
 	level_t lev[LMAX];
 	double *inv_mass;
+	double *jac[9], *jinv[9];       /* level 0: J_ab, (J^-1)_ab         */
+	double *ur[3][3];               /* [time level: 0 now, 1 old, 2 older][comp] */
+	const int *neigh;               /* face neighbours (mesh)           */
+	int *neigh_copy;
+	double pdt, pt;                 /* params->dt, params->t for particles */
 	int *bc;                        /* per element Dirichlet bit mask */
 	int *part;                      /* nel*26 partner (elid*26+slot) or -1 */
 	double lambda;
@@ -425,7 +430,7 @@ static void transfer_t(const double *a, int m, int n, const double *u,
 
 static void geometry_element(const basis_t *b, const double *x,
 		const double *y, const double *z, double *g[6], double *detj,
-		size_t off)
+		size_t off, double **jout, double **jiout)
 {
 	const int n = b->n, n3 = b->n3;
 	double *jac = xcalloc((size_t) 9 * n3, sizeof(double));
@@ -476,6 +481,12 @@ static void geometry_element(const basis_t *b, const double *x,
 			}
 		/* symmetric storage: row 1 and 2 writes overwrite the aliased
 		 * upper entries with the same products (tripalg.c:91-132) */
+		if (jout)
+			for (int r = 0; r < 3; ++r)
+				for (int s2 = 0; s2 < 3; ++s2) {
+					jout[r * 3 + s2][off + q] = a[r][s2];
+					jiout[r * 3 + s2][off + q] = ai[r][s2];
+				}
 		g[0][off + q] = gm[0][0];
 		g[1][off + q] = gm[1][0];
 		g[2][off + q] = gm[2][0];
@@ -1072,12 +1083,22 @@ orc_ctx *orc_create(const orc_mesh *m)
 		free(t2);
 		free(q);
 	}
+	for (int a = 0; a < 9; ++a) {
+		c->jac[a] = xcalloc(c->lev[0].ntot, sizeof(double));
+		c->jinv[a] = xcalloc(c->lev[0].ntot, sizeof(double));
+	}
+	for (int tl = 0; tl < 3; ++tl)
+		for (int a = 0; a < 3; ++a)
+			c->ur[tl][a] = xcalloc(c->lev[0].ntot, sizeof(double));
+	c->neigh_copy = xcalloc((size_t) m->nel * 6, sizeof(int));
+	memcpy(c->neigh_copy, m->neigh, sizeof(int) * 6 * (size_t) m->nel);
 	for (int l = 0; l < nl; ++l) {
 		level_t *L = &c->lev[l];
 		for (int e = 0; e < c->nel; ++e) {
 			size_t off = (size_t) e * L->n3;
 			geometry_element(&L->b, L->x + off, L->y + off, L->z + off,
-					L->g, L->detj, off);
+					L->g, L->detj, off, l == 0 ? c->jac : NULL,
+					l == 0 ? c->jinv : NULL);
 		}
 	}
 	/* Jacobi diagonals, precon.c:336-359 */
@@ -1174,6 +1195,14 @@ void orc_destroy(orc_ctx *c)
 		free(c->mg_e[l]);
 	}
 	free(c->inv_mass);
+	for (int a = 0; a < 9; ++a) {
+		free(c->jac[a]);
+		free(c->jinv[a]);
+	}
+	for (int tl = 0; tl < 3; ++tl)
+		for (int a = 0; a < 3; ++a)
+			free(c->ur[tl][a]);
+	free(c->neigh_copy);
 	free(c->bc);
 	free(c->part);
 	free(c);
@@ -1231,4 +1260,333 @@ int orc_get_threads(void)
 #else
 	return 1;
 #endif
+}
+
+/* ------------------------------------------------------------------ */
+/* Particles: interpolation.c:41-104, particles.c:182-402, 493-539      */
+/* ------------------------------------------------------------------ */
+
+/* interpolation.c:98-104 */
+static int first_point(const double *x, double xp, int n, int m)
+{
+	for (int i = 0; i < n - m + 1; ++i)
+		if (x[i] > xp)
+			return (i - m / 2) > 0 ? i - m / 2 : 0;
+	return n - m;
+}
+
+double orc_interp_point(int n, const double *xi, const double *f, double x,
+		double y, double z)
+{
+	if (n <= 5) {           /* lagrangian_interpolation, :41-65 */
+		double lx[NMAX], ly[NMAX], lz[NMAX], tmp[NMAX * NMAX];
+		for (int i = 0; i < n; ++i) {
+			lx[i] = lagrange(x, xi, i, n);
+			ly[i] = lagrange(y, xi, i, n);
+			lz[i] = lagrange(z, xi, i, n);
+		}
+		/* dgemv: tmp[r] = sum_k f[k][r] lz[k]; then sum_j tmp[j][i] ly[j];
+		 * then the dot with lx */
+		for (int r = 0; r < n * n; ++r) {
+			double s = 0.0;
+			for (int k = 0; k < n; ++k)
+				s += f[k * n * n + r] * lz[k];
+			tmp[r] = s;
+		}
+		double out = 0.0;
+		for (int i = 0; i < n; ++i) {
+			double s = 0.0;
+			for (int j = 0; j < n; ++j)
+				s += tmp[j * n + i] * ly[j];
+			out += lx[i] * s;
+		}
+		return out;
+	}
+	const int m = 5;        /* :67-96 */
+	const int i1 = first_point(xi, x, n, m), j1 = first_point(xi, y, n, m),
+		k1 = first_point(xi, z, n, m);
+	double lx[5], ly[5], lz[5];
+	for (int i = 0; i < m; ++i) {
+		lx[i] = lagrange(x, xi + i1, i, m);
+		ly[i] = lagrange(y, xi + j1, i, m);
+		lz[i] = lagrange(z, xi + k1, i, m);
+	}
+	double tmp = 0.0;
+	for (int k = 0; k < m; ++k)
+		for (int j = 0; j < m; ++j)
+			for (int i = 0; i < m; ++i)
+				tmp += f[((k1 + k) * n + j1 + j) * n + i1 + i]
+					* lx[i] * ly[j] * lz[k];
+	return tmp;
+}
+
+void orc_particles_set_velocity(orc_ctx *c, const double *u, const double *v,
+		const double *w)
+{
+	const size_t nt = c->lev[0].ntot;
+	const double *uu[3] = { u, v, w };
+	/* time_shift: old -> older, now -> old (time.c:37-41) */
+	for (int a = 0; a < 3; ++a) {
+		memcpy(c->ur[2][a], c->ur[1][a], nt * sizeof(double));
+		memcpy(c->ur[1][a], c->ur[0][a], nt * sizeof(double));
+	}
+	/* ur_i = sum_j (J^-1)_ij u_j, first term replaces (tripalg.c:224-234) */
+	for (size_t q = 0; q < nt; ++q)
+		for (int i = 0; i < 3; ++i) {
+			double s = c->jinv[i * 3 + 0][q] * uu[0][q];
+			s = c->jinv[i * 3 + 1][q] * uu[1][q] + s;
+			s = c->jinv[i * 3 + 2][q] * uu[2][q] + s;
+			c->ur[0][i][q] = s;
+		}
+}
+
+enum { P_NORMAL, P_COLLISION, P_OUTSIDE };
+
+static double field_at(orc_ctx *c, const double *f, int el, double x,
+		double y, double z)
+{
+	const level_t *L = &c->lev[0];
+	return orc_interp_point(L->n, L->b.xi, f + (size_t) el * L->n3, x, y, z);
+}
+
+/* particles.c:493-515 */
+static double fluid_velocity(orc_ctx *c, int el, const double *y, double t,
+		int comp)
+{
+	const double eps = 1.0e-6, T = c->pt, dt = c->pdt;
+	double u[3] = { 0.0, 0.0, 0.0 };
+	const double d[3] = { t - T + 2.0 * dt, t - T + dt, t - T };
+	const double a = 0.5 * d[1] * d[2] / pow(dt, 2);
+	const double b = -d[0] * d[2] / pow(dt, 2);
+	const double cc = 0.5 * d[0] * d[1] / pow(dt, 2);
+	if (fabs(a) > eps)
+		u[0] = field_at(c, c->ur[2][comp], el, y[0], y[1], y[2]);
+	if (fabs(b) > eps)
+		u[1] = field_at(c, c->ur[1][comp], el, y[0], y[1], y[2]);
+	if (fabs(cc) > eps)
+		u[2] = field_at(c, c->ur[0][comp], el, y[0], y[1], y[2]);
+	return a * u[0] + b * u[1] + cc * u[2];
+}
+
+static void prhs(orc_ctx *c, const orc_particle *p, const double *y,
+		double t, double *f)
+{
+	const double a = 1.0 / p->tau;          /* particles.c:228-236 */
+	for (int i = 0; i < 3; ++i) {
+		f[i] = y[i + 3];
+		f[i + 3] = a * (fluid_velocity(c, p->el, y, t, i) - y[i + 3]);
+	}
+}
+
+static int crossed(const double *y)
+{
+	for (int i = 0; i < 3; ++i)
+		if (fabs(y[i]) > 1.0)
+			return 1;
+	return 0;
+}
+
+static void reinit(orc_ctx *c, orc_particle *p)
+{
+	/* the reference re-seeds with rand() (particles.c:170-180); the
+	 * restatement places it at the element centre and flags it */
+	double y[6] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
+	p->x = p->y = p->z = 0.0;
+	p->u = fluid_velocity(c, p->el, y, c->pt, 0);
+	p->v = fluid_velocity(c, p->el, y, c->pt, 1);
+	p->w = fluid_velocity(c, p->el, y, c->pt, 2);
+	p->reinit = 1;
+}
+
+/* particles.c:238-269; force: ignore intermediate crossings */
+static void rk4(orc_ctx *c, orc_particle *p, double dt, int force)
+{
+	static const double a[4] = { 0.0, 0.5, 0.5, 1.0 };
+	double y[6], f[4][6];
+	const double y0[6] = { p->x, p->y, p->z, p->u, p->v, p->w };
+	for (int j = 0; j < 4; ++j) {
+		const double t = p->t + a[j] * dt;
+		for (int i = 0; i < 6; ++i)
+			y[i] = j == 0 ? y0[i] : y0[i] + a[j] * dt * f[j - 1][i];
+		if (j > 0 && crossed(y) && !force) {
+			p->status = P_OUTSIDE;
+			return;
+		}
+		prhs(c, p, y, t, f[j]);
+	}
+	for (int i = 0; i < 6; ++i)
+		y[i] = y0[i] + dt * (f[0][i] / 6.0 + f[1][i] / 3.0 + f[2][i] / 3.0
+				+ f[3][i] / 6.0);
+	if (crossed(y)) {
+		p->status = P_OUTSIDE;
+		return;
+	}
+	p->x = y[0]; p->y = y[1]; p->z = y[2];
+	p->u = y[3]; p->v = y[4]; p->w = y[5];
+	p->t = c->pt;
+}
+
+static double dmin2(double a, double b)
+{
+	return a < b ? a : b;                   /* utils.c:220-226 */
+}
+
+static double step_to_boundary(const double *y, const double *f)
+{
+	double d[3];                            /* particles.c:310-316 */
+	for (int i = 0; i < 3; ++i)
+		d[i] = (copysign(1.0, f[i]) - y[i]) / f[i];
+	return dmin2(dmin2(d[0], d[1]), d[2]);
+}
+
+/* particles.c:271-308 */
+static void to_boundary(orc_ctx *c, orc_particle *p, double dt)
+{
+	static const double a[4] = { 0.0, 0.5, 0.5, 1.0 };
+	double y[6], f[5][6];
+	const double y0[6] = { p->x, p->y, p->z, p->u, p->v, p->w };
+	double *yn[6] = { &p->x, &p->y, &p->z, &p->u, &p->v, &p->w };
+	double dtau = 0.0;
+	for (int j = 0; j < 4; ++j) {
+		const double t = p->t + a[j] * dtau;
+		for (int i = 0; i < 6; ++i)
+			y[i] = j == 0 ? y0[i] : y0[i] + a[j] * dtau * f[j - 1][i];
+		prhs(c, p, y, t, f[j]);
+		dtau = step_to_boundary(y0, f[j]);
+	}
+	for (int i = 0; i < 6; ++i)
+		f[4][i] = (f[0][i] / 6.0 + f[1][i] / 3.0 + f[2][i] / 3.0
+				+ f[3][i] / 6.0);
+	dtau = step_to_boundary(y0, f[4]);
+	if (dtau < dt) {
+		p->t += dtau;
+		for (int i = 0; i < 6; ++i)
+			*yn[i] += dtau * f[4][i];
+	} else {
+		p->status = P_NORMAL;
+		rk4(c, p, dt, 1);
+		if (p->status != P_NORMAL) {
+			p->status = P_NORMAL;
+			reinit(c, p);
+			p->t = c->pt;
+		}
+	}
+}
+
+/* particles.c:380-393 */
+static int closest(const orc_particle *p)
+{
+	const double x[3] = { p->x, p->y, p->z };
+	double hmin = 2.0;
+	int k = 0;
+	for (int i = 0; i < 6; ++i) {
+		const double h = fabs(x[i / 2] - ((i % 2) ? 1.0 : -1.0));
+		if (h < hmin) {
+			hmin = h;
+			k = i;
+		}
+	}
+	return k;
+}
+
+/* physical_particle_velocity / parametric_particle_velocity, :517-539;
+ * the callers overwrite u, v, w one component at a time, so later
+ * components see the already converted earlier ones (kept as is) */
+static double phys_vel(orc_ctx *c, const orc_particle *p, int comp)
+{
+	double dx[3];
+	for (int i = 0; i < 3; ++i)
+		dx[i] = field_at(c, c->jac[comp * 3 + i], p->el, p->x, p->y, p->z);
+	return p->u * dx[0] + p->v * dx[1] + p->w * dx[2];
+}
+
+static double param_vel(orc_ctx *c, const orc_particle *p, int comp)
+{
+	double dx[3];                   /* tinv_jac[i][comp] = (J^-1)[comp][i] */
+	for (int i = 0; i < 3; ++i)
+		dx[i] = field_at(c, c->jinv[comp * 3 + i], p->el, p->x, p->y, p->z);
+	return p->u * dx[0] + p->v * dx[1] + p->w * dx[2];
+}
+
+/* particles.c:333-349 */
+static void new_element(orc_ctx *c, orc_particle *p)
+{
+	double *u[3] = { &p->u, &p->v, &p->w };
+	const int j = closest(p);
+	const int nb = c->neigh_copy[p->el * 6 + j];
+	if (nb >= 0) {
+		for (int i = 0; i < 3; ++i)
+			*u[i] = phys_vel(c, p, i);
+		p->el = nb;
+	} else {
+		p->status = P_COLLISION;
+		*u[j / 2] *= -1.0;
+	}
+}
+
+/* particles.c:351-378 (neigh_rot is always 0) */
+static void new_location(orc_ctx *c, orc_particle *p)
+{
+	const double y0[6] = { p->x, p->y, p->z, p->u, p->v, p->w };
+	double *yn[6] = { &p->x, &p->y, &p->z, &p->u, &p->v, &p->w };
+	double x2[2];
+	int k = closest(p);
+	int j = 0;
+	for (int i = 0; i < 3; ++i)
+		if (i != k / 2)
+			x2[j++] = y0[i];
+	for (int i = 0; i < 6; ++i)
+		if (c->neigh_copy[p->el * 6 + i] == p->old_el)
+			k = i;
+	*yn[k / 2] = (k % 2) ? 1.0 : -1.0;
+	j = 0;
+	for (int i = 0; i < 3; ++i) {
+		if (i != k / 2)
+			*yn[i] = x2[j++];
+		*yn[i + 3] = param_vel(c, p, i);
+	}
+	p->old_el = p->el;
+}
+
+int orc_particles_step(orc_ctx *c, orc_particle *ps, long n, double t_now,
+		double dt)
+{
+	c->pt = t_now;
+	c->pdt = dt;
+	int maxpass = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(max:maxpass)
+	for (long q = 0; q < n; ++q) {
+		orc_particle *p = ps + q;
+		p->status = P_OUTSIDE;
+		int j = 0;
+		do {                            /* particles.c:187-217 */
+			if (p->status != P_NORMAL) {
+				p->status = P_NORMAL;
+				if (j > 10) {
+					reinit(c, p);
+					p->t = c->pt;
+				} else {
+					rk4(c, p, c->pt - p->t, 0);
+					if (p->status == P_OUTSIDE)
+						to_boundary(c, p, 2.0 * dt);
+					if (p->status == P_OUTSIDE)
+						new_element(c, p);
+				}
+			}
+			if (p->status == P_OUTSIDE)
+				new_location(c, p);
+			++j;
+		} while (p->status != P_NORMAL);
+		if (j > maxpass)
+			maxpass = j;
+	}
+	return maxpass;
+}
+
+void orc_particle_position(orc_ctx *c, const orc_particle *p, double *xyz)
+{
+	const level_t *L = &c->lev[0];
+	const double *xs[3] = { L->x, L->y, L->z };
+	for (int a = 0; a < 3; ++a)
+		xyz[a] = field_at(c, xs[a], p->el, p->x, p->y, p->z);
 }

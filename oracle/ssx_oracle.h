@@ -121,6 +121,33 @@ void orc_set_diag_choice(orc_ctx *c, int choice);
 /* Cumulative counts since creation: [0] fine-level opeval calls,
  * [1] coarse CG iterations, [2] smoother CG iterations, [3] V-cycles. */
 void orc_counters(const orc_ctx *c, long long *out4);
+/* --- particles (particles.c) --- */
+/* particle state as struct particle (particles.h:25-41), single rank */
+typedef struct orc_particle {
+	long id;
+	int el, old_el, status;
+	double x, y, z;         /* reference coordinates in element el */
+	double u, v, w;         /* velocity in reference coordinates   */
+	double t, tau;
+	int reinit;             /* the reference would re-seed it randomly */
+} orc_particle;
+/* Point interpolation of an element field f (n^3, [k][j][i]) at reference
+ * coordinates (interpolation.c:41-104): full tensor Lagrange for n <= 5,
+ * else the 5-point stencil around the point. */
+double orc_interp_point(int n, const double *xi, const double *f, double x,
+		double y, double z);
+/* compute_reference_domain_velocity (geometry.c:188-198): shifts the
+ * reference-space velocity history (time.c:30-41) and sets
+ * ur = J^-1 (u,v,w) from physical E-vectors at level 0. */
+void orc_particles_set_velocity(orc_ctx *c, const double *u, const double *v,
+		const double *w);
+/* integrate_particle_equations (particles.c:182-218) to time t_now with
+ * time step dt (params->t, params->dt); returns the number of passes. */
+int orc_particles_step(orc_ctx *c, orc_particle *p, long n, double t_now,
+		double dt);
+/* physical coordinates of a particle (particles.c:539-544) */
+void orc_particle_position(orc_ctx *c, const orc_particle *p, double *xyz);
+
 /* OpenMP threads used by the element loops (1 = serial) */
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

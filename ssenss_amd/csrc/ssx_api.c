@@ -388,6 +388,9 @@ int ssx_create(ssx_ctx **out, const ssx_mesh *m)
 	}
 	c->own_stream = st;
 	c->stream = st;
+	c->h_neigh = malloc(sizeof(int) * 6 * (size_t) m->nel);
+	if (c->h_neigh)
+		memcpy(c->h_neigh, m->neigh, sizeof(int) * 6 * (size_t) m->nel);
 	int rc = SSX_OK;
 	/* coordinates of the previous (finer) level; level 0 keeps its own */
 	double *px = NULL, *py = NULL, *pz = NULL;
@@ -511,6 +514,15 @@ void ssx_destroy(ssx_ctx *c)
 		hipFree(c->fcg_ap[i]);
 	hipFree(c->d_f);
 	hipFree(c->d_u);
+	free(c->h_neigh);
+	hipFree(c->d_neigh);
+	for (int a = 0; a < 9; ++a) {
+		hipFree(c->p_jac[a]);
+		hipFree(c->p_jinv[a]);
+		hipFree(c->p_ur[a / 3][a % 3]);
+	}
+	hipFree(c->p_xi);
+	hipFree(c->p_maxpass);
 	if (c->h_scal)
 		hipHostFree(c->h_scal);
 	if (c->own_stream)
@@ -1342,4 +1354,156 @@ int ssx_solve_helmholtz_cg(ssx_ctx *c, double *f, double *u, int var,
 {
 	TRY(chk(c, 0));
 	return host_solve(c, f, u, var, eps, 1, stats);
+}
+
+/* ------------------------------------------------------------------ */
+/* particles                                                            */
+/* ------------------------------------------------------------------ */
+
+/* J and J^-1 of level 0 in the reference's operation order
+ * (deriv.c:68-79 derivatives summed in index order, tripalg.c:165-222
+ * determinant and cofactors), so that particle positions and owning
+ * elements follow the reference bit for bit */
+static void particle_jacobians(const ssx_basis *b, int nel, const double *x,
+		const double *y, const double *z, double *jac[9], double *jinv[9])
+{
+	const int n = b->n, n3 = n * n * n;
+	const double *D = b->d;
+	static const int k0[3] = { 1, 0, 0 }, k1[3] = { 2, 2, 1 },
+		k2[3] = { 2, 0, 1 }, k3[3] = { 1, 2, 0 };
+	static const double sg[3] = { 1.0, -1.0, 1.0 };
+	for (int e = 0; e < nel; ++e) {
+		const size_t off = (size_t) e * n3;
+		const double *c[3] = { x + off, y + off, z + off };
+		for (int k = 0; k < n; ++k)
+		for (int j = 0; j < n; ++j)
+		for (int i = 0; i < n; ++i) {
+			const int q = (k * n + j) * n + i;
+			double a[3][3], ai[3][3];
+			for (int r = 0; r < 3; ++r) {
+				double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+				for (int l = 0; l < n; ++l) {
+					s0 += D[i * n + l] * c[r][(k * n + j) * n + l];
+					s1 += c[r][(k * n + l) * n + i] * D[j * n + l];
+					s2 += c[r][(l * n + j) * n + i] * D[k * n + l];
+				}
+				a[r][0] = s0;
+				a[r][1] = s1;
+				a[r][2] = s2;
+			}
+			double t, det;
+			t = a[1][1] * a[2][2];
+			t = -a[1][2] * a[2][1] + t;
+			det = a[0][0] * t;
+			t = a[1][0] * a[2][2];
+			t = -a[1][2] * a[2][0] + t;
+			det = -a[0][1] * t + det;
+			t = a[1][0] * a[2][1];
+			t = -a[1][1] * a[2][0] + t;
+			det = a[0][2] * t + det;
+			for (int ii = 0; ii < 3; ++ii)
+				for (int jj = 0; jj < 3; ++jj) {
+					double u = a[k0[jj]][k3[ii]] * a[k1[jj]][k2[ii]];
+					u = -a[k0[jj]][k2[ii]] * a[k1[jj]][k3[ii]] + u;
+					ai[ii][jj] = sg[jj] * u / det;
+				}
+			for (int r = 0; r < 3; ++r)
+				for (int s = 0; s < 3; ++s) {
+					jac[r * 3 + s][off + q] = a[r][s];
+					jinv[r * 3 + s][off + q] = ai[r][s];
+				}
+		}
+	}
+}
+
+static int particles_init(ssx_ctx *c)
+{
+	if (c->p_jac[0])
+		return SSX_OK;
+	ssx_level *L = &c->lev[0];
+	const size_t nt = L->ntot;
+	double *hj[9], *hi[9];
+	int rc = SSX_OK;
+	for (int a = 0; a < 9; ++a) {
+		hj[a] = malloc(sizeof(double) * nt);
+		hi[a] = malloc(sizeof(double) * nt);
+		if (!hj[a] || !hi[a])
+			rc = SSX_ENOMEM;
+	}
+	if (rc == SSX_OK) {
+		particle_jacobians(&L->b, c->nel, L->h_x, L->h_y, L->h_z, hj, hi);
+		for (int a = 0; a < 9 && rc == SSX_OK; ++a) {
+			rc = up(&c->p_jac[a], hj[a], nt);
+			if (!rc)
+				rc = up(&c->p_jinv[a], hi[a], nt);
+			if (!rc)
+				rc = dalloc(&c->p_ur[a / 3][a % 3], nt);
+			if (!rc)
+				HIPC(hipMemset(c->p_ur[a / 3][a % 3], 0, nt * sizeof(double)));
+		}
+	}
+	for (int a = 0; a < 9; ++a) {
+		free(hj[a]);
+		free(hi[a]);
+	}
+	if (rc == SSX_OK)
+		rc = up(&c->p_xi, L->b.xi, (size_t) L->n);
+	if (rc == SSX_OK) {
+		if (hipMalloc((void **) &c->d_neigh, sizeof(int) * 6 * c->nel)
+				!= hipSuccess || hipMalloc((void **) &c->p_maxpass,
+				sizeof(int)) != hipSuccess) {
+			ssx_set_error("hipMalloc particle tables");
+			return SSX_ENOMEM;
+		}
+		HIPC(hipMemcpy(c->d_neigh, c->h_neigh, sizeof(int) * 6 * c->nel,
+				hipMemcpyHostToDevice));
+	}
+	if (rc != SSX_OK)
+		ssx_set_error("particle set-up failed");
+	return rc;
+}
+
+int ssx_particles_set_velocity(ssx_ctx *c, const double *u, const double *v,
+		const double *w)
+{
+	TRY(chk(c, 0));
+	TRY(particles_init(c));
+	/* time_shift (time.c:37-41): now -> old -> older, by rotation */
+	for (int a = 0; a < 3; ++a) {
+		double *older = c->p_ur[2][a];
+		c->p_ur[2][a] = c->p_ur[1][a];
+		c->p_ur[1][a] = c->p_ur[0][a];
+		c->p_ur[0][a] = older;
+	}
+	return ssxk_particles_ur(c->stream, c->lev[0].ntot,
+			(const double *const *) c->p_jinv, u, v, w, c->p_ur[0][0],
+			c->p_ur[0][1], c->p_ur[0][2]);
+}
+
+int ssx_particles_step(ssx_ctx *c, const ssx_particles *p, double t_now,
+		double dt, int *max_passes)
+{
+	TRY(chk(c, 0));
+	if (!p || p->n < 0) {
+		ssx_set_error("bad particle list");
+		return SSX_EINVAL;
+	}
+	TRY(particles_init(c));
+	const double *ur[9];
+	for (int a = 0; a < 9; ++a)
+		ur[a] = c->p_ur[a / 3][a % 3];
+	TRY(ssxk_set_ints(c->stream, c->p_maxpass, 0, 1));
+	TRY(ssxk_particles_step(c->stream, c->lev[0].n, c->p_xi, ur,
+			(const double *const *) c->p_jac,
+			(const double *const *) c->p_jinv, c->d_neigh, t_now, dt, p->n,
+			p->el, p->old_el, p->x, p->y, p->z, p->u, p->v, p->w, p->t,
+			p->tau, p->reinit, c->p_maxpass));
+	if (max_passes) {
+		int mp = 0;
+		HIPC(hipMemcpyAsync(&mp, c->p_maxpass, sizeof(int),
+				hipMemcpyDeviceToHost, (hipStream_t) c->stream));
+		HIPC(hipStreamSynchronize((hipStream_t) c->stream));
+		*max_passes = mp;
+	}
+	return SSX_OK;
 }

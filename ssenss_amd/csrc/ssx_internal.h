@@ -94,6 +94,10 @@ struct ssx_ctx {
 	double *fcg_r, *fcg_z, *fcg_p[11], *fcg_ap[10];
 	/* host staging for the host-array entry points */
 	double *d_f, *d_u;
+	/* particles (level 0): J, J^-1, reference-space velocity history */
+	int *h_neigh, *d_neigh;
+	double *p_jac[9], *p_jinv[9], *p_ur[3][3], *p_xi;
+	int *p_maxpass;
 	long long cnt[4];
 };
 
@@ -164,6 +168,15 @@ int ssxk_shift_s_lp(void *stream, size_t n, const double *scal, int ia,
 		double nunique, void *y);
 int ssxk_fill_lp(void *stream, void *dst, double v, size_t n);
 int ssxk_copy_lp(void *stream, void *dst, const void *src, size_t n);
+int ssxk_particles_ur(void *stream, size_t nt, const double *const jinv[9],
+		const double *u, const double *v, const double *w, double *u0,
+		double *u1, double *u2);
+int ssxk_particles_step(void *stream, int n_pts, const double *xi,
+		const double *const ur[9], const double *const jac[9],
+		const double *const jinv[9], const int *neigh, double T, double dt,
+		long np, int *el, int *old_el, double *x, double *y, double *z,
+		double *u, double *v, double *w, double *t, const double *tau,
+		int *reinit, int *max_pass);
 /* precision conversion (SSX_PREC_FP64 -> FP32 / BF16, FP32 -> FP64) */
 int ssxk_cvt(void *stream, int from, int to, const void *x, void *y,
 		size_t n);

@@ -5,7 +5,7 @@ import ctypes as C
 
 import numpy as np
 
-from .native import Stats, check, load
+from .native import ParticlesStruct, Stats, check, load
 
 
 def _ptr(t):
@@ -160,3 +160,21 @@ class Context:
                                                u.ctypes.data_as(dp), var, eps,
                                                C.byref(st)))
         return st
+
+    # --- particles (torch device tensors) ---
+    def particles_set_velocity(self, u, v, w):
+        check(self._lib.ssx_particles_set_velocity(self.h, _ptr(u), _ptr(v),
+                                                   _ptr(w)))
+
+    def particles_step(self, p, t_now, dt):
+        """p: dict with int32 tensors el, old_el, reinit and float64
+        tensors x, y, z, u, v, w, t, tau (all on the GPU)"""
+        ps = ParticlesStruct(int(p["x"].numel()),
+                             *(p[k].data_ptr() if p.get(k) is not None
+                               else None for k in
+                               ("el", "old_el", "x", "y", "z", "u", "v", "w",
+                                "t", "tau", "reinit")))
+        mp = C.c_int()
+        check(self._lib.ssx_particles_step(self.h, C.byref(ps), t_now, dt,
+                                           C.byref(mp)))
+        return mp.value

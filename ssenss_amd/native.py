@@ -27,7 +27,8 @@ EXPORTED_SYMBOLS = [
     "ssx_opeval", "ssx_dss", "ssx_dss_average", "ssx_dot", "ssx_mask",
     "ssx_nullspace", "ssx_restrict", "ssx_prolong", "ssx_jacobi", "ssx_cg",
     "ssx_fcg", "ssx_mg_precon", "ssx_solve_poisson_cg",
-    "ssx_solve_helmholtz_cg",
+    "ssx_solve_helmholtz_cg", "ssx_particles_set_velocity",
+    "ssx_particles_step",
 ]
 
 
@@ -48,6 +49,11 @@ class MeshStruct(C.Structure):
         ("dt", C.c_double), ("visc", C.c_double),
         ("svvn", C.c_int), ("svva", C.c_double),
     ]
+
+
+class ParticlesStruct(C.Structure):
+    _fields_ = [("n", C.c_long)] + [(f, C.c_void_p) for f in (
+        "el", "old_el", "x", "y", "z", "u", "v", "w", "t", "tau", "reinit")]
 
 
 class Stats(C.Structure):
@@ -119,6 +125,9 @@ def load():
         "ssx_mg_precon": (i, [vp, i, i, d, vp, vp]),
         "ssx_solve_poisson_cg": (i, [vp, dp, dp, i, d, C.POINTER(Stats)]),
         "ssx_solve_helmholtz_cg": (i, [vp, dp, dp, i, d, C.POINTER(Stats)]),
+        "ssx_particles_set_velocity": (i, [vp, vp, vp, vp]),
+        "ssx_particles_step": (i, [vp, C.POINTER(ParticlesStruct), d, d,
+                                   C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

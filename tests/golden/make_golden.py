@@ -39,16 +39,17 @@ H5DUMP = "/opt/conda/bin/h5dump"
 CASES = [
     # name, n, m, lo, hi, bc, deform, amp, dt, visc, seed
     dict(name="box2_n5", n=5, m=(2, 2, 2), lo=(-1, -1, -1), hi=(1, 1, 1),
-         bc=(NOSLIP,) * 6, deform=1, amp=0.1, dt=11 / 60, visc=1.0, seed=11),
+         bc=(NOSLIP,) * 6, deform=1, amp=0.1, dt=11 / 60, visc=1.0, seed=11,
+         particles=1000),
     dict(name="cube2_n8", n=8, m=(2, 2, 2), lo=(0, 0, 0), hi=(1, 1, 1),
          bc=(NOSLIP,) * 6, deform=1, amp=0.1, dt=11 / 600, visc=1.0, seed=12),
     dict(name="periodic3_n6", n=6, m=(3, 3, 3), lo=(0, 0, 0),
          hi=(2 * np.pi,) * 3, bc=(PERIODIC,) * 6, deform=0, amp=0.0,
-         dt=1e-3, visc=1 / 1600, seed=13),
+         dt=1e-3, visc=1 / 1600, seed=13, particles=1000),
     dict(name="channel_n6", n=6, m=(3, 2, 3), lo=(0, -1, 0),
          hi=(2 * np.pi, 1, np.pi), bc=(PERIODIC, PERIODIC, NOSLIP, NOSLIP,
                                        PERIODIC, PERIODIC),
-         deform=2, amp=2.0, dt=2e-3, visc=1 / 180, seed=14),
+         deform=2, amp=2.0, dt=2e-3, visc=1 / 180, seed=14, particles=1000),
     dict(name="cube2_n12", n=12, m=(2, 2, 2), lo=(0, 0, 0), hi=(1, 1, 1),
          bc=(NOSLIP,) * 6, deform=1, amp=0.1, dt=11 / 600, visc=1.0, seed=15),
 ]
@@ -78,6 +79,32 @@ def weak_rhs(mesh, f_nodal):
     detj = orc.data(0, pyoracle.DETJ)
     b = np.tile(w3, mesh.nel) * detj * f_nodal.ravel()
     return orc.dss(0, b)
+
+
+PARTICLE_T, PARTICLE_DT, PARTICLE_SCALES = 1.0, 0.05, (0.8, 0.9, 1.0)
+
+
+def particle_inputs(mesh, lo, hi, count, seed):
+    """A smooth periodic velocity field (three history levels are this
+    field times PARTICLE_SCALES, oldest first) and `count` particles at
+    random reference positions with random reference velocities, last
+    updated two time steps ago (particles.c is called every 2nd step)."""
+    L = np.asarray(hi, float) - np.asarray(lo, float)
+    X = 2 * np.pi * (mesh.x - lo[0]) / L[0]
+    Y = 2 * np.pi * (mesh.y - lo[1]) / L[1]
+    Zc = 2 * np.pi * (mesh.z - lo[2]) / L[2]
+    u = np.sin(X) * np.cos(Y) * np.cos(Zc)
+    v = -np.cos(X) * np.sin(Y) * np.cos(Zc)
+    w = 0.5 * np.sin(X + Zc)
+    vel = np.concatenate([u.ravel(), v.ravel(), w.ravel()])
+    rng = np.random.default_rng(seed)
+    rec = np.zeros((count, 9))
+    rec[:, 0] = rng.integers(0, mesh.nel, count)
+    rec[:, 1:4] = rng.uniform(-0.95, 0.95, (count, 3))
+    rec[:, 4:7] = 1.5 * rng.standard_normal((count, 3))
+    rec[:, 7] = PARTICLE_T - 2 * PARTICLE_DT
+    rec[:, 8] = np.exp(rng.uniform(np.log(0.1), 0.0, count))
+    return vel, rec
 
 
 def write_mesh_bin(path, mesh):
@@ -126,6 +153,19 @@ def run_case(c):
         u.tofile(os.path.join(d, "in_u.bin"))
         b.tofile(os.path.join(d, "in_f.bin"))
         v1.tofile(os.path.join(d, "in_v1.bin"))
+        pin = {}
+        if c.get("particles"):
+            vel, rec = particle_inputs(mesh, c["lo"], c["hi"], c["particles"],
+                                       c["seed"] + 2000)
+            vel.tofile(os.path.join(d, "in_vel.bin"))
+            with open(os.path.join(d, "in_particles.bin"), "wb") as fh:
+                fh.write(np.array([len(rec)], np.int32).tobytes())
+                fh.write(np.array([PARTICLE_T, PARTICLE_DT,
+                                   *PARTICLE_SCALES]).tobytes())
+                fh.write(rec.tobytes())
+            pin = dict(in_vel=vel, in_particles=rec,
+                       particle_hdr=np.array([PARTICLE_T, PARTICLE_DT,
+                                              *PARTICLE_SCALES]))
         env = dict(os.environ, MKL_CBWR="COMPATIBLE", MKL_NUM_THREADS="1",
                    MKL_THREADING_LAYER="SEQUENTIAL", OMP_NUM_THREADS="1")
         subprocess.run([HARNESS, "mesh", d], check=True, env=env)
@@ -135,7 +175,7 @@ def run_case(c):
             edge_size=mesh.edge_size, corn_neigh=mesh.corn_neigh,
             corn_size=mesh.corn_size, dt=np.float64(mesh.dt),
             visc=np.float64(mesh.visc), in_u=u, in_f=b, in_v1=v1,
-            levels=np.array(levels, np.int32),
+            levels=np.array(levels, np.int32), **pin,
         )
         for name in sorted(os.listdir(d)):
             if name.startswith("ref_"):

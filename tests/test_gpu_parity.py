@@ -327,3 +327,81 @@ def test_mixed_precision_smoother(name, prec):
             assert err <= 1e-7
         ctx.set_option(S.OPT_SMOOTHER_PRECISION, 0)
         ctx.set_diag_choice(-1)
+
+
+def _dev_particles(rec):
+    n = len(rec)
+    ip = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int32)).cuda()
+    return {"el": ip(rec[:, 0]), "old_el": ip(rec[:, 0]),
+            "x": dev(rec[:, 1]), "y": dev(rec[:, 2]), "z": dev(rec[:, 3]),
+            "u": dev(rec[:, 4]), "v": dev(rec[:, 5]), "w": dev(rec[:, 6]),
+            "t": dev(rec[:, 7]), "tau": dev(rec[:, 8]),
+            "reinit": ip(np.zeros(n))}
+
+
+def _host_particles(p):
+    torch.cuda.synchronize()
+    cols = [p["el"].cpu().numpy().astype(np.float64)] + [
+        p[k].cpu().numpy() for k in ("x", "y", "z", "u", "v", "w", "t")]
+    return np.stack(cols, 1), p["reinit"].cpu().numpy()
+
+
+def _run_gpu_particles(ctx, vel, scales, rec, t_now, dt):
+    nt = ctx.ntot(0)
+    for s in scales:
+        ctx.particles_set_velocity(dev(s * vel[:nt]), dev(s * vel[nt:2 * nt]),
+                                   dev(s * vel[2 * nt:]))
+    p = _dev_particles(rec)
+    passes = ctx.particles_step(p, t_now, dt)
+    out, re = _host_particles(p)
+    return out, re, passes
+
+
+@pytest.mark.parametrize("name", ["box2_n5", "periodic3_n6", "channel_n6"])
+def test_particles_against_reference(name):
+    """owning elements bit-exact against the compiled reference; states
+    within 1e-12 (the engine's GLL basis and J^-1 agree with the
+    reference's to rounding)"""
+    Z = golden(name)
+    h = Z["particle_hdr"]
+    with make_ctx(golden_mesh(Z)) as ctx:
+        out, re, passes = _run_gpu_particles(ctx, Z["in_vel"], h[2:],
+                                             Z["in_particles"], h[0], h[1])
+    ref = Z["ref_particles"].reshape(-1, 8)
+    print(f"[gpu] particles {name}: passes {passes}, max state diff "
+          f"{np.abs(out[:, 1:] - ref[:, 1:]).max():.2e}")
+    assert re.sum() == 0
+    assert np.array_equal(out[:, 0], ref[:, 0])
+    assert np.abs(out[:, 1:] - ref[:, 1:]).max() <= 1e-12
+
+
+def test_particles_many_hops_against_oracle():
+    """20k particles, fast and heavy, several hops per step on a periodic
+    4^3 mesh at n = 8 (stencil interpolation): elements bit-exact against
+    the oracle"""
+    mesh = box_mesh(8, (4, 4, 4), lo=(0, 0, 0), hi=(2 * np.pi,) * 3,
+                    bc=(PERIODIC,) * 6)
+    rng = np.random.default_rng(31)
+    X, Y, Zc = mesh.x, mesh.y, mesh.z
+    vel = np.concatenate([(np.sin(X) * np.cos(Y) * np.cos(Zc)).ravel(),
+                          (-np.cos(X) * np.sin(Y) * np.cos(Zc)).ravel(),
+                          (0.5 * np.sin(X + Zc)).ravel()])
+    n = 20000
+    rec = np.zeros((n, 9))
+    rec[:, 0] = rng.integers(0, mesh.nel, n)
+    rec[:, 1:4] = rng.uniform(-0.99, 0.99, (n, 3))
+    rec[:, 4:7] = 4.0 * rng.standard_normal((n, 3))
+    rec[:, 7] = 1.0 - 2 * 0.2
+    rec[:, 8] = np.exp(rng.uniform(np.log(0.1), 0.0, n))
+    o = P.Oracle(mesh)
+    ref, re_o, passes_o = o.particles(vel, (0.8, 0.9, 1.0), rec, 1.0, 0.2)
+    with make_ctx(mesh) as ctx:
+        out, re, passes = _run_gpu_particles(ctx, vel, (0.8, 0.9, 1.0), rec,
+                                             1.0, 0.2)
+    hops = (out[:, 0] != rec[:, 0]).sum()
+    print(f"[gpu] 20k particles: {hops} changed element, passes {passes} "
+          f"(oracle {passes_o}), reinit {re.sum()}")
+    assert hops > 2000
+    assert np.array_equal(out[:, 0], ref[:, 0])
+    assert np.array_equal(re, re_o)
+    assert np.abs(out[:, 1:] - ref[:, 1:]).max() <= 1e-10

@@ -130,3 +130,41 @@ def test_oracle_thread_count_does_not_change_results(threads):
         P.set_threads(1)
     o2 = P.Oracle(golden_mesh(Z))
     assert np.array_equal(a, o2.ax(0, P.OP_POISSON, Z["in_u"]))
+
+
+@pytest.mark.parametrize("name", ["box2_n5", "periodic3_n6", "channel_n6"])
+def test_particles_match_reference(name):
+    """integrate_particle_equations with injected particles: owning
+    elements bit-exact; states bit-exact where the reference interpolates
+    with plain loops (n > 5), to rounding where it calls BLAS (n <= 5)."""
+    from conftest import golden, golden_mesh
+    Z = golden(name)
+    o = P.Oracle(golden_mesh(Z))
+    h = Z["particle_hdr"]
+    out, re, passes = o.particles(Z["in_vel"], h[2:], Z["in_particles"],
+                                  h[0], h[1])
+    ref = Z["ref_particles"].reshape(-1, 8)
+    assert re.sum() == 0 and passes >= 2
+    assert np.array_equal(out[:, 0], ref[:, 0])
+    assert (out[:, 0] != Z["in_particles"][:, 0]).sum() > 20   # hops happen
+    if int(Z["n"]) > 5:
+        assert np.array_equal(out, ref)
+    else:
+        assert np.abs(out[:, 1:] - ref[:, 1:]).max() <= 1e-13
+
+
+def test_point_interpolation_closed_form():
+    """full Lagrange (n <= 5) and the 5-point stencil (n > 5) reproduce
+    polynomials of degree <= min(n-1, 4) exactly"""
+    rng = np.random.default_rng(4)
+    for n in (3, 5, 6, 8, 12):
+        xi, _, _, _ = P.gll_basis(n)
+        deg = min(n - 1, 4)
+        c = rng.standard_normal((deg + 1, deg + 1, deg + 1))
+        Z, Y, X = np.meshgrid(xi, xi, xi, indexing="ij")
+        f = np.polynomial.polynomial.polyval3d(X, Y, Z, c)
+        for _ in range(20):
+            x, y, z = rng.uniform(-1, 1, 3)
+            ref = np.polynomial.polynomial.polyval3d(x, y, z, c)
+            got = P.interp_point(n, xi, f.ravel(), x, y, z)
+            assert abs(got - ref) <= 1e-12 * max(1, abs(ref)), (n, got, ref)
