@@ -171,6 +171,14 @@ int ssx_set_diag_choice(ssx_ctx *ctx, int choice);
  * its inner products, the residual update and the coarsest-level CG stay
  * FP64; inner products always accumulate in FP64. */
 #define SSX_OPT_SMOOTHER_PRECISION 2
+/* SSX_OPT_PERSISTENT_CG: 1 runs the fused CG of every multigrid
+ * level with n <= 4 (the coarse levels) as one cooperative launch with
+ * grid-wide barriers instead of six launches per iteration; 0 keeps the
+ * per-iteration launches (default); a value 2..1024 also fixes the number of
+ * workgroups of that launch (default: as many as the device keeps
+ * co-resident).  Same algorithm and stopping test; inner products are
+ * summed in a different fixed order. */
+#define SSX_OPT_PERSISTENT_CG 3
 #define SSX_PREC_FP64 0
 #define SSX_PREC_FP32 1
 #define SSX_PREC_BF16 2
@@ -219,7 +227,10 @@ int ssx_particles_set_velocity(ssx_ctx *ctx, const double *u,
 		const double *v, const double *w);
 /* integrate_particle_equations (particles.c:182-218): advance every
  * particle from its t to t_now (params->t) with history step dt
- * (params->dt); *max_passes receives the largest relocation pass count */
+ * (params->dt); *max_passes (may be NULL) receives the largest relocation
+ * pass count.  Synchronous: returns after the step has finished.  Particles
+ * whose el is not an element index are left untouched and the call returns
+ * SSX_EINVAL. */
 int ssx_particles_step(ssx_ctx *ctx, const ssx_particles *p, double t_now,
 		double dt, int *max_passes);
 

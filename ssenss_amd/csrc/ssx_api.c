@@ -369,6 +369,7 @@ int ssx_create(ssx_ctx **out, const ssx_mesh *m)
 	c->n = m->n;
 	c->diag_choice = -1;
 	c->fused = 1;
+	c->persist = 0;
 	c->lambda = (11.0 / 6.0) / (m->dt * m->visc);
 	int nl = SSX_LMAX;
 	for (int i = 0; i < SSX_LMAX; ++i) {     /* precon.c:62-70 */
@@ -450,6 +451,9 @@ int ssx_create(ssx_ctx **out, const ssx_mesh *m)
 		c->npart = np;
 		rc = dalloc(&c->scal, SSX_NSCAL);
 		if (!rc) rc = dalloc(&c->part, np);
+		if (!rc && hipMalloc((void **) &c->pbar, 4 * sizeof(unsigned))
+				!= hipSuccess)
+			rc = SSX_ENOMEM;
 		if (!rc && hipMalloc((void **) &c->cg_flags, sizeof(int) * 2 * SSX_LMAX)
 				!= hipSuccess) {
 			rc = SSX_ENOMEM;
@@ -502,6 +506,7 @@ void ssx_destroy(ssx_ctx *c)
 	}
 	hipFree(c->inv_mass);
 	hipFree(c->cg_flags);
+	hipFree(c->pbar);
 	if (c->h_flags)
 		hipHostFree(c->h_flags);
 	hipFree(c->scal);
@@ -569,6 +574,14 @@ int ssx_set_option(ssx_ctx *c, int option, int value)
 	switch (option) {
 	case SSX_OPT_FUSED_CG:
 		c->fused = value != 0;
+		return SSX_OK;
+	case SSX_OPT_PERSISTENT_CG:
+		if (value < 0 || value > 1024) {
+			ssx_set_error("persistent CG option %d out of range", value);
+			return SSX_EINVAL;
+		}
+		c->persist = value != 0;
+		c->persist_grid = value > 1 ? value : 0;
 		return SSX_OK;
 	case SSX_OPT_SMOOTHER_PRECISION:
 		if (value != SSX_PREC_FP64 && value != SSX_PREC_FP32
@@ -823,8 +836,8 @@ static int cg_level_plain(ssx_ctx *c, int l, int op, int var, double *f,
  * exact, so skipping the product changes nothing).
  */
 static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
-		double *u, double eps, int max_iter, int x_zero, int *iters,
-		double *err_out)
+		double *u, double eps, double rtol, int max_iter, int x_zero,
+		int *iters, double *err_out)
 {
 	ssx_level *L = &c->lev[l];
 	const size_t nt = L->ntot;
@@ -848,6 +861,40 @@ static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
 	TRY(ssxk_copy(st, p, z, nt));
 	TRY(dot_to(c, l, r, z, o + CG_RDZ));
 	TRY(ssxk_set_ints(st, flags, 0, 2));
+	double eps_dev = eps;
+	if (rtol > 0.0) {       /* stop at max(eps, rtol * |r0|_w) */
+		TRY(dot_to(c, l, r, r, o + CG_RR));
+		TRY(ssxk_cg_tol(st, c->scal, o + CG_TOL, o + CG_RR, eps, rtol));
+		eps_dev = -1.0;
+	}
+	if (c->persist && L->n <= 4) {
+		int grid = c->persist_grid;
+		if (grid <= 0) {
+			TRY(ssxk_cg_persist_max_grid(L->n, &grid));
+			/* partials: 6 doubles per block in part */
+			if ((size_t) grid * 6 > c->npart)
+				grid = (int) (c->npart / 6);
+			if (grid > 1024)
+				grid = 1024;
+		}
+		TRY(ssxk_set_ints(st, (int *) c->pbar, 0, 4));
+		TRY(ssxk_cg_persist(st, L->n, grid, c->nel, op, !pres, pres,
+				max_iter, eps_dev, c->lambda, (double) L->nunique, L->gf,
+				L->mass, L->dmat, dinv, L->scale, mask, L->bc, L->grp,
+				L->ngrp, u, r, z, p, ap, c->part, c->scal, o, flags,
+				c->pbar));
+		hipStream_t hs = (hipStream_t) st;
+		HIPC(hipMemcpyAsync(c->h_flags, flags, 2 * sizeof(int),
+				hipMemcpyDeviceToHost, hs));
+		HIPC(hipMemcpyAsync(c->h_scal + o + CG_RR, c->scal + o + CG_RR,
+				sizeof(double), hipMemcpyDeviceToHost, hs));
+		HIPC(hipStreamSynchronize(hs));
+		if (c->h_flags[0] < 0) {
+			ssx_set_error("CG level %d: a grid barrier of the persistent "
+					"solve timed out", l);
+			return SSX_EHIP;
+		}
+	} else {
 	const long long nba = ssxk_ax_blocks(L->n, c->nel);
 	const long long nbd = ssxk_dss_blocks(L->ngrp);
 	int launched = 0;
@@ -861,11 +908,11 @@ static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
 			TRY(ssxk_dss_all(st, L->grp, L->ngrp, ap, p, c->part + nba, 0,
 					flags));
 			TRY(ssxk_cg_reduce(st, 0, c->part, nba + nbd, c->scal, o, flags,
-					eps, max_iter, L->nunique, pres));
+					eps_dev, max_iter, L->nunique, pres));
 			TRY(ssxk_cg_update(st, nt, c->scal, o + CG_ALPHA, p, ap, u, r, z,
 					dinv, L->scale, pres, c->part, flags));
-			TRY(ssxk_cg_reduce(st, 1, c->part, 0, c->scal, o, flags, eps,
-					max_iter, L->nunique, pres));
+			TRY(ssxk_cg_reduce(st, 1, c->part, 0, c->scal, o, flags,
+					eps_dev, max_iter, L->nunique, pres));
 			TRY(ssxk_cg_pupdate(st, nt, z, p, c->scal, o + CG_BETA,
 					pres ? o + CG_NULLC : -1, mask, flags));
 		}
@@ -879,6 +926,7 @@ static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
 		if (c->h_flags[0] || launched >= max_iter)
 			break;
 		chunk = 4;
+	}
 	}
 	const int it = c->h_flags[1];
 	const double err = sqrt(c->h_scal[o + CG_RR]);
@@ -897,8 +945,8 @@ static int cg_level(ssx_ctx *c, int l, int op, int var, double *f, double *u,
 		double eps, int max_iter, int x_zero, int *iters, double *err_out)
 {
 	if (c->fused && c->lev[l].fused_ok)
-		return cg_level_fused(c, l, op, var, f, u, eps, max_iter, x_zero,
-				iters, err_out);
+		return cg_level_fused(c, l, op, var, f, u, eps, 0.0, max_iter,
+				x_zero, iters, err_out);
 	return cg_level_plain(c, l, op, var, f, u, eps, max_iter, iters,
 			err_out);
 }
@@ -971,17 +1019,16 @@ static int build_lp(ssx_ctx *c, int prec)
 			TRY(falloc(&L->e_lp, nt));
 		if (l == c->nlev - 1)
 			continue;                       /* coarsest: FP64 */
-		if (L->lp_prec != prec) {
-			hipFree(L->gf_lp);
-			L->gf_lp = NULL;
-			const size_t bytes = 6 * nt * (prec == SSX_PREC_BF16 ? 2 : 4);
-			if (hipMalloc(&L->gf_lp, bytes) != hipSuccess) {
-				ssx_set_error("hipMalloc low-precision factors");
-				return SSX_ENOMEM;
-			}
-			TRY(ssxk_cvt(c->stream, SSX_PREC_FP64, prec, L->gf, L->gf_lp,
-					6 * nt));
+		if (L->lp_prec == prec)
+			continue;                       /* already converted */
+		hipFree(L->gf_lp);
+		L->gf_lp = NULL;
+		const size_t bytes = 6 * nt * (prec == SSX_PREC_BF16 ? 2 : 4);
+		if (hipMalloc(&L->gf_lp, bytes) != hipSuccess) {
+			ssx_set_error("hipMalloc low-precision factors");
+			return SSX_ENOMEM;
 		}
+		TRY(ssxk_cvt(c->stream, SSX_PREC_FP64, prec, L->gf, L->gf_lp, 6 * nt));
 		TRY(cvt_new(c, &L->mass_lp, L->mass, nt));
 		TRY(cvt_new(c, &L->diagp_lp, L->diag_p, nt));
 		TRY(cvt_new(c, &L->diagh_lp, L->diag_h, nt));
@@ -1094,11 +1141,15 @@ static int mg_recurse_lp(ssx_ctx *c, int l, int op, int var, double eps)
 	if (l == c->nlev - 1) {          /* coarsest level in FP64 */
 		double err;
 		int it;
-		TRY(ssxk_cvt(st, SSX_PREC_FP32, SSX_PREC_FP64, L->fn_lp, L->mg_fn,
-				L->ntot));
+		/* Right-hand side: FP64 and DSS-averaged by the caller.  The
+		 * correction is handed to FP32 levels, so the solve stops at
+		 * max(eps, SSX_LP_COARSE_RTOL * |f|_w): below that its digits are
+		 * lost in the conversion, and FP32 rounding of the fine residual
+		 * leaves a component in the null space of singular problems
+		 * (periodic velocity) that CG cannot remove. */
 		TRY(ssxk_fill(st, L->mg_e, 0.0, L->ntot));
-		TRY(cg_level(c, l, op, var, L->mg_fn, L->mg_e, eps, 500, 1, &it,
-				&err));
+		TRY(cg_level_fused(c, l, op, var, L->mg_fn, L->mg_e, eps,
+				SSX_LP_COARSE_RTOL, 500, 1, &it, &err));
 		c->cnt[1] += it;
 		return ssxk_cvt(st, SSX_PREC_FP64, SSX_PREC_FP32, L->mg_e, L->e_lp,
 				L->ntot);
@@ -1112,7 +1163,21 @@ static int mg_recurse_lp(ssx_ctx *c, int l, int op, int var, double eps)
 		TRY(ssxk_copy_lp(st, r, L->fn_lp, L->ntot));
 	}
 	TRY(ssxk_restrict_lp(st, L->n, C->n, c->nel, L->prol_lp, r, C->fn_lp));
-	TRY(ssxk_dss_all_lp(st, C->grp, C->ngrp, C->fn_lp, NULL, NULL, 1, NULL));
+	if (l + 1 == c->nlev - 1) {
+		/* The coarsest solve runs in FP64 to eps.  Its right-hand side is
+		 * averaged over the copies of each shared node in FP64: an FP32
+		 * average leaves the copies unequal at FP32 rounding, a part of
+		 * the residual that no assembled correction removes, and on the
+		 * singular pressure problem CG then stalls above eps and breaks
+		 * down. */
+		TRY(ssxk_cvt(st, SSX_PREC_FP32, SSX_PREC_FP64, C->fn_lp, C->mg_fn,
+				C->ntot));
+		TRY(ssxk_dss_all(st, C->grp, C->ngrp, C->mg_fn, NULL, NULL, 1,
+				NULL));
+	} else {
+		TRY(ssxk_dss_all_lp(st, C->grp, C->ngrp, C->fn_lp, NULL, NULL, 1,
+				NULL));
+	}
 	TRY(ssxk_fill_lp(st, C->e_lp, 0.0, C->ntot));
 	TRY(mg_recurse_lp(c, l + 1, op, var, eps));
 	TRY(ssxk_prolong_lp(st, L->n, C->n, c->nel, L->prol_lp, C->e_lp,
@@ -1133,7 +1198,11 @@ static int mg_precon_lp(ssx_ctx *c, int op, int var, double eps,
 	TRY(ssxk_cvt(st, SSX_PREC_FP64, SSX_PREC_FP32, x, L->fn_lp, L->ntot));
 	TRY(ssxk_mul_lp(st, L->ntot, L->fn_lp, jdiag_lp(c, 0, var), L->e_lp));
 	TRY(mg_recurse_lp(c, 0, op, var, eps));
-	return ssxk_cvt(st, SSX_PREC_FP32, SSX_PREC_FP64, L->e_lp, px, L->ntot);
+	TRY(ssxk_cvt(st, SSX_PREC_FP32, SSX_PREC_FP64, L->e_lp, px, L->ntot));
+	/* the FP32 smoother leaves the copies of a shared node unequal at FP32
+	 * rounding; averaging them in FP64 returns an assembled vector, as the
+	 * FP64 cycle does, so the FCG iterate stays continuous */
+	return ssxk_dss_all(st, L->grp, L->ngrp, px, NULL, NULL, 1, NULL);
 }
 
 int ssx_mg_precon(ssx_ctx *c, int op, int var, double eps, const double *x,
@@ -1451,7 +1520,7 @@ static int particles_init(ssx_ctx *c)
 	if (rc == SSX_OK) {
 		if (hipMalloc((void **) &c->d_neigh, sizeof(int) * 6 * c->nel)
 				!= hipSuccess || hipMalloc((void **) &c->p_maxpass,
-				sizeof(int)) != hipSuccess) {
+				2 * sizeof(int)) != hipSuccess) {
 			ssx_set_error("hipMalloc particle tables");
 			return SSX_ENOMEM;
 		}
@@ -1484,26 +1553,36 @@ int ssx_particles_step(ssx_ctx *c, const ssx_particles *p, double t_now,
 		double dt, int *max_passes)
 {
 	TRY(chk(c, 0));
-	if (!p || p->n < 0) {
+	if (!p || p->n < 0 || (p->n > 0 && (!p->el || !p->old_el || !p->x
+			|| !p->y || !p->z || !p->u || !p->v || !p->w || !p->t
+			|| !p->tau))) {
 		ssx_set_error("bad particle list");
+		return SSX_EINVAL;
+	}
+	if (!(dt > 0.0)) {
+		ssx_set_error("particle step: dt must be positive");
 		return SSX_EINVAL;
 	}
 	TRY(particles_init(c));
 	const double *ur[9];
 	for (int a = 0; a < 9; ++a)
 		ur[a] = c->p_ur[a / 3][a % 3];
-	TRY(ssxk_set_ints(c->stream, c->p_maxpass, 0, 1));
-	TRY(ssxk_particles_step(c->stream, c->lev[0].n, c->p_xi, ur,
+	TRY(ssxk_set_ints(c->stream, c->p_maxpass, 0, 2));
+	TRY(ssxk_particles_step(c->stream, c->lev[0].n, c->nel, c->p_xi, ur,
 			(const double *const *) c->p_jac,
 			(const double *const *) c->p_jinv, c->d_neigh, t_now, dt, p->n,
 			p->el, p->old_el, p->x, p->y, p->z, p->u, p->v, p->w, p->t,
 			p->tau, p->reinit, c->p_maxpass));
-	if (max_passes) {
-		int mp = 0;
-		HIPC(hipMemcpyAsync(&mp, c->p_maxpass, sizeof(int),
-				hipMemcpyDeviceToHost, (hipStream_t) c->stream));
-		HIPC(hipStreamSynchronize((hipStream_t) c->stream));
-		*max_passes = mp;
+	int fl[2] = { 0, 0 };
+	HIPC(hipMemcpyAsync(fl, c->p_maxpass, sizeof(fl), hipMemcpyDeviceToHost,
+			(hipStream_t) c->stream));
+	HIPC(hipStreamSynchronize((hipStream_t) c->stream));
+	if (max_passes)
+		*max_passes = fl[0];
+	if (fl[1]) {
+		ssx_set_error("%d particles name no element of the mesh (skipped)",
+				fl[1]);
+		return SSX_EINVAL;
 	}
 	return SSX_OK;
 }

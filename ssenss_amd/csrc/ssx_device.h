@@ -1,0 +1,57 @@
+// ssx_device.h -- device helpers shared by the gfx950 kernel files.
+#ifndef SSX_DEVICE_H
+#define SSX_DEVICE_H
+#include <hip/hip_runtime.h>
+
+typedef unsigned short bf16_t;
+
+// factor loads: bf16 is the upper half of an IEEE float
+__device__ __forceinline__ double ldg(const double *p, size_t i) { return p[i]; }
+__device__ __forceinline__ float ldg(const float *p, size_t i) { return p[i]; }
+__device__ __forceinline__ float ldg(const bf16_t *p, size_t i)
+{
+	return __uint_as_float(((unsigned) p[i]) << 16);
+}
+
+// partner order of copy t: t ^ ord(k), k = 1..M-1 (faces x,y,z, then edges
+// along z, y, x, then the corner; global.c:252-256)
+__device__ __forceinline__ int dss_ord(int M, int k)
+{
+	return M == 8 ? ((k == 3) ? 4 : (k == 4) ? 3 : k) : k;
+}
+
+// One group of M copies.  Optionally each sum is scaled by 1/M
+// (direct_stiffness_averaging, global.c:259-268: every copy in a group of M
+// has multiplicity exactly M), and the shared-node part of <p, DSS(u)>_w is
+// returned.
+template <int M, typename T>
+__device__ __forceinline__ double dss_group(long long G, long long g,
+		const int *__restrict__ idx, T *__restrict__ u,
+		const T *__restrict__ p, int average)
+{
+	int ix[M];
+	T v[M];
+#pragma unroll
+	for (int t = 0; t < M; ++t)
+		ix[t] = idx[(long long) t * G + g];
+#pragma unroll
+	for (int t = 0; t < M; ++t)
+		v[t] = u[ix[t]];
+	const T w = T(1) / T(M);
+	double pd = 0.0;
+#pragma unroll
+	for (int t = 0; t < M; ++t) {
+		T s = v[t];
+#pragma unroll
+		for (int k = 1; k < M; ++k)
+			s = s + v[t ^ dss_ord(M, k)];
+		if (average)
+			s = s * w;
+		u[ix[t]] = s;
+		if (p)
+			pd += (double) (p[ix[t]] * w) * (double) s;
+	}
+	return pd;
+}
+
+#endif

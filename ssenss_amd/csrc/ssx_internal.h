@@ -18,6 +18,9 @@ extern "C" {
 #define SSX_LMAX 10          /* reference structs.c:58 new_preconditioner(n,10) */
 #define SSX_NPART 1024       /* blocks of the reduction kernels            */
 #define SSX_NSCAL 256        /* device scalar slots                        */
+/* mixed-precision V-cycle: the FP64 coarsest solve stops at
+ * max(eps, SSX_LP_COARSE_RTOL * |f|_w), 16 FP32 rounding units */
+#define SSX_LP_COARSE_RTOL 1e-6
 
 /* device scalar slots */
 enum {
@@ -29,7 +32,8 @@ enum {
 };
 
 /* per-level scalar block of the fused CG (solver.c:33-81) */
-enum { CG_RDZ = 0, CG_PAP, CG_ALPHA, CG_BETA, CG_RR, CG_NULLC, CG_NSLOT = 8 };
+enum { CG_RDZ = 0, CG_PAP, CG_ALPHA, CG_BETA, CG_RR, CG_NULLC, CG_TOL,
+	CG_NSLOT = 8 };
 #define SC_CG(l) (64 + CG_NSLOT * (l))
 
 /* host-side basis of one level */
@@ -89,6 +93,9 @@ struct ssx_ctx {
 	int *cg_flags;               /* device: [2*l] done, [2*l+1] count    */
 	int *h_flags;                /* pinned host mirror                   */
 	int cg_hint[SSX_LMAX];       /* last iteration count per level       */
+	int persist;                 /* SSX_OPT_PERSISTENT_CG                */
+	int persist_grid;            /* blocks of the cooperative CG launch  */
+	unsigned *pbar;              /* device: grid-barrier words [4]       */
 	size_t npart;                /* doubles in part                      */
 	/* FCG history (level 0) */
 	double *fcg_r, *fcg_z, *fcg_p[11], *fcg_ap[10];
@@ -135,9 +142,13 @@ int ssxk_cg_update(void *stream, size_t n, const double *scal, int ia,
 int ssxk_cg_pupdate(void *stream, size_t n, const double *z, double *p,
 		const double *scal, int ib, int ic, const double *mask,
 		const int *done);
+/* eps < 0: the stopping tolerance is read from scal[o + CG_TOL] */
 int ssxk_cg_reduce(void *stream, int mode, const double *part, long long cnt,
 		double *scal, int o, int *flags, double eps, int maxit,
 		double nunique, int pressure);
+/* scal[dst] = max(eps, rtol * sqrt(scal[src])) */
+int ssxk_cg_tol(void *stream, double *scal, int dst, int src, double eps,
+		double rtol);
 int ssxk_set_ints(void *stream, int *dst, int v, int n);
 /* low-precision (FP32 fields) variants; prec selects the factor type of
  * ssxk_ax_lp (SSX_PREC_FP32 or SSX_PREC_BF16) */
@@ -171,12 +182,24 @@ int ssxk_copy_lp(void *stream, void *dst, const void *src, size_t n);
 int ssxk_particles_ur(void *stream, size_t nt, const double *const jinv[9],
 		const double *u, const double *v, const double *w, double *u0,
 		double *u1, double *u2);
-int ssxk_particles_step(void *stream, int n_pts, const double *xi,
+int ssxk_particles_step(void *stream, int n_pts, int nel, const double *xi,
 		const double *const ur[9], const double *const jac[9],
 		const double *const jinv[9], const int *neigh, double T, double dt,
 		long np, int *el, int *old_el, double *x, double *y, double *z,
 		double *u, double *v, double *w, double *t, const double *tau,
-		int *reinit, int *max_pass);
+		int *reinit, int *flags);
+/* whole Jacobi-PCG solve of one level with n <= 4 in one cooperative
+ * launch (ssx_persist.hip); setup (r, z, p, <r,z>_w, flags) done by the
+ * caller, bar[0..2] zeroed */
+int ssxk_cg_persist(void *stream, int n, int grid, int nel, int op,
+		int apply_mask, int pressure, int maxit, double eps, double lambda,
+		double nunique, const double *gf, const double *mass,
+		const double *dmat, const double *dinv, const double *scale,
+		const double *mask, const int *bc, int *const grp[3],
+		const long long ng[3], double *u, double *r, double *z, double *p,
+		double *ap, double *part, double *scal, int o, int *flags,
+		unsigned *bar);
+int ssxk_cg_persist_max_grid(int n, int *grid);
 /* precision conversion (SSX_PREC_FP64 -> FP32 / BF16, FP32 -> FP64) */
 int ssxk_cvt(void *stream, int from, int to, const void *x, void *y,
 		size_t n);

@@ -26,6 +26,7 @@
 //   k_dot / k_reduce         deterministic two-pass weighted inner product.
 #include <hip/hip_runtime.h>
 #include "ssx_internal.h"
+#include "ssx_device.h"
 
 #define LAUNCH_CHECK(name)                                                   \
 	do {                                                                 \
@@ -35,8 +36,6 @@
 			return SSX_EHIP;                                     \
 		}                                                            \
 	} while (0)
-
-typedef unsigned short bf16_t;
 
 static inline dim3 vgrid(size_t n)
 {
@@ -51,14 +50,6 @@ static inline dim3 vgrid(size_t n)
 #define GRID_STRIDE(q, n) \
 	for (size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x; q < (n); \
 			q += (size_t) gridDim.x * blockDim.x)
-
-// factor loads: bf16 is the upper half of an IEEE float
-__device__ __forceinline__ double ldg(const double *p, size_t i) { return p[i]; }
-__device__ __forceinline__ float ldg(const float *p, size_t i) { return p[i]; }
-__device__ __forceinline__ float ldg(const bf16_t *p, size_t i)
-{
-	return __uint_as_float(((unsigned) p[i]) << 16);
-}
 
 // ---------------------------------------------------------------------
 // Operator
@@ -277,47 +268,6 @@ extern "C" int ssxk_ax_lp(void *stream, int prec, int n, int nel, int op,
 // Direct stiffness summation over shared-node groups
 // ---------------------------------------------------------------------
 
-// partner order of copy t: t ^ ord(k), k = 1..M-1 (faces x,y,z, then edges
-// along z, y, x, then the corner; global.c:252-256)
-__device__ __forceinline__ int dss_ord(int M, int k)
-{
-	return M == 8 ? ((k == 3) ? 4 : (k == 4) ? 3 : k) : k;
-}
-
-// One group of M copies.  Optionally each sum is scaled by 1/M
-// (direct_stiffness_averaging, global.c:259-268: every copy in a group of M
-// has multiplicity exactly M), and the shared-node part of <p, DSS(u)>_w is
-// returned.
-template <int M, typename T>
-__device__ __forceinline__ double dss_group(long long G, long long g,
-		const int *__restrict__ idx, T *__restrict__ u,
-		const T *__restrict__ p, int average)
-{
-	int ix[M];
-	T v[M];
-#pragma unroll
-	for (int t = 0; t < M; ++t)
-		ix[t] = idx[(long long) t * G + g];
-#pragma unroll
-	for (int t = 0; t < M; ++t)
-		v[t] = u[ix[t]];
-	const T w = T(1) / T(M);
-	double pd = 0.0;
-#pragma unroll
-	for (int t = 0; t < M; ++t) {
-		T s = v[t];
-#pragma unroll
-		for (int k = 1; k < M; ++k)
-			s = s + v[t ^ dss_ord(M, k)];
-		if (average)
-			s = s * w;
-		u[ix[t]] = s;
-		if (p)
-			pd += (double) (p[ix[t]] * w) * (double) s;
-	}
-	return pd;
-}
-
 // One launch for all three multiplicities: blocks [0,b2) take the pairs,
 // [b2,b2+b4) the quadruples, the rest the octuples.
 template <typename T>
@@ -473,6 +423,7 @@ k_cg_pupdate(size_t n, const T *__restrict__ z, T *__restrict__ p,
 //   mode 1: rz, rr (, zs, rs) from the 4 update partials; null-space
 //           constant c = zs/nunique; rdz = rz - c rs; beta = rdz / rdz_old;
 //           iteration count ++, done when sqrt(rr) <= eps or count >= maxit
+//           (eps < 0: the tolerance is s[o+TOL], formed on the device)
 __global__ void __launch_bounds__(1024)
 k_cg_reduce(int mode, const double *__restrict__ part, long long cnt,
 		double *__restrict__ scal, int o, int *__restrict__ flags,
@@ -518,8 +469,25 @@ k_cg_reduce(int mode, const double *__restrict__ part, long long cnt,
 	scal[o + CG_RR] = rr;
 	const int it = flags[1] + 1;
 	flags[1] = it;
-	if (sqrt(rr) <= eps || it >= maxit || !(rr == rr))
+	const double tol = eps >= 0.0 ? eps : scal[o + CG_TOL];
+	if (sqrt(rr) <= tol || it >= maxit || !(rr == rr))
 		flags[0] = 1;
+}
+
+__global__ void k_cg_tol(double *scal, int dst, int src, double eps,
+		double rtol)
+{
+	if (threadIdx.x == 0)
+		scal[dst] = fmax(eps, rtol * sqrt(scal[src]));
+}
+
+extern "C" int ssxk_cg_tol(void *stream, double *scal, int dst, int src,
+		double eps, double rtol)
+{
+	hipLaunchKernelGGL(k_cg_tol, dim3(1), dim3(64), 0, (hipStream_t) stream,
+			scal, dst, src, eps, rtol);
+	LAUNCH_CHECK("k_cg_tol");
+	return SSX_OK;
 }
 
 template <typename T>

@@ -23,6 +23,7 @@
 
 struct PField {
 	int n;
+	int nel;
 	const double *xi;       // n GLL points
 	const double *ur[3][3]; // [history level: 0 now, 1 old, 2 older][comp]
 	const double *jac[9];   // J_ab
@@ -327,7 +328,7 @@ k_particles_step(PField F, long n, int *__restrict__ el,
 		double *__restrict__ u, double *__restrict__ v,
 		double *__restrict__ w, double *__restrict__ t,
 		const double *__restrict__ tau, int *__restrict__ reinit_out,
-		int *__restrict__ max_pass)
+		int *__restrict__ flags)
 {
 	__shared__ double sxi[PMAXN];
 	if (threadIdx.x < F.n)
@@ -339,6 +340,10 @@ k_particles_step(PField F, long n, int *__restrict__ el,
 		return;
 	PState p;
 	p.el = el[q];
+	if (p.el < 0 || p.el >= F.nel) {        // not an element: report, skip
+		atomicAdd(flags + 1, 1);
+		return;
+	}
 	p.old_el = old_el[q];
 	p.s[0] = x[q]; p.s[1] = y[q]; p.s[2] = z[q];
 	p.s[3] = u[q]; p.s[4] = v[q]; p.s[5] = w[q];
@@ -372,7 +377,7 @@ k_particles_step(PField F, long n, int *__restrict__ el,
 	t[q] = p.t;
 	if (reinit_out)
 		reinit_out[q] = p.reinit;
-	atomicMax(max_pass, j);                 // a maximum: order-independent
+	atomicMax(flags, j);                    // a maximum: order-independent
 }
 
 struct P9 {
@@ -419,15 +424,17 @@ extern "C" int ssxk_particles_ur(void *stream, size_t nt,
 	return SSX_OK;
 }
 
-extern "C" int ssxk_particles_step(void *stream, int n_pts, const double *xi,
+extern "C" int ssxk_particles_step(void *stream, int n_pts, int nel,
+		const double *xi,
 		const double *const ur[9], const double *const jac[9],
 		const double *const jinv[9], const int *neigh, double T, double dt,
 		long np, int *el, int *old_el, double *x, double *y, double *z,
 		double *u, double *v, double *w, double *t, const double *tau,
-		int *reinit, int *max_pass)
+		int *reinit, int *flags)
 {
 	PField F;
 	F.n = n_pts;
+	F.nel = nel;
 	F.xi = xi;
 	for (int l = 0; l < 3; ++l)
 		for (int c = 0; c < 3; ++c)
@@ -443,7 +450,7 @@ extern "C" int ssxk_particles_step(void *stream, int n_pts, const double *xi,
 		return SSX_OK;
 	hipLaunchKernelGGL(k_particles_step, dim3((unsigned) ((np + 63) / 64)),
 			dim3(64), 0, (hipStream_t) stream, F, np, el, old_el, x, y, z, u,
-			v, w, t, tau, reinit, max_pass);
+			v, w, t, tau, reinit, flags);
 	hipError_t e = hipGetLastError();
 	if (e != hipSuccess) {
 		ssx_set_error("k_particles_step: %s", hipGetErrorString(e));

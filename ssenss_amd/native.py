@@ -9,6 +9,7 @@ OP_POISSON, OP_HELMHOLTZ = 0, 1
 PC_JACOBI, PC_MG = 0, 1
 OPT_FUSED_CG = 1
 OPT_SMOOTHER_PRECISION = 2
+OPT_PERSISTENT_CG = 3
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 (DATA_GFAC, DATA_MASS, DATA_DIAG_P, DATA_DIAG_H, DATA_SCALE, DATA_MASK,
  DATA_INVMASS, DATA_X, DATA_Y, DATA_Z) = range(10)
