@@ -441,7 +441,7 @@ int ssx_create(ssx_ctx **out, const ssx_mesh *m)
 	}
 	if (rc == SSX_OK) {
 		const size_t nt = c->lev[0].ntot;
-		size_t np = (size_t) 8 * SSX_NPART;
+		size_t np = (size_t) 12 * SSX_NPART;   /* FCG: 10 dots at once */
 		for (int l = 0; l < nl; ++l) {
 			size_t q = (size_t) ssxk_ax_blocks(c->lev[l].n, c->nel)
 				+ (size_t) ssxk_dss_blocks(c->lev[l].ngrp);
@@ -903,8 +903,16 @@ static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
 		if (chunk > max_iter - launched)
 			chunk = max_iter - launched;
 		for (int k = 0; k < chunk; ++k) {
-			TRY(ssxk_ax(st, L->n, c->nel, op, p, ap, L->gf, L->mass,
-					L->dmat, c->lambda, L->bc, !pres, flags, c->part));
+			/* from the second iteration on, the operator kernel first forms
+			 * p = (z - c) + beta p (solver.c:70-72) */
+			if (launched + k == 0)
+				TRY(ssxk_ax(st, L->n, c->nel, op, p, ap, L->gf, L->mass,
+						L->dmat, c->lambda, L->bc, !pres, flags, c->part));
+			else
+				TRY(ssxk_ax_pupd(st, L->n, c->nel, op, z, p, c->scal,
+						o + CG_BETA, pres ? o + CG_NULLC : -1, ap, L->gf,
+						L->mass, L->dmat, c->lambda, L->bc, !pres, flags,
+						c->part));
 			TRY(ssxk_dss_all(st, L->grp, L->ngrp, ap, p, c->part + nba, 0,
 					flags));
 			TRY(ssxk_cg_reduce(st, 0, c->part, nba + nbd, c->scal, o, flags,
@@ -913,8 +921,6 @@ static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
 					dinv, L->scale, pres, c->part, flags));
 			TRY(ssxk_cg_reduce(st, 1, c->part, 0, c->scal, o, flags,
 					eps_dev, max_iter, L->nunique, pres));
-			TRY(ssxk_cg_pupdate(st, nt, z, p, c->scal, o + CG_BETA,
-					pres ? o + CG_NULLC : -1, mask, flags));
 		}
 		launched += chunk;
 		hipStream_t hs = (hipStream_t) st;
@@ -1108,9 +1114,15 @@ static int cg_level_lp(ssx_ctx *c, int l, int op, int var, float *f,
 	TRY(ssxk_set_ints(st, flags, 0, 2));
 	const long long nba = ssxk_ax_blocks(L->n, c->nel);
 	for (int k = 0; k < max_iter; ++k) {
-		TRY(ssxk_ax_lp(st, L->lp_prec, L->n, c->nel, op, p, ap, L->gf_lp,
-				L->mass_lp, L->dmat_lp, c->lambda, L->bc, !pres, flags,
-				c->part));
+		if (k == 0)
+			TRY(ssxk_ax_lp(st, L->lp_prec, L->n, c->nel, op, p, ap, L->gf_lp,
+					L->mass_lp, L->dmat_lp, c->lambda, L->bc, !pres, flags,
+					c->part));
+		else
+			TRY(ssxk_ax_pupd_lp(st, L->lp_prec, L->n, c->nel, op, z, p,
+					c->scal, o + CG_BETA, pres ? o + CG_NULLC : -1, ap,
+					L->gf_lp, L->mass_lp, L->dmat_lp, c->lambda, L->bc, !pres,
+					flags, c->part));
 		TRY(ssxk_dss_all_lp(st, L->grp, L->ngrp, ap, p, c->part + nba, 0,
 				flags));
 		TRY(ssxk_cg_reduce(st, 0, c->part, nba + ssxk_dss_blocks(L->ngrp),
@@ -1119,8 +1131,6 @@ static int cg_level_lp(ssx_ctx *c, int l, int op, int var, float *f,
 				dinv, L->scale_lp, pres, c->part, flags));
 		TRY(ssxk_cg_reduce(st, 1, c->part, 0, c->scal, o, flags, eps,
 				max_iter, L->nunique, pres));
-		TRY(ssxk_cg_pupdate_lp(st, nt, z, p, c->scal, o + CG_BETA,
-				pres ? o + CG_NULLC : -1, mask, flags));
 	}
 	if (iters) {
 		hipStream_t hs = (hipStream_t) st;
@@ -1283,21 +1293,16 @@ int ssx_fcg(ssx_ctx *c, int op, int var, int precon, double *f, double *u,
 		else
 			TRY(ssx_jacobi(c, 0, var, r, z));
 		TRY(ssx_nullspace(c, 0, var, z));
-		for (int k = 0; k < m; ++k) {
-			TRY(dot_to(c, 0, z, ap[k], SC_ZAP0 + k));
-			TRY(ssxk_scalar(st, c->scal, 0, SC_BETA0 + k, SC_ZAP0 + k,
-					SC_PAP0 + k, 0.0));
-		}
+		/* beta_k = <z, Ap_k>_w / <p_k, Ap_k>_w, k < m, in one pass */
+		TRY(ssxk_multidot(st, nt, z, L->scale, (const double *const *) ap,
+				m, c->part, c->scal, SC_ZAP0, SC_PAP0, SC_BETA0));
 		double *tp = p[m];                 /* p[k] -> p[k+1], k < m */
 		for (int k = m; k > 0; --k)
 			p[k] = p[k - 1];
 		p[0] = tp;
-		TRY(ssxk_copy(st, p[0], z, nt));
-		for (int k = 1; k < m + 1; ++k)
-			TRY(ssxk_axpy_s(st, nt, c->scal, SC_BETA0 + k - 1, -1.0, p[k],
-					p[0]));
-		if (mask)
-			TRY(ssxk_mul(st, nt, p[0], mask, p[0]));
+		/* p_0 = mask (z - sum_k beta_k p_{k+1}), one pass */
+		TRY(ssxk_multi_axpy(st, nt, z, (const double *const *) (p + 1), m,
+				c->scal, SC_BETA0, mask, p[0]));
 		int m2 = (int) (j + 1), mm2 = (int) ((j + 1) % (mmax + 1));
 		if (mm2 < 1)
 			mm2 = 1;
@@ -1311,12 +1316,16 @@ int ssx_fcg(ssx_ctx *c, int op, int var, int precon, double *f, double *u,
 		for (int k = m2 - 1; k > 0; --k)
 			TRY(ssxk_scalar(st, c->scal, 2, SC_PAP0 + k, SC_PAP0 + k - 1, 0,
 					0.0));
-		TRY(dot_to(c, 0, p[0], ap[0], SC_PAP0));
-		TRY(dot_to(c, 0, p[0], r, SC_PR));
-		TRY(ssxk_scalar(st, c->scal, 0, SC_ALPHA, SC_PR, SC_PAP0, 0.0));
-		TRY(ssxk_axpy_s(st, nt, c->scal, SC_ALPHA, 1.0, p[0], u));
-		TRY(ssxk_axpy_s(st, nt, c->scal, SC_ALPHA, -1.0, ap[0], r));
-		TRY(dot_to(c, 0, r, r, SC_RR));
+		/* <p_0, Ap_0>_w and <p_0, r>_w in one pass */
+		const double *pr2[2] = { ap[0], r };
+		TRY(ssxk_multidot(st, nt, p[0], L->scale, pr2, 2, c->part, c->scal,
+				SC_PAPN, -1, 0));
+		TRY(ssxk_scalar(st, c->scal, 2, SC_PAP0, SC_PAPN, 0, 0.0));
+		TRY(ssxk_scalar(st, c->scal, 0, SC_ALPHA, SC_PAPN + 1, SC_PAPN, 0.0));
+		/* u += alpha p_0; r -= alpha Ap_0; <r,r>_w */
+		TRY(ssxk_fcg_step(st, nt, c->scal, SC_ALPHA, p[0], ap[0], u, r,
+				L->scale, c->part));
+		TRY(ssxk_reduce(st, c->part, 0, c->scal, SC_RR));
 		double rr;
 		TRY(read_scal(c, SC_RR, &rr));
 		err = sqrt(rr);

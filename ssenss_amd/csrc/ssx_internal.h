@@ -18,6 +18,7 @@ extern "C" {
 #define SSX_LMAX 10          /* reference structs.c:58 new_preconditioner(n,10) */
 #define SSX_NPART 1024       /* blocks of the reduction kernels            */
 #define SSX_NSCAL 256        /* device scalar slots                        */
+#define SSX_FCG_M 10         /* FCG history length (solver.c:90-92)        */
 /* mixed-precision V-cycle: the FP64 coarsest solve stops at
  * max(eps, SSX_LP_COARSE_RTOL * |f|_w), 16 FP32 rounding units */
 #define SSX_LP_COARSE_RTOL 1e-6
@@ -28,7 +29,8 @@ enum {
 	SC_NULLC, SC_DONE, SC_PR, SC_ZAP0 /* .. SC_ZAP0+9 */ ,
 	SC_PAP0 = SC_ZAP0 + 10 /* .. SC_PAP0+9 */,
 	SC_BETA0 = SC_PAP0 + 10 /* .. SC_BETA0+9 */,
-	SC_END = SC_BETA0 + 10
+	SC_PAPN = SC_BETA0 + 10 /* <p0,Ap0>, <p0,r> */,
+	SC_END = SC_PAPN + 2
 };
 
 /* per-level scalar block of the fused CG (solver.c:33-81) */
@@ -129,6 +131,19 @@ int ssxk_ax(void *stream, int n, int nel, int op, const double *u,
 		const double *dmat, double lambda, const int *bc, int apply_mask,
 		const int *done, double *part);
 int ssxk_ax_blocks(int n, int nel);
+/* ssxk_ax of the new CG direction: p = (z - scal[ic]) + scal[ib] p
+ * (ic < 0: no null-space constant), masked where apply_mask, written back
+ * to p, and ap = A p -- the k_cg_pupdate of the previous iteration fused
+ * into the operator */
+int ssxk_ax_pupd(void *stream, int n, int nel, int op, const double *z,
+		double *p, const double *scal, int ib, int ic, double *ap,
+		const double *gf, const double *mass, const double *dmat,
+		double lambda, const int *bc, int apply_mask, const int *done,
+		double *part);
+int ssxk_ax_pupd_lp(void *stream, int prec, int n, int nel, int op,
+		const void *z, void *p, const double *scal, int ib, int ic, void *ap,
+		const void *gf, const void *mass, const void *dmat, double lambda,
+		const int *bc, int apply_mask, const int *done, double *part);
 long long ssxk_dss_blocks(const long long ng[3]);
 /* DSS of all groups in one launch; average: scale by 1/M; p, part
  * (nullable): per block shared-node part of <p, DSS(u)>_w */
@@ -150,6 +165,20 @@ int ssxk_cg_reduce(void *stream, int mode, const double *part, long long cnt,
 int ssxk_cg_tol(void *stream, double *scal, int dst, int src, double eps,
 		double rtol);
 int ssxk_set_ints(void *stream, int *dst, int v, int n);
+/* FCG: scal[dst+k] = <x, ys[k]>_w for k < m (each bit-identical to
+ * ssxk_dot + ssxk_reduce); div >= 0: scal[quot+k] = scal[dst+k] /
+ * scal[div+k] */
+int ssxk_multidot(void *stream, size_t n, const double *x, const double *s,
+		const double *const *ys, int m, double *part, double *scal, int dst,
+		int div, int quot);
+/* y = mask * (x - sum_k scal[ib+k] ps[k]), k = 0..m-1 in order */
+int ssxk_multi_axpy(void *stream, size_t n, const double *x,
+		const double *const *ps, int m, const double *scal, int ib,
+		const double *mask, double *y);
+/* u += a p; r -= a ap; part[blk] = partial <r,r>_w (a = scal[ia]) */
+int ssxk_fcg_step(void *stream, size_t n, const double *scal, int ia,
+		const double *p, const double *ap, double *u, double *r,
+		const double *s, double *part);
 /* low-precision (FP32 fields) variants; prec selects the factor type of
  * ssxk_ax_lp (SSX_PREC_FP32 or SSX_PREC_BF16) */
 int ssxk_ax_lp(void *stream, int prec, int n, int nel, int op, const void *u,

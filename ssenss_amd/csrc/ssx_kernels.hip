@@ -64,13 +64,18 @@ struct AxShape {
 	static constexpr int BLOCK = TEAMS * N2;
 };
 
-template <int N, int OP, typename T, typename G>
+// PUPD: the CG direction update p = (z - c) + beta p (masked; the
+// k_cg_pupdate arithmetic) is done while the element is loaded, p is
+// written back and the operator is applied to the new p.  u is not read.
+template <int N, int OP, typename T, typename G, bool PUPD>
 __global__ void __launch_bounds__(AxShape<N>::BLOCK)
 k_ax(int nel, const T *__restrict__ u, T *__restrict__ au,
 		const G *__restrict__ gf, const T *__restrict__ mass,
 		const T *__restrict__ dmat, T lambda,
 		const int *__restrict__ bc, int apply_mask,
-		const int *__restrict__ done, double *__restrict__ part)
+		const int *__restrict__ done, double *__restrict__ part,
+		const T *__restrict__ zin, T *__restrict__ pio,
+		const double *__restrict__ scal, int ib, int ic)
 {
 	using S = AxShape<N>;
 	constexpr int N2 = S::N2, N3 = S::N3, P = S::P, TEAMS = S::TEAMS;
@@ -96,12 +101,39 @@ k_ax(int nel, const T *__restrict__ u, T *__restrict__ au,
 
 	const size_t eoff = (size_t) (act ? e : 0) * N3;
 	T *su = s_u[team], *sa = s_a[team], *sb = s_b[team];
+	// bc[e]: bits 0-5 Dirichlet faces, bits 8-13 faces with a neighbour
+	const int eb = act ? bc[e] : 0;
+	const int bits = apply_mask ? eb : 0;
+	const bool mi = ((bits & 1) && i == 0) || ((bits & 2) && i == N - 1)
+		|| ((bits & 4) && j == 0) || ((bits & 8) && j == N - 1);
 	T uc[N], acc[N];
+	if (PUPD) {
+		const T bb = (T) scal[ib];
+		const T cc = ic >= 0 ? (T) -scal[ic] : T(0);
 #pragma unroll
-	for (int k = 0; k < N; ++k) {
-		uc[k] = act ? u[eoff + k * N2 + tid] : T(0);
-		su[(k * N + j) * P + i] = uc[k];
-		acc[k] = T(0);
+		for (int k = 0; k < N; ++k) {
+			T v = T(0);
+			if (act) {
+				const size_t q = eoff + k * N2 + tid;
+				const T zv = ic >= 0 ? cc * T(1) + zin[q] : zin[q];
+				v = bb * pio[q] + zv;
+				const bool m = mi || ((bits & 16) && k == 0)
+					|| ((bits & 32) && k == N - 1);
+				if (apply_mask)
+					v = v * (m ? T(0) : T(1));
+				pio[q] = v;
+			}
+			uc[k] = v;
+			su[(k * N + j) * P + i] = v;
+			acc[k] = T(0);
+		}
+	} else {
+#pragma unroll
+		for (int k = 0; k < N; ++k) {
+			uc[k] = act ? u[eoff + k * N2 + tid] : T(0);
+			su[(k * N + j) * P + i] = uc[k];
+			acc[k] = T(0);
+		}
 	}
 	__syncthreads();
 	const T *Dn = s_d;
@@ -148,11 +180,6 @@ k_ax(int nel, const T *__restrict__ u, T *__restrict__ au,
 			acc[kk] += (kk == k ? r : T(0)) + gDs[k * N + kk] * t2;
 		__syncthreads();
 	}
-	// bc[e]: bits 0-5 Dirichlet faces, bits 8-13 faces with a neighbour
-	const int eb = act ? bc[e] : 0;
-	const int bits = apply_mask ? eb : 0;
-	const bool mi = ((bits & 1) && i == 0) || ((bits & 2) && i == N - 1)
-		|| ((bits & 4) && j == 0) || ((bits & 8) && j == N - 1);
 	const bool si = ((eb & 256) && i == 0) || ((eb & 512) && i == N - 1)
 		|| ((eb & 1024) && j == 0) || ((eb & 2048) && j == N - 1);
 	double pd = 0.0;
@@ -193,24 +220,51 @@ k_ax(int nel, const T *__restrict__ u, T *__restrict__ au,
 	}
 }
 
+// the p update of a fused CG iteration (PUPD): z, p (read and written),
+// beta at scal[ib], null-space constant at scal[ic] (ic < 0: none)
+struct PUpd {
+	const void *z;
+	void *p;
+	const double *scal;
+	int ib, ic;
+};
+
+template <int N, typename T, typename G, bool PU>
+static void launch_ax1(hipStream_t st, int grid, int nel, int op,
+		const void *u, void *au, const void *gf, const void *mass,
+		const void *dmat, double lambda, const int *bc, int apply_mask,
+		const int *done, double *part, const PUpd &pu)
+{
+	using S = AxShape<N>;
+	if (op == SSX_OP_HELMHOLTZ)
+		hipLaunchKernelGGL((k_ax<N, 1, T, G, PU>), dim3(grid), dim3(S::BLOCK),
+				0, st, nel, (const T *) u, (T *) au, (const G *) gf,
+				(const T *) mass, (const T *) dmat, (T) lambda, bc,
+				apply_mask, done, part, (const T *) pu.z, (T *) pu.p, pu.scal,
+				pu.ib, pu.ic);
+	else
+		hipLaunchKernelGGL((k_ax<N, 0, T, G, PU>), dim3(grid), dim3(S::BLOCK),
+				0, st, nel, (const T *) u, (T *) au, (const G *) gf,
+				(const T *) mass, (const T *) dmat, (T) lambda, bc,
+				apply_mask, done, part, (const T *) pu.z, (T *) pu.p, pu.scal,
+				pu.ib, pu.ic);
+}
+
 template <int N, typename T, typename G>
 static int launch_ax(hipStream_t st, int nel, int op, const void *u,
 		void *au, const void *gf, const void *mass, const void *dmat,
 		double lambda, const int *bc, int apply_mask, const int *done,
-		double *part)
+		double *part, const PUpd *pu)
 {
 	using S = AxShape<N>;
 	const int grid = (nel + S::TEAMS - 1) / S::TEAMS;
-	if (op == SSX_OP_HELMHOLTZ)
-		hipLaunchKernelGGL((k_ax<N, 1, T, G>), dim3(grid), dim3(S::BLOCK), 0,
-				st, nel, (const T *) u, (T *) au, (const G *) gf,
-				(const T *) mass, (const T *) dmat, (T) lambda, bc,
-				apply_mask, done, part);
+	if (pu)
+		launch_ax1<N, T, G, true>(st, grid, nel, op, u, au, gf, mass, dmat,
+				lambda, bc, apply_mask, done, part, *pu);
 	else
-		hipLaunchKernelGGL((k_ax<N, 0, T, G>), dim3(grid), dim3(S::BLOCK), 0,
-				st, nel, (const T *) u, (T *) au, (const G *) gf,
-				(const T *) mass, (const T *) dmat, (T) lambda, bc,
-				apply_mask, done, part);
+		launch_ax1<N, T, G, false>(st, grid, nel, op, u, au, gf, mass, dmat,
+				lambda, bc, apply_mask, done, part, PUpd{ nullptr, nullptr,
+				nullptr, 0, -1 });
 	LAUNCH_CHECK("k_ax");
 	return SSX_OK;
 }
@@ -219,11 +273,11 @@ template <typename T, typename G>
 static int dispatch_ax(hipStream_t st, int n, int nel, int op, const void *u,
 		void *au, const void *gf, const void *mass, const void *dmat,
 		double lambda, const int *bc, int apply_mask, const int *done,
-		double *part)
+		double *part, const PUpd *pu = nullptr)
 {
 	switch (n) {
 #define CASE(NN) case NN: return launch_ax<NN, T, G>(st, nel, op, u, au, gf, \
-			mass, dmat, lambda, bc, apply_mask, done, part);
+			mass, dmat, lambda, bc, apply_mask, done, part, pu);
 	CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9)
 	CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
 #undef CASE
@@ -248,6 +302,38 @@ extern "C" int ssxk_ax(void *stream, int n, int nel, int op, const double *u,
 		return SSX_OK;
 	return dispatch_ax<double, double>((hipStream_t) stream, n, nel, op, u,
 			au, gf, mass, dmat, lambda, bc, apply_mask, done, part);
+}
+
+extern "C" int ssxk_ax_pupd(void *stream, int n, int nel, int op,
+		const double *z, double *p, const double *scal, int ib, int ic,
+		double *ap, const double *gf, const double *mass, const double *dmat,
+		double lambda, const int *bc, int apply_mask, const int *done,
+		double *part)
+{
+	if (nel <= 0)
+		return SSX_OK;
+	const PUpd pu = { z, p, scal, ib, ic };
+	return dispatch_ax<double, double>((hipStream_t) stream, n, nel, op,
+			nullptr, ap, gf, mass, dmat, lambda, bc, apply_mask, done, part,
+			&pu);
+}
+
+extern "C" int ssxk_ax_pupd_lp(void *stream, int prec, int n, int nel,
+		int op, const void *z, void *p, const double *scal, int ib, int ic,
+		void *ap, const void *gf, const void *mass, const void *dmat,
+		double lambda, const int *bc, int apply_mask, const int *done,
+		double *part)
+{
+	if (nel <= 0)
+		return SSX_OK;
+	const PUpd pu = { z, p, scal, ib, ic };
+	if (prec == SSX_PREC_BF16)
+		return dispatch_ax<float, bf16_t>((hipStream_t) stream, n, nel, op,
+				nullptr, ap, gf, mass, dmat, lambda, bc, apply_mask, done,
+				part, &pu);
+	return dispatch_ax<float, float>((hipStream_t) stream, n, nel, op,
+			nullptr, ap, gf, mass, dmat, lambda, bc, apply_mask, done, part,
+			&pu);
 }
 
 extern "C" int ssxk_ax_lp(void *stream, int prec, int n, int nel, int op,
@@ -775,6 +861,153 @@ extern "C" int ssxk_reduce(void *stream, double *part, int slot, double *scal,
 	hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, (hipStream_t) stream,
 			part + (size_t) slot * SSX_NPART, scal, dst);
 	LAUNCH_CHECK("k_reduce");
+	return SSX_OK;
+}
+
+// ---------------------------------------------------------------------
+// Flexible CG (solver.c:83-157) orthogonalisation, fused: each kernel
+// reads every vector once.  Per-thread accumulation order, grid and tree
+// are those of k_dot / k_reduce, so every inner product is bit-identical
+// to the one-dot-per-launch sequence.
+// ---------------------------------------------------------------------
+
+struct VecSet {
+	const double *v[SSX_FCG_M];
+};
+
+// part[k*NPART + blk] = block partial of <x, y_k>_w, k < m
+__global__ void __launch_bounds__(256)
+k_multidot(size_t n, const double *__restrict__ x,
+		const double *__restrict__ s, VecSet ys, int m,
+		double *__restrict__ part)
+{
+	__shared__ double sh[256];
+	double acc[SSX_FCG_M];
+#pragma unroll
+	for (int k = 0; k < SSX_FCG_M; ++k)
+		acc[k] = 0.0;
+	GRID_STRIDE(q, n) {
+		const double a = x[q] * s[q];
+#pragma unroll
+		for (int k = 0; k < SSX_FCG_M; ++k)
+			if (k < m)
+				acc[k] += a * ys.v[k][q];
+	}
+	for (int k = 0; k < m; ++k) {
+		const double t = block_sum256(acc[k], sh);
+		if (threadIdx.x == 0)
+			part[(size_t) k * SSX_NPART + blockIdx.x] = t;
+		__syncthreads();
+	}
+}
+
+// block k: scal[dst + k] = sum of part[k*NPART ..]; with div >= 0 also
+// scal[quot + k] = scal[dst + k] / scal[div + k]
+__global__ void __launch_bounds__(256)
+k_reduce_multi(const double *__restrict__ part, double *__restrict__ scal,
+		int dst, int div, int quot)
+{
+	__shared__ double sh[256];
+	const double *pp = part + (size_t) blockIdx.x * SSX_NPART;
+	double acc = 0.0;
+	for (int q = threadIdx.x; q < SSX_NPART; q += 256)
+		acc += pp[q];
+	const double t = block_sum256(acc, sh);
+	if (threadIdx.x == 0) {
+		scal[dst + blockIdx.x] = t;
+		if (div >= 0)
+			scal[quot + blockIdx.x] = t / scal[div + blockIdx.x];
+	}
+}
+
+// y = mask * (x - b_0 p_0 - b_1 p_1 - ... - b_{m-1} p_{m-1}), the updates
+// applied in that order (one axpy per history vector in the reference)
+__global__ void __launch_bounds__(256)
+k_multi_axpy(size_t n, const double *__restrict__ x, VecSet ps, int m,
+		const double *__restrict__ scal, int ib,
+		const double *__restrict__ mask, double *__restrict__ y)
+{
+	double b[SSX_FCG_M];
+#pragma unroll
+	for (int k = 0; k < SSX_FCG_M; ++k)
+		b[k] = k < m ? -scal[ib + k] : 0.0;
+	GRID_STRIDE(q, n) {
+		double v = x[q];
+#pragma unroll
+		for (int k = 0; k < SSX_FCG_M; ++k)
+			if (k < m)
+				v = b[k] * ps.v[k][q] + v;
+		y[q] = mask ? v * mask[q] : v;
+	}
+}
+
+extern "C" int ssxk_multidot(void *stream, size_t n, const double *x,
+		const double *s, const double *const *ys, int m, double *part,
+		double *scal, int dst, int div, int quot)
+{
+	if (m <= 0)
+		return SSX_OK;
+	if (m > SSX_FCG_M) {
+		ssx_set_error("ssxk_multidot: m=%d > %d", m, SSX_FCG_M);
+		return SSX_EINVAL;
+	}
+	VecSet v;
+	for (int k = 0; k < SSX_FCG_M; ++k)
+		v.v[k] = k < m ? ys[k] : ys[0];
+	hipLaunchKernelGGL(k_multidot, dim3(SSX_NPART), dim3(256), 0,
+			(hipStream_t) stream, n, x, s, v, m, part);
+	LAUNCH_CHECK("k_multidot");
+	hipLaunchKernelGGL(k_reduce_multi, dim3(m), dim3(256), 0,
+			(hipStream_t) stream, part, scal, dst, div, quot);
+	LAUNCH_CHECK("k_reduce_multi");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_multi_axpy(void *stream, size_t n, const double *x,
+		const double *const *ps, int m, const double *scal, int ib,
+		const double *mask, double *y)
+{
+	if (m > SSX_FCG_M) {
+		ssx_set_error("ssxk_multi_axpy: m=%d > %d", m, SSX_FCG_M);
+		return SSX_EINVAL;
+	}
+	VecSet v;
+	for (int k = 0; k < SSX_FCG_M; ++k)
+		v.v[k] = k < m ? ps[k] : x;
+	hipLaunchKernelGGL(k_multi_axpy, vgrid(n), dim3(256), 0,
+			(hipStream_t) stream, n, x, v, m, scal, ib, mask, y);
+	LAUNCH_CHECK("k_multi_axpy");
+	return SSX_OK;
+}
+
+// u += a p; r -= a ap; partial <r,r>_w (the FCG step, solver.c:139-147)
+__global__ void __launch_bounds__(256)
+k_fcg_step(size_t n, const double *__restrict__ scal, int ia,
+		const double *__restrict__ p, const double *__restrict__ ap,
+		double *__restrict__ u, double *__restrict__ r,
+		const double *__restrict__ s, double *__restrict__ part)
+{
+	__shared__ double sh[256];
+	const double a = scal[ia], am = -scal[ia];
+	double acc = 0.0;
+	GRID_STRIDE(q, n) {
+		u[q] = a * p[q] + u[q];
+		const double rv = am * ap[q] + r[q];
+		r[q] = rv;
+		acc += (double) (rv * s[q]) * (double) rv;
+	}
+	const double t = block_sum256(acc, sh);
+	if (threadIdx.x == 0)
+		part[blockIdx.x] = t;
+}
+
+extern "C" int ssxk_fcg_step(void *stream, size_t n, const double *scal,
+		int ia, const double *p, const double *ap, double *u, double *r,
+		const double *s, double *part)
+{
+	hipLaunchKernelGGL(k_fcg_step, dim3(SSX_NPART), dim3(256), 0,
+			(hipStream_t) stream, n, scal, ia, p, ap, u, r, s, part);
+	LAUNCH_CHECK("k_fcg_step");
 	return SSX_OK;
 }
 
