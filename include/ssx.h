@@ -36,6 +36,7 @@
  *   ssx_particles_set_velocity  compute_reference_domain_velocity
  *                       geometry.c:188-198
  *   ssx_particles_step  integrate_particle_equations particles.c:182-218
+ *   ssx_particles_*_host    the same on host arrays (the shim's entry)
  */
 #ifndef SSX_H
 #define SSX_H
@@ -233,6 +234,13 @@ int ssx_particles_set_velocity(ssx_ctx *ctx, const double *u,
  * SSX_EINVAL. */
 int ssx_particles_step(ssx_ctx *ctx, const ssx_particles *p, double t_now,
 		double dt, int *max_passes);
+/* the same two calls on HOST arrays (velocity E-vectors; a particle list
+ * whose pointers are host memory), staged through device memory the
+ * context keeps; synchronous */
+int ssx_particles_set_velocity_host(ssx_ctx *ctx, const double *u,
+		const double *v, const double *w);
+int ssx_particles_step_host(ssx_ctx *ctx, const ssx_particles *p,
+		double t_now, double dt, int *max_passes);
 
 /* ---- reference entry points on HOST E-vectors: f (masked in place) and u
  * (initial guess in, solution out), FCG + p-multigrid for Poisson and

@@ -519,6 +519,7 @@ void ssx_destroy(ssx_ctx *c)
 		hipFree(c->fcg_ap[i]);
 	hipFree(c->d_f);
 	hipFree(c->d_u);
+	hipFree(c->p_stage);
 	free(c->h_neigh);
 	hipFree(c->d_neigh);
 	for (int a = 0; a < 9; ++a) {
@@ -1594,4 +1595,93 @@ int ssx_particles_step(ssx_ctx *c, const ssx_particles *p, double t_now,
 		return SSX_EINVAL;
 	}
 	return SSX_OK;
+}
+
+/* host-array variants: stage through device memory held by the context */
+static int stage_bytes(ssx_ctx *c, size_t bytes)
+{
+	if (c->p_stage_bytes >= bytes)
+		return SSX_OK;
+	hipFree(c->p_stage);
+	c->p_stage = NULL;
+	c->p_stage_bytes = 0;
+	if (hipMalloc(&c->p_stage, bytes ? bytes : 1) != hipSuccess) {
+		c->p_stage = NULL;
+		ssx_set_error("hipMalloc of %zu bytes of particle staging", bytes);
+		return SSX_ENOMEM;
+	}
+	c->p_stage_bytes = bytes;
+	return SSX_OK;
+}
+
+int ssx_particles_set_velocity_host(ssx_ctx *c, const double *u,
+		const double *v, const double *w)
+{
+	TRY(chk(c, 0));
+	if (!u || !v || !w) {
+		ssx_set_error("null velocity array");
+		return SSX_EINVAL;
+	}
+	const size_t nt = c->lev[0].ntot;
+	TRY(stage_bytes(c, 3 * nt * sizeof(double)));
+	double *d = (double *) c->p_stage;
+	hipStream_t st = (hipStream_t) c->stream;
+	HIPC(hipMemcpyAsync(d, u, nt * sizeof(double), hipMemcpyHostToDevice,
+			st));
+	HIPC(hipMemcpyAsync(d + nt, v, nt * sizeof(double),
+			hipMemcpyHostToDevice, st));
+	HIPC(hipMemcpyAsync(d + 2 * nt, w, nt * sizeof(double),
+			hipMemcpyHostToDevice, st));
+	TRY(ssx_particles_set_velocity(c, d, d + nt, d + 2 * nt));
+	HIPC(hipStreamSynchronize(st));
+	return SSX_OK;
+}
+
+int ssx_particles_step_host(ssx_ctx *c, const ssx_particles *h, double t_now,
+		double dt, int *max_passes)
+{
+	TRY(chk(c, 0));
+	if (!h || h->n < 0 || (h->n > 0 && (!h->el || !h->old_el || !h->x
+			|| !h->y || !h->z || !h->u || !h->v || !h->w || !h->t
+			|| !h->tau))) {
+		ssx_set_error("bad particle list");
+		return SSX_EINVAL;
+	}
+	const size_t n = (size_t) h->n;
+	if (n == 0) {
+		if (max_passes)
+			*max_passes = 0;
+		return SSX_OK;
+	}
+	/* layout: 8 double columns x..tau, then el, old_el, reinit */
+	TRY(stage_bytes(c, n * (8 * sizeof(double) + 3 * sizeof(int))));
+	double *dd = (double *) c->p_stage;
+	int *di = (int *) (dd + 8 * n);
+	double *hcol[8] = { h->x, h->y, h->z, h->u, h->v, h->w, h->t, h->tau };
+	hipStream_t st = (hipStream_t) c->stream;
+	for (int k = 0; k < 8; ++k)
+		HIPC(hipMemcpyAsync(dd + k * n, hcol[k], n * sizeof(double),
+				hipMemcpyHostToDevice, st));
+	HIPC(hipMemcpyAsync(di, h->el, n * sizeof(int), hipMemcpyHostToDevice,
+			st));
+	HIPC(hipMemcpyAsync(di + n, h->old_el, n * sizeof(int),
+			hipMemcpyHostToDevice, st));
+	ssx_particles d = { h->n, di, di + n, dd, dd + n, dd + 2 * n, dd + 3 * n,
+		dd + 4 * n, dd + 5 * n, dd + 6 * n, dd + 7 * n, di + 2 * n };
+	const int rc = ssx_particles_step(c, &d, t_now, dt, max_passes);
+	if (rc != SSX_OK && rc != SSX_EINVAL)
+		return rc;
+	/* results come back also when some particles named no element */
+	for (int k = 0; k < 7; ++k)
+		HIPC(hipMemcpyAsync(hcol[k], dd + k * n, n * sizeof(double),
+				hipMemcpyDeviceToHost, st));
+	HIPC(hipMemcpyAsync(h->el, di, n * sizeof(int), hipMemcpyDeviceToHost,
+			st));
+	HIPC(hipMemcpyAsync(h->old_el, di + n, n * sizeof(int),
+			hipMemcpyDeviceToHost, st));
+	if (h->reinit)
+		HIPC(hipMemcpyAsync(h->reinit, di + 2 * n, n * sizeof(int),
+				hipMemcpyDeviceToHost, st));
+	HIPC(hipStreamSynchronize(st));
+	return rc;
 }

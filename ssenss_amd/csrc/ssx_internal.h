@@ -107,6 +107,8 @@ struct ssx_ctx {
 	int *h_neigh, *d_neigh;
 	double *p_jac[9], *p_jinv[9], *p_ur[3][3], *p_xi;
 	int *p_maxpass;
+	void *p_stage;               /* device staging of host particle lists */
+	size_t p_stage_bytes;
 	long long cnt[4];
 };
 

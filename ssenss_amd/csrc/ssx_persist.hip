@@ -50,17 +50,21 @@ struct PersistArgs {
 	unsigned *bar;                  // [0] arrivals, [1] generation, [2] abort
 };
 
-// Grid barrier.  Stores before it are made visible device-wide (agent-scope
-// release; the L2 of each XCD is written back), loads after it see them
-// (agent-scope acquire).  Returns false when the grid is aborting.
+// Grid barrier, one thread per block doing the handshake.  __syncthreads
+// completes the block's stores into this XCD's L2; the agent-scope release
+// fence then writes L2 back so other XCDs can see them, and the acquire
+// fence after the wait invalidates this CU's L1 and the stale part of L2.
+// The wait itself polls with relaxed loads (an acquire per poll would
+// invalidate the caches on every poll).  Returns false when the grid is
+// aborting.
 __device__ bool grid_sync(unsigned *bar, unsigned nblocks, unsigned &phase)
 {
 	__shared__ int s_ok;
-	__threadfence();
 	__syncthreads();
 	if (threadIdx.x == 0) {
 		int ok = 1;
 		const unsigned my = phase;
+		__threadfence();
 		const unsigned prev = atomicAdd(&bar[0], 1u);
 		if (prev == nblocks - 1) {
 			atomicExch(&bar[0], 0u);
@@ -68,7 +72,7 @@ __device__ bool grid_sync(unsigned *bar, unsigned nblocks, unsigned &phase)
 			atomicAdd(&bar[1], 1u);
 		} else {
 			unsigned n = 0;
-			while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE,
+			while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED,
 					__HIP_MEMORY_SCOPE_AGENT) == my) {
 				if (__hip_atomic_load(&bar[2], __ATOMIC_RELAXED,
 						__HIP_MEMORY_SCOPE_AGENT) || ++n > SPIN_LIMIT) {
@@ -76,7 +80,7 @@ __device__ bool grid_sync(unsigned *bar, unsigned nblocks, unsigned &phase)
 					ok = 0;
 					break;
 				}
-				__builtin_amdgcn_s_sleep(1);
+				__builtin_amdgcn_s_sleep(2);
 			}
 		}
 		__threadfence();
@@ -85,7 +89,6 @@ __device__ bool grid_sync(unsigned *bar, unsigned nblocks, unsigned &phase)
 				__HIP_MEMORY_SCOPE_AGENT);
 	}
 	__syncthreads();
-	__threadfence();
 	return s_ok;
 }
 
@@ -367,6 +370,7 @@ extern "C" int ssxk_cg_persist_max_grid(int n, int *grid)
 		ssx_set_error("persistent CG: occupancy query failed");
 		return SSX_EHIP;
 	}
-	*grid = ncu * per;
+	/* one block per CU: every barrier costs one handshake per block */
+	*grid = ncu;
 	return SSX_OK;
 }

@@ -14,6 +14,10 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr())
 
 
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
 class Context:
     def __init__(self, mesh, stream=None):
         self._lib = load()
@@ -177,4 +181,25 @@ class Context:
         mp = C.c_int()
         check(self._lib.ssx_particles_step(self.h, C.byref(ps), t_now, dt,
                                            C.byref(mp)))
+        return mp.value
+
+    # --- particles on host arrays (numpy) ---
+    def particles_set_velocity_host(self, u, v, w):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (u, v, w)]
+        check(self._lib.ssx_particles_set_velocity_host(
+            self.h, *(_dp(a) for a in arrs)))
+
+    def particles_step_host(self, p, t_now, dt):
+        """p: dict of contiguous numpy arrays (int32 el, old_el, reinit;
+        float64 x, y, z, u, v, w, t, tau), updated in place"""
+        for k in ("el", "old_el", "reinit"):
+            assert p[k].dtype == np.int32 and p[k].flags.c_contiguous
+        for k in ("x", "y", "z", "u", "v", "w", "t", "tau"):
+            assert p[k].dtype == np.float64 and p[k].flags.c_contiguous
+        ps = ParticlesStruct(int(p["x"].size), *(p[k].ctypes.data for k in
+                             ("el", "old_el", "x", "y", "z", "u", "v", "w",
+                              "t", "tau", "reinit")))
+        mp = C.c_int()
+        check(self._lib.ssx_particles_step_host(self.h, C.byref(ps), t_now,
+                                                dt, C.byref(mp)))
         return mp.value

@@ -29,7 +29,8 @@ EXPORTED_SYMBOLS = [
     "ssx_nullspace", "ssx_restrict", "ssx_prolong", "ssx_jacobi", "ssx_cg",
     "ssx_fcg", "ssx_mg_precon", "ssx_solve_poisson_cg",
     "ssx_solve_helmholtz_cg", "ssx_particles_set_velocity",
-    "ssx_particles_step",
+    "ssx_particles_step", "ssx_particles_set_velocity_host",
+    "ssx_particles_step_host",
 ]
 
 
@@ -127,6 +128,9 @@ def load():
         "ssx_solve_poisson_cg": (i, [vp, dp, dp, i, d, C.POINTER(Stats)]),
         "ssx_solve_helmholtz_cg": (i, [vp, dp, dp, i, d, C.POINTER(Stats)]),
         "ssx_particles_set_velocity": (i, [vp, vp, vp, vp]),
+        "ssx_particles_set_velocity_host": (i, [vp, dp, dp, dp]),
+        "ssx_particles_step_host": (i, [vp, C.POINTER(ParticlesStruct), d, d,
+                                        C.POINTER(C.c_int)]),
         "ssx_particles_step": (i, [vp, C.POINTER(ParticlesStruct), d, d,
                                    C.POINTER(C.c_int)]),
     }

@@ -493,3 +493,31 @@ def test_persistent_coarse_cg(name, grid):
     assert rel(a[2][1], Z["ref_fcg_pressure"]) <= 1e-8
     assert rel(a[3][1], Z["ref_fcg_dirichlet"]) <= 1e-8
     assert abs(a[2][0] - int(sc[3])) <= 1 and abs(a[3][0] - int(sc[5])) <= 1
+
+
+def test_particles_host_entry_matches_device_entry():
+    """ssx_particles_*_host (host arrays, the drop-in for the reference's
+    host particle list) gives bit for bit what the device entry gives"""
+    Z = golden("channel_n6")
+    h = Z["particle_hdr"]
+    rec = Z["in_particles"]
+    vel = Z["in_vel"]
+    with make_ctx(golden_mesh(Z)) as ctx:
+        out_d, re_d, pass_d = _run_gpu_particles(ctx, vel, h[2:], rec, h[0],
+                                                 h[1])
+    with make_ctx(golden_mesh(Z)) as ctx:
+        nt = ctx.ntot(0)
+        for s in h[2:]:
+            ctx.particles_set_velocity_host(s * vel[:nt], s * vel[nt:2 * nt],
+                                            s * vel[2 * nt:])
+        ip = lambda a: np.ascontiguousarray(a, np.int32)
+        fp = lambda a: np.ascontiguousarray(a, np.float64)
+        p = {"el": ip(rec[:, 0]), "old_el": ip(rec[:, 0]),
+             "reinit": np.zeros(len(rec), np.int32)}
+        for c, k in enumerate(("x", "y", "z", "u", "v", "w", "t", "tau")):
+            p[k] = fp(rec[:, 1 + c])
+        pass_h = ctx.particles_step_host(p, h[0], h[1])
+    out_h = np.stack([p["el"].astype(np.float64)] + [
+        p[k] for k in ("x", "y", "z", "u", "v", "w", "t")], 1)
+    assert pass_h == pass_d
+    assert np.array_equal(out_h, out_d) and np.array_equal(p["reinit"], re_d)
