@@ -242,6 +242,19 @@ int ssx_particles_set_velocity_host(ssx_ctx *ctx, const double *u,
 int ssx_particles_step_host(ssx_ctx *ctx, const ssx_particles *p,
 		double t_now, double dt, int *max_passes);
 
+/* ---- operator launch autotuning ----
+ * The element operator has SSX_AX_NVARIANTS launch shapes (elements per
+ * workgroup; prefetch of the next plane's geometric factors).  ssx_create
+ * times every shape on every level in FP64 and keeps the fastest; the
+ * reduced-precision copies are tuned when first built.  prec_mask: bit p
+ * for precision SSX_PREC_p.  Results are bit-identical across shapes.
+ * ssx_ax_variant returns the shape in use (us, nullable: the measured
+ * microseconds per launch of each shape); ssx_set_ax_variant forces one. */
+#define SSX_AX_NVARIANTS 4
+int ssx_autotune(ssx_ctx *ctx, int prec_mask);
+int ssx_ax_variant(const ssx_ctx *ctx, int level, int prec, double us[4]);
+int ssx_set_ax_variant(ssx_ctx *ctx, int level, int prec, int variant);
+
 /* ---- reference entry points on HOST E-vectors: f (masked in place) and u
  * (initial guess in, solution out), FCG + p-multigrid for Poisson and
  * Jacobi PCG for Helmholtz, max_iter 5000 ---- */

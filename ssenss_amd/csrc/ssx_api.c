@@ -443,7 +443,7 @@ int ssx_create(ssx_ctx **out, const ssx_mesh *m)
 		const size_t nt = c->lev[0].ntot;
 		size_t np = (size_t) 12 * SSX_NPART;   /* FCG: 10 dots at once */
 		for (int l = 0; l < nl; ++l) {
-			size_t q = (size_t) ssxk_ax_blocks(c->lev[l].n, c->nel)
+			size_t q = (size_t) ssxk_ax_blocks(c->lev[l].n, c->nel, 1)
 				+ (size_t) ssxk_dss_blocks(c->lev[l].ngrp);
 			if (q > np)
 				np = q;
@@ -474,6 +474,8 @@ int ssx_create(ssx_ctx **out, const ssx_mesh *m)
 		if (!rc)
 			HIPC(hipMemset(c->scal, 0, SSX_NSCAL * sizeof(double)));
 	}
+	if (rc == SSX_OK)
+		rc = ssx_autotune(c, 1 << SSX_PREC_FP64);
 	if (rc != SSX_OK) {
 		ssx_destroy(c);
 		return rc;
@@ -662,7 +664,8 @@ int ssx_ax(ssx_ctx *c, int l, int op, const double *u, double *au)
 {
 	TRY(chk(c, l));
 	ssx_level *L = &c->lev[l];
-	return ssxk_ax(c->stream, L->n, c->nel, op, u, au, L->gf, L->mass,
+	return ssxk_ax(c->stream, L->n, L->axv[0], c->nel, op, u, au, L->gf,
+			L->mass,
 			L->dmat, c->lambda, L->bc, 0, NULL, NULL);
 }
 
@@ -681,7 +684,8 @@ int ssx_opeval(ssx_ctx *c, int l, int op, int var, const double *p,
 	if (l == 0)
 		c->cnt[0]++;
 	/* element operator, mask (var != PRESSURE), DSS: solver.c:191-198 */
-	TRY(ssxk_ax(c->stream, L->n, c->nel, op, p, ap, L->gf, L->mass,
+	TRY(ssxk_ax(c->stream, L->n, L->axv[0], c->nel, op, p, ap, L->gf,
+			L->mass,
 			L->dmat, c->lambda, L->bc, var != SSX_PRESSURE, NULL, NULL));
 	return ssxk_dss_all(c->stream, L->grp, L->ngrp, ap, NULL, NULL, 0, NULL);
 }
@@ -896,7 +900,7 @@ static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
 			return SSX_EHIP;
 		}
 	} else {
-	const long long nba = ssxk_ax_blocks(L->n, c->nel);
+	const long long nba = ssxk_ax_blocks(L->n, c->nel, L->axv[0]);
 	const long long nbd = ssxk_dss_blocks(L->ngrp);
 	int launched = 0;
 	int chunk = c->cg_hint[l] > 0 ? c->cg_hint[l] + 1 : 8;
@@ -907,10 +911,12 @@ static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
 			/* from the second iteration on, the operator kernel first forms
 			 * p = (z - c) + beta p (solver.c:70-72) */
 			if (launched + k == 0)
-				TRY(ssxk_ax(st, L->n, c->nel, op, p, ap, L->gf, L->mass,
+				TRY(ssxk_ax(st, L->n, L->axv[0], c->nel, op, p, ap, L->gf,
+						L->mass,
 						L->dmat, c->lambda, L->bc, !pres, flags, c->part));
 			else
-				TRY(ssxk_ax_pupd(st, L->n, c->nel, op, z, p, c->scal,
+				TRY(ssxk_ax_pupd(st, L->n, L->axv[0], c->nel, op, z, p,
+						c->scal,
 						o + CG_BETA, pres ? o + CG_NULLC : -1, ap, L->gf,
 						L->mass, L->dmat, c->lambda, L->bc, !pres, flags,
 						c->part));
@@ -1011,6 +1017,7 @@ static int cvt_new(ssx_ctx *c, float **dst, const double *src, size_t n)
 
 static int build_lp(ssx_ctx *c, int prec)
 {
+	int tuned = 0;
 	for (int l = 0; l < c->nlev; ++l)
 		if (!c->lev[l].fused_ok) {
 			ssx_set_error("level %d: reduced-precision smoother needs the "
@@ -1052,7 +1059,10 @@ static int build_lp(ssx_ctx *c, int prec)
 			TRY(falloc(&L->mgr_lp, nt));
 		}
 		L->lp_prec = prec;
+		tuned = 1;
 	}
+	if (tuned)
+		TRY(ssx_autotune(c, 1 << prec));
 	return SSX_OK;
 }
 
@@ -1070,7 +1080,8 @@ static int opeval_lp(ssx_ctx *c, int l, int op, int var, const float *p,
 	ssx_level *L = &c->lev[l];
 	if (l == 0)
 		c->cnt[0]++;
-	TRY(ssxk_ax_lp(c->stream, L->lp_prec, L->n, c->nel, op, p, ap, L->gf_lp,
+	TRY(ssxk_ax_lp(c->stream, L->lp_prec, L->n, L->axv[L->lp_prec], c->nel,
+			op, p, ap, L->gf_lp,
 			L->mass_lp, L->dmat_lp, c->lambda, L->bc, var != SSX_PRESSURE,
 			NULL, NULL));
 	return ssxk_dss_all_lp(c->stream, L->grp, L->ngrp, ap, NULL, NULL, 0,
@@ -1113,14 +1124,16 @@ static int cg_level_lp(ssx_ctx *c, int l, int op, int var, float *f,
 	TRY(ssxk_dot_lp(st, nt, r, L->scale_lp, z, c->part, 0));
 	TRY(ssxk_reduce(st, c->part, 0, c->scal, o + CG_RDZ));
 	TRY(ssxk_set_ints(st, flags, 0, 2));
-	const long long nba = ssxk_ax_blocks(L->n, c->nel);
+	const long long nba = ssxk_ax_blocks(L->n, c->nel, L->axv[L->lp_prec]);
 	for (int k = 0; k < max_iter; ++k) {
 		if (k == 0)
-			TRY(ssxk_ax_lp(st, L->lp_prec, L->n, c->nel, op, p, ap, L->gf_lp,
+			TRY(ssxk_ax_lp(st, L->lp_prec, L->n, L->axv[L->lp_prec], c->nel,
+					op, p, ap, L->gf_lp,
 					L->mass_lp, L->dmat_lp, c->lambda, L->bc, !pres, flags,
 					c->part));
 		else
-			TRY(ssxk_ax_pupd_lp(st, L->lp_prec, L->n, c->nel, op, z, p,
+			TRY(ssxk_ax_pupd_lp(st, L->lp_prec, L->n, L->axv[L->lp_prec],
+					c->nel, op, z, p,
 					c->scal, o + CG_BETA, pres ? o + CG_NULLC : -1, ap,
 					L->gf_lp, L->mass_lp, L->dmat_lp, c->lambda, L->bc, !pres,
 					flags, c->part));
@@ -1684,4 +1697,104 @@ int ssx_particles_step_host(ssx_ctx *c, const ssx_particles *h, double t_now,
 				hipMemcpyDeviceToHost, st));
 	HIPC(hipStreamSynchronize(st));
 	return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* operator launch autotuner                                            */
+/* ------------------------------------------------------------------ */
+
+/* average time in microseconds of `reps` Poisson operator launches of
+ * variant v on level l in precision prec, on the context's stream */
+static int time_ax(ssx_ctx *c, int l, int prec, int v, int reps,
+		double *us)
+{
+	ssx_level *L = &c->lev[l];
+	hipStream_t st = (hipStream_t) c->stream;
+	hipEvent_t e0, e1;
+	HIPC(hipEventCreate(&e0));
+	HIPC(hipEventCreate(&e1));
+	int rc = SSX_OK;
+	for (int r = -2; r < reps && rc == SSX_OK; ++r) {
+		if (r == 0 && hipEventRecord(e0, st) != hipSuccess)
+			rc = SSX_EHIP;
+		if (rc)
+			break;
+		if (prec == SSX_PREC_FP64)
+			rc = ssxk_ax(c->stream, L->n, v, c->nel, SSX_OP_POISSON, L->p,
+					L->ap, L->gf, L->mass, L->dmat, c->lambda, L->bc, 1, NULL,
+					c->part);
+		else
+			rc = ssxk_ax_lp(c->stream, prec, L->n, v, c->nel, SSX_OP_POISSON,
+					L->p_lp, L->ap_lp, L->gf_lp, L->mass_lp, L->dmat_lp,
+					c->lambda, L->bc, 1, NULL, c->part);
+	}
+	float ms = 0.0f;
+	if (rc == SSX_OK && (hipEventRecord(e1, st) != hipSuccess
+			|| hipEventSynchronize(e1) != hipSuccess
+			|| hipEventElapsedTime(&ms, e0, e1) != hipSuccess)) {
+		ssx_set_error("autotune: event timing failed");
+		rc = SSX_EHIP;
+	}
+	hipEventDestroy(e0);
+	hipEventDestroy(e1);
+	*us = 1e3 * ms / reps;
+	return rc;
+}
+
+int ssx_autotune(ssx_ctx *c, int prec_mask)
+{
+	if (!c)
+		return SSX_EINVAL;
+	for (int prec = 0; prec < 3; ++prec) {
+		if (!(prec_mask & (1 << prec)))
+			continue;
+		for (int l = 0; l < c->nlev; ++l) {
+			ssx_level *L = &c->lev[l];
+			if (prec != SSX_PREC_FP64 && (l == c->nlev - 1
+					|| L->lp_prec != prec || !L->p_lp))
+				continue;       /* no copies in that precision here */
+			/* operands: any finite values */
+			if (prec == SSX_PREC_FP64)
+				TRY(ssxk_fill(c->stream, L->p, 1.0, L->ntot));
+			else
+				TRY(ssxk_fill_lp(c->stream, L->p_lp, 1.0, L->ntot));
+			/* enough launches for about a millisecond of work */
+			const int reps = L->ntot > (1 << 22) ? 5
+				: L->ntot > (1 << 18) ? 20 : 50;
+			double best = 0.0;
+			int bv = 0;
+			for (int v = 0; v < SSX_AX_NVARIANTS; ++v) {
+				double us;
+				TRY(time_ax(c, l, prec, v, reps, &us));
+				c->ax_us[l][prec][v] = us;
+				if (v == 0 || us < best) {
+					best = us;
+					bv = v;
+				}
+			}
+			L->axv[prec] = bv;
+		}
+	}
+	return SSX_OK;
+}
+
+int ssx_ax_variant(const ssx_ctx *c, int l, int prec, double us[4])
+{
+	if (!c || l < 0 || l >= c->nlev || prec < 0 || prec > 2)
+		return -1;
+	if (us)
+		for (int v = 0; v < SSX_AX_NVARIANTS; ++v)
+			us[v] = c->ax_us[l][prec][v];
+	return c->lev[l].axv[prec];
+}
+
+int ssx_set_ax_variant(ssx_ctx *c, int l, int prec, int v)
+{
+	if (!c || l < 0 || l >= c->nlev || prec < 0 || prec > 2 || v < 0
+			|| v >= SSX_AX_NVARIANTS) {
+		ssx_set_error("bad level, precision or variant");
+		return SSX_EINVAL;
+	}
+	c->lev[l].axv[prec] = v;
+	return SSX_OK;
 }

@@ -1,0 +1,301 @@
+// ssx_ax.h -- the element operator (discrete_laplace_operator,
+// poisson.c:34-56; discrete_helmholtz_operator, helmholtz.c:33-49) as a
+// gfx950 kernel template, instantiated once per precision in
+// ssx_ax_f64.hip, ssx_ax_f32.hip and ssx_ax_bf16.hip.
+//
+// One "team" of N*N lanes per element: lane (i,j) owns the column of the
+// element at (i,j) and loops over the k planes.  The element and the two
+// in-plane fluxes are staged in LDS with rows padded to N+1 (conflict-free),
+// the zeta contraction stays in registers, the six premultiplied factors
+// G~ = G*w*detJ stream from HBM in coalesced per-element blocks.  HBM-bound:
+// FP64 Poisson moves 64 B per local node (u 8, Au 8, factors 48).
+//
+// Launch variants (chosen per order by the autotuner in ssx_api.c):
+//   bit 0  one-wave teams: max(1, 64/N^2) elements per workgroup instead of
+//          max(1, 256/N^2), so the plane barriers span one wavefront
+//   bit 1  the six factors of plane k+1 are loaded while plane k is
+//          computed (more bytes in flight per lane, more registers)
+#ifndef SSX_AX_H
+#define SSX_AX_H
+#include <hip/hip_runtime.h>
+#include "ssx_internal.h"
+#include "ssx_device.h"
+
+
+template <int N, int V>
+struct AxShape {
+	static constexpr int N2 = N * N;
+	static constexpr int N3 = N * N * N;
+	static constexpr int P = N + 1;   // padded LDS row: conflict-free rows
+	static constexpr int TEAMS = (V & 1)
+		? ((64 / N2) > 0 ? (64 / N2) : 1)
+		: ((256 / N2) > 0 ? (256 / N2) : 1);
+	static constexpr int BLOCK = TEAMS * N2;
+};
+
+static inline int ssx_ax_teams(int n, int v)
+{
+	const int n2 = n * n, per = (v & 1) ? 64 : 256;
+	return per / n2 > 0 ? per / n2 : 1;
+}
+
+// the CG direction update fused into the operator (PUPD): p = (z - c) +
+// beta p (masked), with beta at scal[ib] and c at scal[ic] (ic < 0: none)
+struct PUpd {
+	const void *z;
+	void *p;
+	const double *scal;
+	int ib, ic;
+};
+
+// PUPD: the update is done while the element is loaded, p is written back
+// and the operator is applied to the new p (the k_cg_pupdate arithmetic);
+// u is not read.
+template <int N, int OP, typename T, typename G, bool PUPD, int V>
+__global__ void __launch_bounds__((AxShape<N, V>::BLOCK))
+k_ax(int nel, const T *__restrict__ u, T *__restrict__ au,
+		const G *__restrict__ gf, const T *__restrict__ mass,
+		const T *__restrict__ dmat, T lambda,
+		const int *__restrict__ bc, int apply_mask,
+		const int *__restrict__ done, double *__restrict__ part,
+		const T *__restrict__ zin, T *__restrict__ pio,
+		const double *__restrict__ scal, int ib, int ic)
+{
+	using S = AxShape<N, V>;
+	constexpr int N2 = S::N2, N3 = S::N3, P = S::P, TEAMS = S::TEAMS;
+	constexpr bool PF = (V & 2) != 0;
+	if (done && *done)          // solver already converged: no-op
+		return;
+	__shared__ T s_d[2 * N2];
+	__shared__ T s_u[TEAMS][N * N * P];
+	__shared__ T s_a[TEAMS][N * P];
+	__shared__ T s_b[TEAMS][N * P];
+
+	const int team = threadIdx.x / N2;
+	const int tid = threadIdx.x - team * N2;
+	const int i = tid % N, j = tid / N;
+	const int e = blockIdx.x * TEAMS + team;
+	const bool act = e < nel;
+
+	for (int q = threadIdx.x; q < 2 * N2; q += blockDim.x)
+		s_d[q] = dmat[q];
+	// D_il at [i*N+l]; the SVV matrix enters the xi-forward and the
+	// eta/zeta-transpose products only (helmholtz.c:41-44, deriv.c:29-66)
+	const T *gDn = dmat;
+	const T *gDs = dmat + (OP == 1 ? N2 : 0);
+
+	const size_t eoff = (size_t) (act ? e : 0) * N3;
+	T *su = s_u[team], *sa = s_a[team], *sb = s_b[team];
+	// bc[e]: bits 0-5 Dirichlet faces, bits 8-13 faces with a neighbour
+	const int eb = act ? bc[e] : 0;
+	const int bits = apply_mask ? eb : 0;
+	const bool mi = ((bits & 1) && i == 0) || ((bits & 2) && i == N - 1)
+		|| ((bits & 4) && j == 0) || ((bits & 8) && j == N - 1);
+	T uc[N], acc[N];
+	if (PUPD) {
+		const T bb = (T) scal[ib];
+		const T cc = ic >= 0 ? (T) -scal[ic] : T(0);
+#pragma unroll
+		for (int k = 0; k < N; ++k) {
+			T v = T(0);
+			if (act) {
+				const size_t q = eoff + k * N2 + tid;
+				const T zv = ic >= 0 ? cc * T(1) + zin[q] : zin[q];
+				v = bb * pio[q] + zv;
+				const bool m = mi || ((bits & 16) && k == 0)
+					|| ((bits & 32) && k == N - 1);
+				if (apply_mask)
+					v = v * (m ? T(0) : T(1));
+				pio[q] = v;
+			}
+			uc[k] = v;
+			su[(k * N + j) * P + i] = v;
+			acc[k] = T(0);
+		}
+	} else {
+#pragma unroll
+		for (int k = 0; k < N; ++k) {
+			uc[k] = act ? u[eoff + k * N2 + tid] : T(0);
+			su[(k * N + j) * P + i] = uc[k];
+			acc[k] = T(0);
+		}
+	}
+	const G *g = gf + (size_t) (act ? e : 0) * 6 * N3;
+	G gn[6];
+	if (PF && act) {
+#pragma unroll
+		for (int c = 0; c < 6; ++c)
+			gn[c] = g[c * N3 + tid];
+	}
+	__syncthreads();
+	const T *Dn = s_d;
+	const T *Ds = s_d + (OP == 1 ? N2 : 0);
+	T dxf[N], dyf[N], dxt[N], dyt[N];
+#pragma unroll
+	for (int l = 0; l < N; ++l) {
+		dxf[l] = Ds[i * N + l];
+		dyf[l] = Dn[j * N + l];
+		dxt[l] = Dn[l * N + i];
+		dyt[l] = Ds[l * N + j];
+	}
+
+#pragma unroll 1
+	for (int k = 0; k < N; ++k) {
+		G gc[6];
+		if (PF) {
+#pragma unroll
+			for (int c = 0; c < 6; ++c)
+				gc[c] = gn[c];
+			if (act && k + 1 < N) {
+#pragma unroll
+				for (int c = 0; c < 6; ++c)
+					gn[c] = g[c * N3 + (k + 1) * N2 + tid];
+			}
+		}
+		T gx = T(0), gy = T(0), gz = T(0);
+		const T *plane = su + k * N * P;
+#pragma unroll
+		for (int l = 0; l < N; ++l) {
+			gx += dxf[l] * plane[j * P + l];
+			gy += dyf[l] * plane[l * P + i];
+			gz += gDn[k * N + l] * uc[l];      // uniform: scalar loads
+		}
+		T t0 = T(0), t1 = T(0), t2 = T(0);
+		if (act) {
+			const int q = k * N2 + tid;
+			T gxx, gxy, gxz, gyy, gyz, gzz;
+			if (PF) {
+				gxx = ldv(gc[0]); gxy = ldv(gc[1]); gxz = ldv(gc[2]);
+				gyy = ldv(gc[3]); gyz = ldv(gc[4]); gzz = ldv(gc[5]);
+			} else {
+				gxx = ldg(g, q); gxy = ldg(g, N3 + q);
+				gxz = ldg(g, 2 * N3 + q); gyy = ldg(g, 3 * N3 + q);
+				gyz = ldg(g, 4 * N3 + q); gzz = ldg(g, 5 * N3 + q);
+			}
+			t0 = gxx * gx + gxy * gy + gxz * gz;
+			t1 = gxy * gx + gyy * gy + gyz * gz;
+			t2 = gxz * gx + gyz * gy + gzz * gz;
+		}
+		sa[j * P + i] = t0;
+		sb[j * P + i] = t1;
+		__syncthreads();
+		T r = T(0);
+#pragma unroll
+		for (int l = 0; l < N; ++l)
+			r += dxt[l] * sa[j * P + l] + dyt[l] * sb[l * P + i];
+#pragma unroll
+		for (int kk = 0; kk < N; ++kk)
+			acc[kk] += (kk == k ? r : T(0)) + gDs[k * N + kk] * t2;
+		__syncthreads();
+	}
+	const bool si = ((eb & 256) && i == 0) || ((eb & 512) && i == N - 1)
+		|| ((eb & 1024) && j == 0) || ((eb & 2048) && j == N - 1);
+	double pd = 0.0;
+	if (act) {
+#pragma unroll
+		for (int k = 0; k < N; ++k) {
+			const int q = k * N2 + tid;
+			T v = acc[k];
+			if (OP == 1)
+				v += lambda * (mass[eoff + q] * uc[k]);
+			const bool m = mi || ((bits & 16) && k == 0)
+				|| ((bits & 32) && k == N - 1);
+			v = m ? T(0) : v;
+			au[eoff + q] = v;
+			// <p, Ap>_w over nodes no other element shares (weight 1);
+			// the shared ones are added by the DSS kernel after assembly
+			const bool sh = si || ((eb & 4096) && k == 0)
+				|| ((eb & 8192) && k == N - 1);
+			if (!sh)
+				pd += (double) uc[k] * (double) v;
+		}
+	}
+	if (part) {
+		__shared__ double s_red[256];
+		for (int q = threadIdx.x; q < 256; q += blockDim.x)
+			s_red[q] = 0.0;
+		__syncthreads();
+		s_red[threadIdx.x] = pd;
+		__syncthreads();
+#pragma unroll
+		for (int st = 128; st > 0; st >>= 1) {
+			if ((int) threadIdx.x < st)
+				s_red[threadIdx.x] += s_red[threadIdx.x + st];
+			__syncthreads();
+		}
+		if (threadIdx.x == 0)
+			part[blockIdx.x] = s_red[0];
+	}
+}
+
+template <int N, typename T, typename G, bool PU, int V>
+static void launch_ax1(hipStream_t st, int nel, int op, const void *u,
+		void *au, const void *gf, const void *mass, const void *dmat,
+		double lambda, const int *bc, int apply_mask, const int *done,
+		double *part, const PUpd &pu)
+{
+	using S = AxShape<N, V>;
+	const int grid = (nel + S::TEAMS - 1) / S::TEAMS;
+	if (op == SSX_OP_HELMHOLTZ)
+		hipLaunchKernelGGL((k_ax<N, 1, T, G, PU, V>), dim3(grid),
+				dim3(S::BLOCK), 0, st, nel, (const T *) u, (T *) au,
+				(const G *) gf, (const T *) mass, (const T *) dmat, (T) lambda,
+				bc, apply_mask, done, part, (const T *) pu.z, (T *) pu.p,
+				pu.scal, pu.ib, pu.ic);
+	else
+		hipLaunchKernelGGL((k_ax<N, 0, T, G, PU, V>), dim3(grid),
+				dim3(S::BLOCK), 0, st, nel, (const T *) u, (T *) au,
+				(const G *) gf, (const T *) mass, (const T *) dmat, (T) lambda,
+				bc, apply_mask, done, part, (const T *) pu.z, (T *) pu.p,
+				pu.scal, pu.ib, pu.ic);
+}
+
+template <int N, typename T, typename G, int V>
+static void launch_ax_v(hipStream_t st, int nel, int op, const void *u,
+		void *au, const void *gf, const void *mass, const void *dmat,
+		double lambda, const int *bc, int apply_mask, const int *done,
+		double *part, const PUpd *pu)
+{
+	if (pu)
+		launch_ax1<N, T, G, true, V>(st, nel, op, u, au, gf, mass, dmat,
+				lambda, bc, apply_mask, done, part, *pu);
+	else
+		launch_ax1<N, T, G, false, V>(st, nel, op, u, au, gf, mass, dmat,
+				lambda, bc, apply_mask, done, part, PUpd{ nullptr, nullptr,
+				nullptr, 0, -1 });
+}
+
+// dispatcher body shared by the per-precision files: n in [2,16], variant
+// v in [0, SSX_AX_NVARIANTS)
+template <typename T, typename G>
+static int ax_dispatch(hipStream_t st, int n, int v, int nel, int op,
+		const void *u, void *au, const void *gf, const void *mass,
+		const void *dmat, double lambda, const int *bc, int apply_mask,
+		const int *done, double *part, const PUpd *pu)
+{
+	if (v < 0 || v >= SSX_AX_NVARIANTS) {
+		ssx_set_error("ssxk_ax: variant %d unsupported", v);
+		return SSX_EINVAL;
+	}
+	switch (n * SSX_AX_NVARIANTS + v) {
+#define CASEV(NN, VV) case NN * SSX_AX_NVARIANTS + VV: \
+		launch_ax_v<NN, T, G, VV>(st, nel, op, u, au, gf, mass, dmat, \
+				lambda, bc, apply_mask, done, part, pu); break;
+#define CASE(NN) CASEV(NN, 0) CASEV(NN, 1) CASEV(NN, 2) CASEV(NN, 3)
+	CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9)
+	CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+#undef CASEV
+	default:
+		ssx_set_error("ssxk_ax: n=%d unsupported", n);
+		return SSX_EINVAL;
+	}
+	hipError_t e = hipGetLastError();
+	if (e != hipSuccess) {
+		ssx_set_error("k_ax: %s", hipGetErrorString(e));
+		return SSX_EHIP;
+	}
+	return SSX_OK;
+}
+
+#endif

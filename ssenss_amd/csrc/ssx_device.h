@@ -13,6 +13,14 @@ __device__ __forceinline__ float ldg(const bf16_t *p, size_t i)
 	return __uint_as_float(((unsigned) p[i]) << 16);
 }
 
+// a loaded factor as arithmetic type
+__device__ __forceinline__ double ldv(double x) { return x; }
+__device__ __forceinline__ float ldv(float x) { return x; }
+__device__ __forceinline__ float ldv(bf16_t x)
+{
+	return __uint_as_float(((unsigned) x) << 16);
+}
+
 // partner order of copy t: t ^ ord(k), k = 1..M-1 (faces x,y,z, then edges
 // along z, y, x, then the corner; global.c:252-256)
 __device__ __forceinline__ int dss_ord(int M, int k)

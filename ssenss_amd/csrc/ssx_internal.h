@@ -63,6 +63,8 @@ typedef struct {
 	long long ngrp[3];
 	int *bc;                     /* per element Dirichlet bits           */
 	double nunique;              /* <1,1>_w                              */
+	int axv[3];                  /* operator launch variant per precision
+	                                (SSX_PREC_*), set by the autotuner   */
 	int fused_ok;                /* multiplicity > 1 exactly on faces with
 	                                a neighbour: fused inner products OK */
 	/* solver scratch */
@@ -110,6 +112,7 @@ struct ssx_ctx {
 	void *p_stage;               /* device staging of host particle lists */
 	size_t p_stage_bytes;
 	long long cnt[4];
+	double ax_us[SSX_LMAX][3][SSX_AX_NVARIANTS];  /* autotuner timings  */
 };
 
 /* ---- host setup (ssx_setup.c) ---- */
@@ -128,21 +131,22 @@ void ssx_set_error(const char *fmt, ...);
 /* bc[e]: bits 0-5 Dirichlet faces (W,E,S,N,B,T), bits 8-13 faces with a
  * neighbour; done (nullable): skip when *done; part (nullable): per block
  * <p,Ap> over nodes of multiplicity 1 */
-int ssxk_ax(void *stream, int n, int nel, int op, const double *u,
+int ssxk_ax(void *stream, int n, int v, int nel, int op, const double *u,
 		double *au, const double *gf, const double *mass,
 		const double *dmat, double lambda, const int *bc, int apply_mask,
 		const int *done, double *part);
-int ssxk_ax_blocks(int n, int nel);
+/* workgroups of one ssxk_ax launch (= partial sums it writes) */
+int ssxk_ax_blocks(int n, int nel, int v);
 /* ssxk_ax of the new CG direction: p = (z - scal[ic]) + scal[ib] p
  * (ic < 0: no null-space constant), masked where apply_mask, written back
  * to p, and ap = A p -- the k_cg_pupdate of the previous iteration fused
  * into the operator */
-int ssxk_ax_pupd(void *stream, int n, int nel, int op, const double *z,
+int ssxk_ax_pupd(void *stream, int n, int v, int nel, int op, const double *z,
 		double *p, const double *scal, int ib, int ic, double *ap,
 		const double *gf, const double *mass, const double *dmat,
 		double lambda, const int *bc, int apply_mask, const int *done,
 		double *part);
-int ssxk_ax_pupd_lp(void *stream, int prec, int n, int nel, int op,
+int ssxk_ax_pupd_lp(void *stream, int prec, int n, int v, int nel, int op,
 		const void *z, void *p, const double *scal, int ib, int ic, void *ap,
 		const void *gf, const void *mass, const void *dmat, double lambda,
 		const int *bc, int apply_mask, const int *done, double *part);
@@ -183,10 +187,10 @@ int ssxk_fcg_step(void *stream, size_t n, const double *scal, int ia,
 		const double *s, double *part);
 /* low-precision (FP32 fields) variants; prec selects the factor type of
  * ssxk_ax_lp (SSX_PREC_FP32 or SSX_PREC_BF16) */
-int ssxk_ax_lp(void *stream, int prec, int n, int nel, int op, const void *u,
-		void *au, const void *gf, const void *mass, const void *dmat,
-		double lambda, const int *bc, int apply_mask, const int *done,
-		double *part);
+int ssxk_ax_lp(void *stream, int prec, int n, int v, int nel, int op,
+		const void *u, void *au, const void *gf, const void *mass,
+		const void *dmat, double lambda, const int *bc, int apply_mask,
+		const int *done, double *part);
 int ssxk_dss_all_lp(void *stream, int *const grp[3], const long long ng[3],
 		void *u, const void *p, double *part, int average, const int *done);
 int ssxk_cg_update_lp(void *stream, size_t n, const double *scal, int ia,

@@ -12,11 +12,8 @@
 // Inner products always accumulate in FP64.
 //
 // Kernels:
-//   k_ax<N,OP,T,G>  Poisson / Helmholtz operator, one "team" of N*N lanes per
-//                   element (lane = (i,j) column, loop over the k planes),
-//                   the element and the two in-plane fluxes staged in LDS,
-//                   the zeta contraction kept in registers.  HBM-bound:
-//                   FP64 64 B per local node (Poisson), 72 B (Helmholtz).
+//   k_ax            the element operator: template in ssx_ax.h, launchers
+//                   here (one instantiation file per precision)
 //   k_dss_all<T>    direct stiffness summation over shared-node groups of
 //                   M = 2, 4, 8 copies; each copy becomes own + partners in
 //                   the reference's add order (bit-identical result).
@@ -52,302 +49,74 @@ static inline dim3 vgrid(size_t n)
 			q += (size_t) gridDim.x * blockDim.x)
 
 // ---------------------------------------------------------------------
-// Operator
+// Operator launchers (kernels in ssx_ax.h, one file per precision)
 // ---------------------------------------------------------------------
 
-template <int N>
-struct AxShape {
-	static constexpr int N2 = N * N;
-	static constexpr int N3 = N * N * N;
-	static constexpr int P = N + 1;   // padded LDS row: conflict-free rows
-	static constexpr int TEAMS = (256 / N2) > 0 ? (256 / N2) : 1;
-	static constexpr int BLOCK = TEAMS * N2;
-};
+#include "ssx_ax.h"
 
-// PUPD: the CG direction update p = (z - c) + beta p (masked; the
-// k_cg_pupdate arithmetic) is done while the element is loaded, p is
-// written back and the operator is applied to the new p.  u is not read.
-template <int N, int OP, typename T, typename G, bool PUPD>
-__global__ void __launch_bounds__(AxShape<N>::BLOCK)
-k_ax(int nel, const T *__restrict__ u, T *__restrict__ au,
-		const G *__restrict__ gf, const T *__restrict__ mass,
-		const T *__restrict__ dmat, T lambda,
-		const int *__restrict__ bc, int apply_mask,
-		const int *__restrict__ done, double *__restrict__ part,
-		const T *__restrict__ zin, T *__restrict__ pio,
-		const double *__restrict__ scal, int ib, int ic)
-{
-	using S = AxShape<N>;
-	constexpr int N2 = S::N2, N3 = S::N3, P = S::P, TEAMS = S::TEAMS;
-	if (done && *done)          // solver already converged: no-op
-		return;
-	__shared__ T s_d[2 * N2];
-	__shared__ T s_u[TEAMS][N * N * P];
-	__shared__ T s_a[TEAMS][N * P];
-	__shared__ T s_b[TEAMS][N * P];
-
-	const int team = threadIdx.x / N2;
-	const int tid = threadIdx.x - team * N2;
-	const int i = tid % N, j = tid / N;
-	const int e = blockIdx.x * TEAMS + team;
-	const bool act = e < nel;
-
-	for (int q = threadIdx.x; q < 2 * N2; q += blockDim.x)
-		s_d[q] = dmat[q];
-	// D_il at [i*N+l]; the SVV matrix enters the xi-forward and the
-	// eta/zeta-transpose products only (helmholtz.c:41-44, deriv.c:29-66)
-	const T *gDn = dmat;
-	const T *gDs = dmat + (OP == 1 ? N2 : 0);
-
-	const size_t eoff = (size_t) (act ? e : 0) * N3;
-	T *su = s_u[team], *sa = s_a[team], *sb = s_b[team];
-	// bc[e]: bits 0-5 Dirichlet faces, bits 8-13 faces with a neighbour
-	const int eb = act ? bc[e] : 0;
-	const int bits = apply_mask ? eb : 0;
-	const bool mi = ((bits & 1) && i == 0) || ((bits & 2) && i == N - 1)
-		|| ((bits & 4) && j == 0) || ((bits & 8) && j == N - 1);
-	T uc[N], acc[N];
-	if (PUPD) {
-		const T bb = (T) scal[ib];
-		const T cc = ic >= 0 ? (T) -scal[ic] : T(0);
-#pragma unroll
-		for (int k = 0; k < N; ++k) {
-			T v = T(0);
-			if (act) {
-				const size_t q = eoff + k * N2 + tid;
-				const T zv = ic >= 0 ? cc * T(1) + zin[q] : zin[q];
-				v = bb * pio[q] + zv;
-				const bool m = mi || ((bits & 16) && k == 0)
-					|| ((bits & 32) && k == N - 1);
-				if (apply_mask)
-					v = v * (m ? T(0) : T(1));
-				pio[q] = v;
-			}
-			uc[k] = v;
-			su[(k * N + j) * P + i] = v;
-			acc[k] = T(0);
-		}
-	} else {
-#pragma unroll
-		for (int k = 0; k < N; ++k) {
-			uc[k] = act ? u[eoff + k * N2 + tid] : T(0);
-			su[(k * N + j) * P + i] = uc[k];
-			acc[k] = T(0);
-		}
-	}
-	__syncthreads();
-	const T *Dn = s_d;
-	const T *Ds = s_d + (OP == 1 ? N2 : 0);
-	T dxf[N], dyf[N], dxt[N], dyt[N];
-#pragma unroll
-	for (int l = 0; l < N; ++l) {
-		dxf[l] = Ds[i * N + l];
-		dyf[l] = Dn[j * N + l];
-		dxt[l] = Dn[l * N + i];
-		dyt[l] = Ds[l * N + j];
-	}
-	const G *g = gf + (size_t) (act ? e : 0) * 6 * N3;
-
-#pragma unroll 1
-	for (int k = 0; k < N; ++k) {
-		T gx = T(0), gy = T(0), gz = T(0);
-		const T *plane = su + k * N * P;
-#pragma unroll
-		for (int l = 0; l < N; ++l) {
-			gx += dxf[l] * plane[j * P + l];
-			gy += dyf[l] * plane[l * P + i];
-			gz += gDn[k * N + l] * uc[l];      // uniform: scalar loads
-		}
-		T t0 = T(0), t1 = T(0), t2 = T(0);
-		if (act) {
-			const int q = k * N2 + tid;
-			const T gxx = ldg(g, q), gxy = ldg(g, N3 + q);
-			const T gxz = ldg(g, 2 * N3 + q), gyy = ldg(g, 3 * N3 + q);
-			const T gyz = ldg(g, 4 * N3 + q), gzz = ldg(g, 5 * N3 + q);
-			t0 = gxx * gx + gxy * gy + gxz * gz;
-			t1 = gxy * gx + gyy * gy + gyz * gz;
-			t2 = gxz * gx + gyz * gy + gzz * gz;
-		}
-		sa[j * P + i] = t0;
-		sb[j * P + i] = t1;
-		__syncthreads();
-		T r = T(0);
-#pragma unroll
-		for (int l = 0; l < N; ++l)
-			r += dxt[l] * sa[j * P + l] + dyt[l] * sb[l * P + i];
-#pragma unroll
-		for (int kk = 0; kk < N; ++kk)
-			acc[kk] += (kk == k ? r : T(0)) + gDs[k * N + kk] * t2;
-		__syncthreads();
-	}
-	const bool si = ((eb & 256) && i == 0) || ((eb & 512) && i == N - 1)
-		|| ((eb & 1024) && j == 0) || ((eb & 2048) && j == N - 1);
-	double pd = 0.0;
-	if (act) {
-#pragma unroll
-		for (int k = 0; k < N; ++k) {
-			const int q = k * N2 + tid;
-			T v = acc[k];
-			if (OP == 1)
-				v += lambda * (mass[eoff + q] * uc[k]);
-			const bool m = mi || ((bits & 16) && k == 0)
-				|| ((bits & 32) && k == N - 1);
-			v = m ? T(0) : v;
-			au[eoff + q] = v;
-			// <p, Ap>_w over nodes no other element shares (weight 1);
-			// the shared ones are added by the DSS kernel after assembly
-			const bool sh = si || ((eb & 4096) && k == 0)
-				|| ((eb & 8192) && k == N - 1);
-			if (!sh)
-				pd += (double) uc[k] * (double) v;
-		}
-	}
-	if (part) {
-		__shared__ double s_red[256];
-		for (int q = threadIdx.x; q < 256; q += blockDim.x)
-			s_red[q] = 0.0;
-		__syncthreads();
-		s_red[threadIdx.x] = pd;
-		__syncthreads();
-#pragma unroll
-		for (int st = 128; st > 0; st >>= 1) {
-			if ((int) threadIdx.x < st)
-				s_red[threadIdx.x] += s_red[threadIdx.x + st];
-			__syncthreads();
-		}
-		if (threadIdx.x == 0)
-			part[blockIdx.x] = s_red[0];
-	}
-}
-
-// the p update of a fused CG iteration (PUPD): z, p (read and written),
-// beta at scal[ib], null-space constant at scal[ic] (ic < 0: none)
-struct PUpd {
-	const void *z;
-	void *p;
-	const double *scal;
-	int ib, ic;
-};
-
-template <int N, typename T, typename G, bool PU>
-static void launch_ax1(hipStream_t st, int grid, int nel, int op,
-		const void *u, void *au, const void *gf, const void *mass,
+extern "C" int ssxk_ax_dispatch_f64(void *stream, int n, int v, int nel,
+		int op, const void *u, void *au, const void *gf, const void *mass,
 		const void *dmat, double lambda, const int *bc, int apply_mask,
-		const int *done, double *part, const PUpd &pu)
-{
-	using S = AxShape<N>;
-	if (op == SSX_OP_HELMHOLTZ)
-		hipLaunchKernelGGL((k_ax<N, 1, T, G, PU>), dim3(grid), dim3(S::BLOCK),
-				0, st, nel, (const T *) u, (T *) au, (const G *) gf,
-				(const T *) mass, (const T *) dmat, (T) lambda, bc,
-				apply_mask, done, part, (const T *) pu.z, (T *) pu.p, pu.scal,
-				pu.ib, pu.ic);
-	else
-		hipLaunchKernelGGL((k_ax<N, 0, T, G, PU>), dim3(grid), dim3(S::BLOCK),
-				0, st, nel, (const T *) u, (T *) au, (const G *) gf,
-				(const T *) mass, (const T *) dmat, (T) lambda, bc,
-				apply_mask, done, part, (const T *) pu.z, (T *) pu.p, pu.scal,
-				pu.ib, pu.ic);
-}
+		const int *done, double *part, const void *pu);
+extern "C" int ssxk_ax_dispatch_f32(void *stream, int n, int v, int nel,
+		int op, const void *u, void *au, const void *gf, const void *mass,
+		const void *dmat, double lambda, const int *bc, int apply_mask,
+		const int *done, double *part, const void *pu);
+extern "C" int ssxk_ax_dispatch_bf16(void *stream, int n, int v, int nel,
+		int op, const void *u, void *au, const void *gf, const void *mass,
+		const void *dmat, double lambda, const int *bc, int apply_mask,
+		const int *done, double *part, const void *pu);
 
-template <int N, typename T, typename G>
-static int launch_ax(hipStream_t st, int nel, int op, const void *u,
-		void *au, const void *gf, const void *mass, const void *dmat,
-		double lambda, const int *bc, int apply_mask, const int *done,
-		double *part, const PUpd *pu)
+extern "C" int ssxk_ax_blocks(int n, int nel, int v)
 {
-	using S = AxShape<N>;
-	const int grid = (nel + S::TEAMS - 1) / S::TEAMS;
-	if (pu)
-		launch_ax1<N, T, G, true>(st, grid, nel, op, u, au, gf, mass, dmat,
-				lambda, bc, apply_mask, done, part, *pu);
-	else
-		launch_ax1<N, T, G, false>(st, grid, nel, op, u, au, gf, mass, dmat,
-				lambda, bc, apply_mask, done, part, PUpd{ nullptr, nullptr,
-				nullptr, 0, -1 });
-	LAUNCH_CHECK("k_ax");
-	return SSX_OK;
-}
-
-template <typename T, typename G>
-static int dispatch_ax(hipStream_t st, int n, int nel, int op, const void *u,
-		void *au, const void *gf, const void *mass, const void *dmat,
-		double lambda, const int *bc, int apply_mask, const int *done,
-		double *part, const PUpd *pu = nullptr)
-{
-	switch (n) {
-#define CASE(NN) case NN: return launch_ax<NN, T, G>(st, nel, op, u, au, gf, \
-			mass, dmat, lambda, bc, apply_mask, done, part, pu);
-	CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9)
-	CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
-#undef CASE
-	default:
-		ssx_set_error("ssxk_ax: n=%d unsupported", n);
-		return SSX_EINVAL;
-	}
-}
-
-extern "C" int ssxk_ax_blocks(int n, int nel)
-{
-	const int teams = (256 / (n * n)) > 0 ? 256 / (n * n) : 1;
+	const int teams = ssx_ax_teams(n, v);
 	return (nel + teams - 1) / teams;
 }
 
-extern "C" int ssxk_ax(void *stream, int n, int nel, int op, const double *u,
-		double *au, const double *gf, const double *mass,
+extern "C" int ssxk_ax(void *stream, int n, int v, int nel, int op,
+		const double *u, double *au, const double *gf, const double *mass,
 		const double *dmat, double lambda, const int *bc, int apply_mask,
 		const int *done, double *part)
 {
-	if (nel <= 0)
-		return SSX_OK;
-	return dispatch_ax<double, double>((hipStream_t) stream, n, nel, op, u,
-			au, gf, mass, dmat, lambda, bc, apply_mask, done, part);
+	return ssxk_ax_dispatch_f64(stream, n, v, nel, op, u, au, gf, mass, dmat,
+			lambda, bc, apply_mask, done, part, nullptr);
 }
 
-extern "C" int ssxk_ax_pupd(void *stream, int n, int nel, int op,
+extern "C" int ssxk_ax_pupd(void *stream, int n, int v, int nel, int op,
 		const double *z, double *p, const double *scal, int ib, int ic,
 		double *ap, const double *gf, const double *mass, const double *dmat,
 		double lambda, const int *bc, int apply_mask, const int *done,
 		double *part)
 {
-	if (nel <= 0)
-		return SSX_OK;
 	const PUpd pu = { z, p, scal, ib, ic };
-	return dispatch_ax<double, double>((hipStream_t) stream, n, nel, op,
-			nullptr, ap, gf, mass, dmat, lambda, bc, apply_mask, done, part,
-			&pu);
+	return ssxk_ax_dispatch_f64(stream, n, v, nel, op, nullptr, ap, gf, mass,
+			dmat, lambda, bc, apply_mask, done, part, &pu);
 }
 
-extern "C" int ssxk_ax_pupd_lp(void *stream, int prec, int n, int nel,
+extern "C" int ssxk_ax_lp(void *stream, int prec, int n, int v, int nel,
+		int op, const void *u, void *au, const void *gf, const void *mass,
+		const void *dmat, double lambda, const int *bc, int apply_mask,
+		const int *done, double *part)
+{
+	if (prec == SSX_PREC_BF16)
+		return ssxk_ax_dispatch_bf16(stream, n, v, nel, op, u, au, gf, mass,
+				dmat, lambda, bc, apply_mask, done, part, nullptr);
+	return ssxk_ax_dispatch_f32(stream, n, v, nel, op, u, au, gf, mass, dmat,
+			lambda, bc, apply_mask, done, part, nullptr);
+}
+
+extern "C" int ssxk_ax_pupd_lp(void *stream, int prec, int n, int v, int nel,
 		int op, const void *z, void *p, const double *scal, int ib, int ic,
 		void *ap, const void *gf, const void *mass, const void *dmat,
 		double lambda, const int *bc, int apply_mask, const int *done,
 		double *part)
 {
-	if (nel <= 0)
-		return SSX_OK;
 	const PUpd pu = { z, p, scal, ib, ic };
 	if (prec == SSX_PREC_BF16)
-		return dispatch_ax<float, bf16_t>((hipStream_t) stream, n, nel, op,
-				nullptr, ap, gf, mass, dmat, lambda, bc, apply_mask, done,
-				part, &pu);
-	return dispatch_ax<float, float>((hipStream_t) stream, n, nel, op,
-			nullptr, ap, gf, mass, dmat, lambda, bc, apply_mask, done, part,
-			&pu);
-}
-
-extern "C" int ssxk_ax_lp(void *stream, int prec, int n, int nel, int op,
-		const void *u, void *au, const void *gf, const void *mass,
-		const void *dmat, double lambda, const int *bc, int apply_mask,
-		const int *done, double *part)
-{
-	if (nel <= 0)
-		return SSX_OK;
-	if (prec == SSX_PREC_BF16)
-		return dispatch_ax<float, bf16_t>((hipStream_t) stream, n, nel, op, u,
-				au, gf, mass, dmat, lambda, bc, apply_mask, done, part);
-	return dispatch_ax<float, float>((hipStream_t) stream, n, nel, op, u, au,
-			gf, mass, dmat, lambda, bc, apply_mask, done, part);
+		return ssxk_ax_dispatch_bf16(stream, n, v, nel, op, nullptr, ap, gf,
+				mass, dmat, lambda, bc, apply_mask, done, part, &pu);
+	return ssxk_ax_dispatch_f32(stream, n, v, nel, op, nullptr, ap, gf, mass,
+			dmat, lambda, bc, apply_mask, done, part, &pu);
 }
 
 // ---------------------------------------------------------------------

@@ -95,6 +95,19 @@ class Context:
     def set_option(self, option, value):
         check(self._lib.ssx_set_option(self.h, option, int(value)))
 
+    # --- operator launch shapes ---
+    def autotune(self, prec_mask=1):
+        check(self._lib.ssx_autotune(self.h, prec_mask))
+
+    def ax_variant(self, level, prec=0):
+        """(variant in use, measured us per launch of each variant)"""
+        us = np.zeros(4)
+        v = self._lib.ssx_ax_variant(self.h, level, prec, _dp(us))
+        return v, us
+
+    def set_ax_variant(self, level, prec, v):
+        check(self._lib.ssx_set_ax_variant(self.h, level, prec, v))
+
     # --- operators (torch device tensors) ---
     def ax(self, level, op, u, au):
         check(self._lib.ssx_ax(self.h, level, op, _ptr(u), _ptr(au)))

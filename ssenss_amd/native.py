@@ -10,6 +10,7 @@ PC_JACOBI, PC_MG = 0, 1
 OPT_FUSED_CG = 1
 OPT_SMOOTHER_PRECISION = 2
 OPT_PERSISTENT_CG = 3
+AX_NVARIANTS = 4
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 (DATA_GFAC, DATA_MASS, DATA_DIAG_P, DATA_DIAG_H, DATA_SCALE, DATA_MASK,
  DATA_INVMASS, DATA_X, DATA_Y, DATA_Z) = range(10)
@@ -30,7 +31,8 @@ EXPORTED_SYMBOLS = [
     "ssx_fcg", "ssx_mg_precon", "ssx_solve_poisson_cg",
     "ssx_solve_helmholtz_cg", "ssx_particles_set_velocity",
     "ssx_particles_step", "ssx_particles_set_velocity_host",
-    "ssx_particles_step_host",
+    "ssx_particles_step_host", "ssx_autotune", "ssx_ax_variant",
+    "ssx_set_ax_variant",
 ]
 
 
@@ -129,6 +131,9 @@ def load():
         "ssx_solve_helmholtz_cg": (i, [vp, dp, dp, i, d, C.POINTER(Stats)]),
         "ssx_particles_set_velocity": (i, [vp, vp, vp, vp]),
         "ssx_particles_set_velocity_host": (i, [vp, dp, dp, dp]),
+        "ssx_autotune": (i, [vp, i]),
+        "ssx_ax_variant": (i, [vp, i, i, dp]),
+        "ssx_set_ax_variant": (i, [vp, i, i, i]),
         "ssx_particles_step_host": (i, [vp, C.POINTER(ParticlesStruct), d, d,
                                         C.POINTER(C.c_int)]),
         "ssx_particles_step": (i, [vp, C.POINTER(ParticlesStruct), d, d,

@@ -521,3 +521,44 @@ def test_particles_host_entry_matches_device_entry():
         p[k] for k in ("x", "y", "z", "u", "v", "w", "t")], 1)
     assert pass_h == pass_d
     assert np.array_equal(out_h, out_d) and np.array_equal(p["reinit"], re_d)
+
+
+@pytest.mark.parametrize("name", ["box2_n5", "cube2_n8", "cube2_n12"])
+def test_operator_launch_variants_bit_identical(name):
+    """every launch shape of the element operator (workgroup size, factor
+    prefetch) gives bit-identical results, in FP64 against the reference's
+    operator to 1e-12; the autotuner's choice is one of them"""
+    Z = golden(name)
+    with make_ctx(golden_mesh(Z)) as ctx:
+        v0, us = ctx.ax_variant(0, 0)
+        print(f"[gpu] {name} autotuned variant {v0}, us per launch "
+              f"{np.round(us, 1).tolist()}")
+        assert 0 <= v0 < S.AX_NVARIANTS and us.min() > 0
+        u = dev(Z["in_u"])
+        outs = {}
+        for op in (S.OP_POISSON, S.OP_HELMHOLTZ):
+            for v in range(S.AX_NVARIANTS):
+                ctx.set_ax_variant(0, 0, v)
+                au = torch.zeros_like(u)
+                ctx.ax(0, op, u, au)
+                outs[(op, v)] = host(au)
+            for v in range(1, S.AX_NVARIANTS):
+                assert np.array_equal(outs[(op, v)], outs[(op, 0)]), (op, v)
+        assert rel(outs[(S.OP_POISSON, 0)], Z["ref_ax_poisson"]) <= 1e-12
+        assert rel(outs[(S.OP_HELMHOLTZ, 0)], Z["ref_ax_helmholtz"]) <= 1e-12
+        # the reduced-precision copies are tuned when first built; forcing
+        # each shape leaves the mixed solve's iterations unchanged
+        ctx.set_option(S.OPT_SMOOTHER_PRECISION, S.PREC_FP32)
+        ctx.set_diag_choice(0)
+        its = []
+        for v in range(S.AX_NVARIANTS):
+            for l in range(ctx.nlevels - 1):
+                ctx.set_ax_variant(l, S.PREC_FP32, v)
+                ctx.set_ax_variant(l, S.PREC_FP64, v)
+            f = dev(Z["in_f"])
+            x = torch.zeros_like(f)
+            st = ctx.fcg(S.OP_POISSON, S.X_VELOCITY, S.PC_MG, f, x,
+                         Z["ref_scalars"][0], 500)
+            its.append((st.iterations, host(x)))
+        assert all(i == its[0][0] for i, _ in its)
+        assert all(np.array_equal(x, its[0][1]) for _, x in its)
