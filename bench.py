@@ -58,6 +58,57 @@ def smooth_source(x, y, z, seed=2024):
     return f
 
 
+def solve_case(S, torch, ctx, mesh, max_iter, precs=("fp64", "fp32", "bf16"),
+               log_prefix=""):
+    """FCG + p-MG Dirichlet Poisson solve of the seeded Fourier source to
+    1e-10 relative (multiplicity-weighted norm), once per smoother
+    precision; the FP64 solution is the reference for the others."""
+    f = smooth_source(mesh.x.ravel(), mesh.y.ravel(), mesh.z.ravel())
+    massv = torch.from_numpy(ctx.level_data(0, S.DATA_MASS)).cuda()
+    b0 = torch.from_numpy(f).cuda() * massv
+    ctx.dss(0, b0)
+    ctx.mask(0, S.X_VELOCITY, b0)
+    bnorm = np.sqrt(ctx.dot(0, b0, b0))
+    eps = 1e-10 * bnorm
+    ctx.set_diag_choice(0)       # Dirichlet Poisson: Poisson diagonal
+    code = {"fp64": S.PREC_FP64, "fp32": S.PREC_FP32, "bf16": S.PREC_BF16}
+    ref_sol, out = None, {}
+    for name in precs:
+        prec = code[name]
+        ctx.set_option(S.OPT_SMOOTHER_PRECISION, prec)
+        if prec != S.PREC_FP64:   # build (and tune) the reduced copies
+            w = torch.zeros_like(b0)
+            ctx.mg_precon(S.OP_POISSON, S.X_VELOCITY, eps, b0.clone(), w)
+        b = b0.clone()
+        sol = torch.zeros_like(b)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = ctx.fcg(S.OP_POISSON, S.X_VELOCITY, S.PC_MG, b, sol, eps,
+                     max_iter, allow_notconv=True)
+        torch.cuda.synchronize()
+        tsolve = time.perf_counter() - t0
+        rec = {"time_to_residual_s": tsolve, "iterations": st.iterations,
+               "converged": bool(st.converged),
+               "final_residual_rel": st.residual / bnorm,
+               "tolerance_rel": 1e-10, "smoother_precision": name,
+               "outer_precision": "fp64", "fine_ax": st.fine_ax,
+               "coarse_cg_iters": st.coarse_iters,
+               "smoother_cg_iters": st.smooth_iters, "vcycles": st.vcycles}
+        if ref_sol is None:
+            ref_sol = sol
+        else:
+            d = sol - ref_sol
+            rec["error_vs_fp64_rel"] = float(np.sqrt(
+                ctx.dot(0, d, d) / ctx.dot(0, ref_sol, ref_sol)))
+            rec["speedup_vs_fp64"] = (out["fp64"]["time_to_residual_s"]
+                                      / tsolve)
+        out[name] = rec
+        log(f"[bench] {log_prefix}solve {name} {tsolve:.3f} s, "
+            f"{st.iterations} its")
+    ctx.set_option(S.OPT_SMOOTHER_PRECISION, S.PREC_FP64)
+    return out
+
+
 def cpu_baseline(mesh, seconds):
     """The oracle (C restatement of the reference CPU path) on a bounded
     sample: Poisson Ax on a 12^3-element piece of the same deformed cube
@@ -96,6 +147,8 @@ def main():
     ap.add_argument("--max-iter", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the N = 5 / 7 / 11 mixed-precision sweep")
     ap.add_argument("--profile", action="store_true",
                     help="kernels only, for rocprofv3 counter runs: the "
                     "Poisson Ax on the workload, then a 1 GiB device copy "
@@ -162,55 +215,36 @@ def main():
     roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
                 "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                 "traffic": read_traffic(),
-                "kernel": f"k_ax<{args.n},0> (FP64 Poisson Ax)",
+                "kernel": f"k_ax<{args.n},0> (FP64 Poisson Ax, launch "
+                          f"variant {ctx.ax_variant(0, 0)[0]})",
                 "bytes_per_launch": BYTES_PER_NODE_POISSON * nt}
 
     solves = {}
     if not args.no_solve:
-        f = smooth_source(mesh.x.ravel(), mesh.y.ravel(), mesh.z.ravel())
-        massv = torch.from_numpy(ctx.level_data(0, S.DATA_MASS)).cuda()
-        b0 = torch.from_numpy(f).cuda() * massv
-        ctx.dss(0, b0)
-        ctx.mask(0, S.X_VELOCITY, b0)
-        bnorm = np.sqrt(ctx.dot(0, b0, b0))
-        eps = 1e-10 * bnorm
-        ctx.set_diag_choice(0)       # Dirichlet Poisson: Poisson diagonal
-        ref_sol = None
-        for name, prec in (("fp64", S.PREC_FP64), ("fp32", S.PREC_FP32),
-                           ("bf16", S.PREC_BF16)):
-            ctx.set_option(S.OPT_SMOOTHER_PRECISION, prec)
-            if prec != S.PREC_FP64:   # build the reduced-precision copies
-                w = torch.zeros_like(b0)
-                ctx.mg_precon(S.OP_POISSON, S.X_VELOCITY, eps, b0.clone(), w)
-            b = b0.clone()
-            sol = torch.zeros_like(b)
-            torch.cuda.synchronize()
-            log(f"[bench] FCG + p-MG solve, smoother {name}")
-            t0 = time.perf_counter()
-            st = ctx.fcg(S.OP_POISSON, S.X_VELOCITY, S.PC_MG, b, sol, eps,
-                         args.max_iter, allow_notconv=True)
-            torch.cuda.synchronize()
-            tsolve = time.perf_counter() - t0
-            rec = {"time_to_residual_s": tsolve, "iterations": st.iterations,
-                   "converged": bool(st.converged),
-                   "final_residual_rel": st.residual / bnorm,
-                   "tolerance_rel": 1e-10,
-                   "smoother_precision": name,
-                   "outer_precision": "fp64",
-                   "fine_ax": st.fine_ax, "coarse_cg_iters": st.coarse_iters,
-                   "smoother_cg_iters": st.smooth_iters,
-                   "vcycles": st.vcycles}
-            if ref_sol is None:
-                ref_sol = sol
-            else:
-                d = sol - ref_sol
-                rec["error_vs_fp64_rel"] = float(np.sqrt(
-                    ctx.dot(0, d, d) / ctx.dot(0, ref_sol, ref_sol)))
-                rec["speedup_vs_fp64"] = (solves["fp64"]["time_to_residual_s"]
-                                          / tsolve)
-            solves[name] = rec
-            log(f"[bench] solve {name} {tsolve:.3f} s, {st.iterations} its")
-        ctx.set_option(S.OPT_SMOOTHER_PRECISION, S.PREC_FP64)
+        solves = solve_case(S, torch, ctx, mesh, args.max_iter)
+    mg_orders = [ctx.level_n(l) - 1 for l in range(ctx.nlevels)]
+    variants = {}
+    for l in range(ctx.nlevels):
+        v, us = ctx.ax_variant(l, S.PREC_FP64)
+        variants[f"level{l}_n{ctx.level_n(l)}"] = {
+            "variant": v, "us_per_variant": [round(x, 2) for x in us]}
+    ctx.close()
+
+    sweep = {}
+    if not args.no_sweep:
+        # the same problem at N = 5, 7, 11 on 48^3, 32^3, 20^3 elements
+        for n, me in ((6, 48), (8, 32), (12, 20)):
+            if n == args.n and me == m and solves:
+                sweep[f"N{n-1}_{me}^3"] = solves
+                continue
+            sm = box_mesh(n, (me, me, me), deform=1, amp=0.1)
+            sctx = S.Context(sm)
+            sctx.set_stream(stream)
+            sweep[f"N{n-1}_{me}^3"] = dict(
+                local_dofs=sm.ntot,
+                **solve_case(S, torch, sctx, sm, args.max_iter,
+                             log_prefix=f"N={n-1} {me}^3 "))
+            sctx.close()
 
     cpu = None if args.no_cpu else cpu_baseline(mesh, args.cpu_seconds)
     out = {
@@ -227,8 +261,7 @@ def main():
                                "Poisson, FP64 operator + FCG/p-MG solve",
                    "elements": m ** 3, "points_per_dir": args.n,
                    "local_dofs": nt, "unique_dofs": (m * (args.n - 1) + 1) ** 3,
-                   "mg_orders": [ctx.level_n(l) - 1 for l in
-                                 range(ctx.nlevels)]},
+                   "mg_orders": mg_orders},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "helmholtz_ax_gdofs": nt / t_helm / 1e9,
@@ -237,8 +270,9 @@ def main():
         "solve_fp64": solves.get("fp64"),
         "solve_mixed_fp32": solves.get("fp32"),
         "solve_mixed_bf16": solves.get("bf16"),
+        "ax_launch_variants": variants,
+        "mixed_precision_sweep": sweep or None,
     }
-    ctx.close()
     print(json.dumps(out), flush=True)
 
 

@@ -242,17 +242,40 @@ int ssx_particles_set_velocity_host(ssx_ctx *ctx, const double *u,
 int ssx_particles_step_host(ssx_ctx *ctx, const ssx_particles *p,
 		double t_now, double dt, int *max_passes);
 
+/* ---- batches of independent problems ----
+ * Jacobi-PCG (conjugate_gradient, solver.c:33-81) on level 0 of `count`
+ * contexts at once: problem i applies op[i] for variable var[i] of ctx[i]
+ * (any element counts and orders; a context may appear more than once, e.g.
+ * for three velocity components).  Each CG stage is one launch for the
+ * whole batch (the operator: one per distinct order); with use_graph the
+ * iterations replay a captured HIP graph.  The batch owns its work vectors;
+ * the contexts must outlive it.  It runs on a stream of its own unless
+ * ssx_batch_set_stream names another (graph replay needs a non-null
+ * stream).  ssx_batch_cg: f[i], u[i] device E-vectors of problem i (f
+ * masked in place, u initial guess in / solution out), stopping test
+ * sqrt(<r,r>_w) <= eps[i] or max_iter; stats (nullable) gets `count`
+ * entries.  Returns SSX_ENOTCONV if any problem stopped above its eps. */
+typedef struct ssx_batch ssx_batch;
+int ssx_batch_create(ssx_batch **batch, ssx_ctx *const *ctx, const int *op,
+		const int *var, int count);
+void ssx_batch_destroy(ssx_batch *batch);
+int ssx_batch_set_stream(ssx_batch *batch, void *hip_stream);
+int ssx_batch_cg(ssx_batch *batch, double *const *f, double *const *u,
+		const double *eps, int max_iter, int use_graph, ssx_stats *stats);
+
 /* ---- operator launch autotuning ----
  * The element operator has SSX_AX_NVARIANTS launch shapes (elements per
- * workgroup; prefetch of the next plane's geometric factors).  ssx_create
+ * workgroup; non-temporal streaming of factors and result; derivative rows
+ * in registers or LDS).  ssx_create
  * times every shape on every level in FP64 and keeps the fastest; the
  * reduced-precision copies are tuned when first built.  prec_mask: bit p
  * for precision SSX_PREC_p.  Results are bit-identical across shapes.
  * ssx_ax_variant returns the shape in use (us, nullable: the measured
  * microseconds per launch of each shape); ssx_set_ax_variant forces one. */
-#define SSX_AX_NVARIANTS 4
+#define SSX_AX_NVARIANTS 8
 int ssx_autotune(ssx_ctx *ctx, int prec_mask);
-int ssx_ax_variant(const ssx_ctx *ctx, int level, int prec, double us[4]);
+int ssx_ax_variant(const ssx_ctx *ctx, int level, int prec,
+		double us[SSX_AX_NVARIANTS]);
 int ssx_set_ax_variant(ssx_ctx *ctx, int level, int prec, int variant);
 
 /* ---- reference entry points on HOST E-vectors: f (masked in place) and u

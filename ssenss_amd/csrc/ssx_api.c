@@ -1778,7 +1778,8 @@ int ssx_autotune(ssx_ctx *c, int prec_mask)
 	return SSX_OK;
 }
 
-int ssx_ax_variant(const ssx_ctx *c, int l, int prec, double us[4])
+int ssx_ax_variant(const ssx_ctx *c, int l, int prec,
+		double us[SSX_AX_NVARIANTS])
 {
 	if (!c || l < 0 || l >= c->nlev || prec < 0 || prec > 2)
 		return -1;

@@ -13,8 +13,12 @@
 // Launch variants (chosen per order by the autotuner in ssx_api.c):
 //   bit 0  one-wave teams: max(1, 64/N^2) elements per workgroup instead of
 //          max(1, 256/N^2), so the plane barriers span one wavefront
-//   bit 1  the six factors of plane k+1 are loaded while plane k is
-//          computed (more bytes in flight per lane, more registers)
+//   bit 1  non-temporal streaming: the factors (read once per application)
+//          are loaded and Au stored with the non-temporal hint, so they do
+//          not displace the element data in the caches
+//   bit 2  low-register: the transposed derivative rows are read from LDS
+//          in the plane loop instead of being held in registers (2N fewer
+//          registers per lane, more waves per SIMD)
 #ifndef SSX_AX_H
 #define SSX_AX_H
 #include <hip/hip_runtime.h>
@@ -51,21 +55,20 @@ struct PUpd {
 // PUPD: the update is done while the element is loaded, p is written back
 // and the operator is applied to the new p (the k_cg_pupdate arithmetic);
 // u is not read.
+// One workgroup's elements [eblk*TEAMS, eblk*TEAMS + TEAMS): Au (masked)
+// written, and each lane's part of <p,Ap> over unshared nodes returned.
 template <int N, int OP, typename T, typename G, bool PUPD, int V>
-__global__ void __launch_bounds__((AxShape<N, V>::BLOCK))
-k_ax(int nel, const T *__restrict__ u, T *__restrict__ au,
+__device__ __forceinline__ double ax_team(int eblk, int nel,
+		const T *__restrict__ u, T *__restrict__ au,
 		const G *__restrict__ gf, const T *__restrict__ mass,
 		const T *__restrict__ dmat, T lambda,
 		const int *__restrict__ bc, int apply_mask,
-		const int *__restrict__ done, double *__restrict__ part,
 		const T *__restrict__ zin, T *__restrict__ pio,
 		const double *__restrict__ scal, int ib, int ic)
 {
 	using S = AxShape<N, V>;
 	constexpr int N2 = S::N2, N3 = S::N3, P = S::P, TEAMS = S::TEAMS;
-	constexpr bool PF = (V & 2) != 0;
-	if (done && *done)          // solver already converged: no-op
-		return;
+	constexpr bool NT = (V & 2) != 0, LR = (V & 4) != 0;
 	__shared__ T s_d[2 * N2];
 	__shared__ T s_u[TEAMS][N * N * P];
 	__shared__ T s_a[TEAMS][N * P];
@@ -74,7 +77,7 @@ k_ax(int nel, const T *__restrict__ u, T *__restrict__ au,
 	const int team = threadIdx.x / N2;
 	const int tid = threadIdx.x - team * N2;
 	const int i = tid % N, j = tid / N;
-	const int e = blockIdx.x * TEAMS + team;
+	const int e = eblk * TEAMS + team;
 	const bool act = e < nel;
 
 	for (int q = threadIdx.x; q < 2 * N2; q += blockDim.x)
@@ -121,37 +124,22 @@ k_ax(int nel, const T *__restrict__ u, T *__restrict__ au,
 		}
 	}
 	const G *g = gf + (size_t) (act ? e : 0) * 6 * N3;
-	G gn[6];
-	if (PF && act) {
-#pragma unroll
-		for (int c = 0; c < 6; ++c)
-			gn[c] = g[c * N3 + tid];
-	}
 	__syncthreads();
 	const T *Dn = s_d;
 	const T *Ds = s_d + (OP == 1 ? N2 : 0);
-	T dxf[N], dyf[N], dxt[N], dyt[N];
+	T dxf[N], dyf[N], dxt[LR ? 1 : N], dyt[LR ? 1 : N];
 #pragma unroll
 	for (int l = 0; l < N; ++l) {
 		dxf[l] = Ds[i * N + l];
 		dyf[l] = Dn[j * N + l];
-		dxt[l] = Dn[l * N + i];
-		dyt[l] = Ds[l * N + j];
+		if (!LR) {
+			dxt[l] = Dn[l * N + i];
+			dyt[l] = Ds[l * N + j];
+		}
 	}
 
 #pragma unroll 1
 	for (int k = 0; k < N; ++k) {
-		G gc[6];
-		if (PF) {
-#pragma unroll
-			for (int c = 0; c < 6; ++c)
-				gc[c] = gn[c];
-			if (act && k + 1 < N) {
-#pragma unroll
-				for (int c = 0; c < 6; ++c)
-					gn[c] = g[c * N3 + (k + 1) * N2 + tid];
-			}
-		}
 		T gx = T(0), gy = T(0), gz = T(0);
 		const T *plane = su + k * N * P;
 #pragma unroll
@@ -164,9 +152,13 @@ k_ax(int nel, const T *__restrict__ u, T *__restrict__ au,
 		if (act) {
 			const int q = k * N2 + tid;
 			T gxx, gxy, gxz, gyy, gyz, gzz;
-			if (PF) {
-				gxx = ldv(gc[0]); gxy = ldv(gc[1]); gxz = ldv(gc[2]);
-				gyy = ldv(gc[3]); gyz = ldv(gc[4]); gzz = ldv(gc[5]);
+			if (NT) {
+				gxx = ldv(__builtin_nontemporal_load(g + q));
+				gxy = ldv(__builtin_nontemporal_load(g + N3 + q));
+				gxz = ldv(__builtin_nontemporal_load(g + 2 * N3 + q));
+				gyy = ldv(__builtin_nontemporal_load(g + 3 * N3 + q));
+				gyz = ldv(__builtin_nontemporal_load(g + 4 * N3 + q));
+				gzz = ldv(__builtin_nontemporal_load(g + 5 * N3 + q));
 			} else {
 				gxx = ldg(g, q); gxy = ldg(g, N3 + q);
 				gxz = ldg(g, 2 * N3 + q); gyy = ldg(g, 3 * N3 + q);
@@ -181,8 +173,11 @@ k_ax(int nel, const T *__restrict__ u, T *__restrict__ au,
 		__syncthreads();
 		T r = T(0);
 #pragma unroll
-		for (int l = 0; l < N; ++l)
-			r += dxt[l] * sa[j * P + l] + dyt[l] * sb[l * P + i];
+		for (int l = 0; l < N; ++l) {
+			const T xt = LR ? Dn[l * N + i] : dxt[l];
+			const T yt = LR ? Ds[l * N + j] : dyt[l];
+			r += xt * sa[j * P + l] + yt * sb[l * P + i];
+		}
 #pragma unroll
 		for (int kk = 0; kk < N; ++kk)
 			acc[kk] += (kk == k ? r : T(0)) + gDs[k * N + kk] * t2;
@@ -201,7 +196,10 @@ k_ax(int nel, const T *__restrict__ u, T *__restrict__ au,
 			const bool m = mi || ((bits & 16) && k == 0)
 				|| ((bits & 32) && k == N - 1);
 			v = m ? T(0) : v;
-			au[eoff + q] = v;
+			if (NT)
+				__builtin_nontemporal_store(v, au + eoff + q);
+			else
+				au[eoff + q] = v;
 			// <p, Ap>_w over nodes no other element shares (weight 1);
 			// the shared ones are added by the DSS kernel after assembly
 			const bool sh = si || ((eb & 4096) && k == 0)
@@ -210,22 +208,45 @@ k_ax(int nel, const T *__restrict__ u, T *__restrict__ au,
 				pd += (double) uc[k] * (double) v;
 		}
 	}
-	if (part) {
-		__shared__ double s_red[256];
-		for (int q = threadIdx.x; q < 256; q += blockDim.x)
-			s_red[q] = 0.0;
-		__syncthreads();
-		s_red[threadIdx.x] = pd;
-		__syncthreads();
+	return pd;
+}
+
+// block sum of one value per lane (any block size <= 256), to part[slot]
+__device__ __forceinline__ void ax_block_partial(double pd, double *part,
+		long long slot)
+{
+	__shared__ double s_red[256];
+	for (int q = threadIdx.x; q < 256; q += blockDim.x)
+		s_red[q] = 0.0;
+	__syncthreads();
+	s_red[threadIdx.x] = pd;
+	__syncthreads();
 #pragma unroll
-		for (int st = 128; st > 0; st >>= 1) {
-			if ((int) threadIdx.x < st)
-				s_red[threadIdx.x] += s_red[threadIdx.x + st];
-			__syncthreads();
-		}
-		if (threadIdx.x == 0)
-			part[blockIdx.x] = s_red[0];
+	for (int st = 128; st > 0; st >>= 1) {
+		if ((int) threadIdx.x < st)
+			s_red[threadIdx.x] += s_red[threadIdx.x + st];
+		__syncthreads();
 	}
+	if (threadIdx.x == 0)
+		part[slot] = s_red[0];
+}
+
+template <int N, int OP, typename T, typename G, bool PUPD, int V>
+__global__ void __launch_bounds__((AxShape<N, V>::BLOCK))
+k_ax(int nel, const T *__restrict__ u, T *__restrict__ au,
+		const G *__restrict__ gf, const T *__restrict__ mass,
+		const T *__restrict__ dmat, T lambda,
+		const int *__restrict__ bc, int apply_mask,
+		const int *__restrict__ done, double *__restrict__ part,
+		const T *__restrict__ zin, T *__restrict__ pio,
+		const double *__restrict__ scal, int ib, int ic)
+{
+	if (done && *done)          // solver already converged: no-op
+		return;
+	const double pd = ax_team<N, OP, T, G, PUPD, V>(blockIdx.x, nel, u, au,
+			gf, mass, dmat, lambda, bc, apply_mask, zin, pio, scal, ib, ic);
+	if (part)
+		ax_block_partial(pd, part, blockIdx.x);
 }
 
 template <int N, typename T, typename G, bool PU, int V>
@@ -281,7 +302,8 @@ static int ax_dispatch(hipStream_t st, int n, int v, int nel, int op,
 #define CASEV(NN, VV) case NN * SSX_AX_NVARIANTS + VV: \
 		launch_ax_v<NN, T, G, VV>(st, nel, op, u, au, gf, mass, dmat, \
 				lambda, bc, apply_mask, done, part, pu); break;
-#define CASE(NN) CASEV(NN, 0) CASEV(NN, 1) CASEV(NN, 2) CASEV(NN, 3)
+#define CASE(NN) CASEV(NN, 0) CASEV(NN, 1) CASEV(NN, 2) CASEV(NN, 3) \
+		CASEV(NN, 4) CASEV(NN, 5) CASEV(NN, 6) CASEV(NN, 7)
 	CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9)
 	CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
 #undef CASE

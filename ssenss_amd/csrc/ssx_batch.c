@@ -1,0 +1,482 @@
+/*
+ * ssx_batch.c -- independent CG problems solved together: the Jacobi-PCG
+ * of conjugate_gradient (solver.c:33-81) on level 0 of several contexts
+ * (any element counts and orders; a context may appear more than once,
+ * e.g. the three velocity components), with one launch per CG stage for
+ * the whole batch and the iterations replayed from a captured HIP graph.
+ *
+ * Set-up (r = f - A u, z = D^-1 r, p = z, <r,z>_w) runs per problem with
+ * the single-problem kernels.  Then every iteration is: the operator (one
+ * launch per distinct order, with the previous direction update fused in
+ * from the second iteration), the DSS, the alpha reduction, the update and
+ * the beta / stopping reduction -- each one launch over all problems.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "ssx_internal.h"
+#include "ssx_batch.h"
+
+#define HIPB(x)                                                              \
+	do {                                                                 \
+		hipError_t e_ = (x);                                         \
+		if (e_ != hipSuccess) {                                      \
+			ssx_set_error("%s: %s", #x, hipGetErrorString(e_));   \
+			return SSX_EHIP;                                     \
+		}                                                            \
+	} while (0)
+#define TRYB(x)                                                              \
+	do {                                                                 \
+		int rc_ = (x);                                               \
+		if (rc_ != SSX_OK)                                           \
+			return rc_;                                          \
+	} while (0)
+
+#define GRAPH_ITERS 4           /* iterations per captured graph */
+
+struct ssx_batch {
+	int np;
+	ssx_ctx **ctx;
+	int *op, *var;
+	void *stream, *own_stream;
+	/* per-problem work vectors and scratch (device) */
+	double **r, **z, **p, **ap;
+	double *part, *scal;
+	int *flags;
+	size_t *poff;              /* partial offsets per problem */
+	ssx_bprob *h_prob, *d_prob;
+	/* work tables */
+	int norder, order_n[SSX_NMAX + 1], order_v[SSX_NMAX + 1];
+	ssx_bwork *d_axtab[SSX_NMAX + 1];
+	long long nax[SSX_NMAX + 1];
+	ssx_bwork *d_dsstab, *d_updtab;
+	long long ndss, nupd;
+	double *bpart;             /* set-up reductions [SSX_NPART] */
+	int *h_flags;
+	double *h_scal;
+	hipGraphExec_t gexec;
+	int have_graph;
+};
+
+static int balloc(void **p, size_t bytes)
+{
+	if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) {
+		*p = NULL;
+		ssx_set_error("hipMalloc of %zu bytes failed (batch)", bytes);
+		return SSX_ENOMEM;
+	}
+	return SSX_OK;
+}
+
+static const double *jdiag0(const ssx_ctx *c, int var)
+{
+	int ch = c->diag_choice;
+	if (ch < 0)
+		ch = var == SSX_PRESSURE ? 0 : 1;      /* precon.c:211-225 */
+	return ch == 0 ? c->lev[0].diag_p : c->lev[0].diag_h;
+}
+
+void ssx_batch_destroy(ssx_batch *b)
+{
+	if (!b)
+		return;
+	if (b->have_graph)
+		hipGraphExecDestroy(b->gexec);
+	for (int i = 0; i < b->np; ++i) {
+		if (b->r) hipFree(b->r[i]);
+		if (b->z) hipFree(b->z[i]);
+		if (b->p) hipFree(b->p[i]);
+		if (b->ap) hipFree(b->ap[i]);
+	}
+	for (int k = 0; k < b->norder; ++k)
+		hipFree(b->d_axtab[k]);
+	hipFree(b->d_dsstab);
+	hipFree(b->d_updtab);
+	hipFree(b->part);
+	hipFree(b->scal);
+	hipFree(b->flags);
+	hipFree(b->d_prob);
+	hipFree(b->bpart);
+	if (b->h_flags)
+		hipHostFree(b->h_flags);
+	if (b->h_scal)
+		hipHostFree(b->h_scal);
+	if (b->own_stream)
+		hipStreamDestroy((hipStream_t) b->own_stream);
+	free(b->r); free(b->z); free(b->p); free(b->ap);
+	free(b->poff); free(b->h_prob);
+	free(b->ctx); free(b->op); free(b->var);
+	free(b);
+}
+
+static int upload_table(ssx_bwork **d, const ssx_bwork *h, long long n)
+{
+	TRYB(balloc((void **) d, sizeof(ssx_bwork) * (size_t) n));
+	if (n)
+		HIPB(hipMemcpy(*d, h, sizeof(ssx_bwork) * (size_t) n,
+				hipMemcpyHostToDevice));
+	return SSX_OK;
+}
+
+static int build(ssx_batch *b)
+{
+	const int np = b->np;
+	b->r = calloc(np, sizeof(double *));
+	b->z = calloc(np, sizeof(double *));
+	b->p = calloc(np, sizeof(double *));
+	b->ap = calloc(np, sizeof(double *));
+	b->poff = calloc(np + 1, sizeof(size_t));
+	b->h_prob = calloc(np, sizeof(ssx_bprob));
+	if (!b->r || !b->z || !b->p || !b->ap || !b->poff || !b->h_prob) {
+		ssx_set_error("out of host memory (batch)");
+		return SSX_ENOMEM;
+	}
+	/* partial-sum layout and per-problem constants */
+	for (int i = 0; i < np; ++i) {
+		const ssx_ctx *c = b->ctx[i];
+		const ssx_level *L = &c->lev[0];
+		ssx_bprob *d = &b->h_prob[i];
+		d->n = L->n;
+		d->nel = c->nel;
+		d->op = b->op[i];
+		d->pressure = b->var[i] == SSX_PRESSURE;
+		d->apply_mask = !d->pressure;
+		d->ntot = (long long) L->ntot;
+		d->lambda = c->lambda;
+		d->nunique = (double) L->nunique;
+		d->gf = L->gf;
+		d->mass = L->mass;
+		d->dmat = L->dmat;
+		d->dinv = jdiag0(c, b->var[i]);
+		d->scale = L->scale;
+		d->mask = d->pressure ? NULL : L->mask;
+		d->bc = L->bc;
+		d->g2 = L->grp[0]; d->g4 = L->grp[1]; d->g8 = L->grp[2];
+		d->G2 = L->ngrp[0]; d->G4 = L->ngrp[1]; d->G8 = L->ngrp[2];
+		const int v = L->axv[SSX_PREC_FP64];
+		const int teams = ssxk_bax_teams(L->n, v);
+		d->nba = (c->nel + teams - 1) / teams;
+		d->nbd = ssxk_dss_blocks(L->ngrp);
+		d->nbu = (d->ntot + SSX_BATCH_CHUNK - 1) / SSX_BATCH_CHUNK;
+		b->poff[i + 1] = b->poff[i] + (size_t) (d->nba + d->nbd
+				+ 4 * d->nbu);
+		TRYB(balloc((void **) &b->r[i], sizeof(double) * L->ntot));
+		TRYB(balloc((void **) &b->z[i], sizeof(double) * L->ntot));
+		TRYB(balloc((void **) &b->p[i], sizeof(double) * L->ntot));
+		TRYB(balloc((void **) &b->ap[i], sizeof(double) * L->ntot));
+		d->r = b->r[i]; d->z = b->z[i]; d->p = b->p[i]; d->ap = b->ap[i];
+	}
+	TRYB(balloc((void **) &b->part, sizeof(double) * b->poff[np]));
+	TRYB(balloc((void **) &b->scal, sizeof(double) * CG_NSLOT * np));
+	TRYB(balloc((void **) &b->flags, sizeof(int) * 2 * np));
+	TRYB(balloc((void **) &b->d_prob, sizeof(ssx_bprob) * np));
+	TRYB(balloc((void **) &b->bpart, sizeof(double) * SSX_NPART));
+	HIPB(hipHostMalloc((void **) &b->h_flags, sizeof(int) * 2 * np, 0));
+	HIPB(hipHostMalloc((void **) &b->h_scal, sizeof(double) * CG_NSLOT * np,
+			0));
+	for (int i = 0; i < np; ++i) {
+		ssx_bprob *d = &b->h_prob[i];
+		d->part = b->part + b->poff[i];
+		d->scal = b->scal + CG_NSLOT * i;
+		d->flags = b->flags + 2 * i;
+	}
+	/* operator tables, one per (order, launch shape) */
+	ssx_bwork *tmp;
+	long long tot = 0;
+	for (int i = 0; i < np; ++i)       /* nel bounds any operator table */
+		tot += b->h_prob[i].nel + b->h_prob[i].nbd + b->h_prob[i].nbu;
+	tmp = malloc(sizeof(ssx_bwork) * (size_t) (tot ? tot : 1));
+	if (!tmp) {
+		ssx_set_error("out of host memory (batch)");
+		return SSX_ENOMEM;
+	}
+	b->norder = 0;
+	for (int i = 0; i < np; ++i) {
+		const int n = b->h_prob[i].n;
+		int k;
+		for (k = 0; k < b->norder; ++k)
+			if (b->order_n[k] == n)
+				break;
+		if (k == b->norder) {
+			b->order_n[k] = n;
+			/* the launch shape the autotuner chose for this order */
+			b->order_v[k] = b->ctx[i]->lev[0].axv[SSX_PREC_FP64];
+			b->norder++;
+		}
+	}
+	int rc = SSX_OK;
+	for (int k = 0; k < b->norder && rc == SSX_OK; ++k) {
+		long long m = 0;
+		for (int i = 0; i < np; ++i) {
+			if (b->h_prob[i].n != b->order_n[k])
+				continue;
+			/* a problem with another shape for the same order gets this
+			 * order's shape: its block count follows */
+			const int teams = ssxk_bax_teams(b->order_n[k], b->order_v[k]);
+			b->h_prob[i].nba = (b->h_prob[i].nel + teams - 1) / teams;
+			for (long long q = 0; q < b->h_prob[i].nba; ++q) {
+				tmp[m].prob = i;
+				tmp[m].blk = (int) q;
+				++m;
+			}
+		}
+		b->nax[k] = m;
+		rc = upload_table(&b->d_axtab[k], tmp, m);
+	}
+	/* the operator blocks may have changed: recompute partial offsets */
+	for (int i = 0; i < np && rc == SSX_OK; ++i) {
+		const ssx_bprob *d = &b->h_prob[i];
+		b->poff[i + 1] = b->poff[i] + (size_t) (d->nba + d->nbd + 4 * d->nbu);
+	}
+	if (rc == SSX_OK) {
+		hipFree(b->part);
+		rc = balloc((void **) &b->part, sizeof(double) * b->poff[np]);
+		for (int i = 0; i < np && rc == SSX_OK; ++i)
+			b->h_prob[i].part = b->part + b->poff[i];
+	}
+	if (rc == SSX_OK) {
+		long long m = 0;
+		for (int i = 0; i < np; ++i)
+			for (long long q = 0; q < b->h_prob[i].nbd; ++q) {
+				tmp[m].prob = i;
+				tmp[m].blk = (int) q;
+				++m;
+			}
+		b->ndss = m;
+		rc = upload_table(&b->d_dsstab, tmp, m);
+	}
+	if (rc == SSX_OK) {
+		long long m = 0;
+		for (int i = 0; i < np; ++i)
+			for (long long q = 0; q < b->h_prob[i].nbu; ++q) {
+				tmp[m].prob = i;
+				tmp[m].blk = (int) q;
+				++m;
+			}
+		b->nupd = m;
+		rc = upload_table(&b->d_updtab, tmp, m);
+	}
+	free(tmp);
+	return rc;
+}
+
+int ssx_batch_create(ssx_batch **out, ssx_ctx *const *ctx, const int *op,
+		const int *var, int count)
+{
+	if (!out)
+		return SSX_EINVAL;
+	*out = NULL;
+	if (!ctx || !op || !var || count < 1) {
+		ssx_set_error("ssx_batch_create: empty batch");
+		return SSX_EINVAL;
+	}
+	for (int i = 0; i < count; ++i) {
+		if (!ctx[i] || (op[i] != SSX_OP_POISSON && op[i] != SSX_OP_HELMHOLTZ)
+				|| var[i] < SSX_PRESSURE || var[i] > SSX_Z_VELOCITY) {
+			ssx_set_error("ssx_batch_create: bad problem %d", i);
+			return SSX_EINVAL;
+		}
+		if (!ctx[i]->lev[0].fused_ok) {
+			ssx_set_error("ssx_batch_create: problem %d: mesh needs the "
+					"plain CG path", i);
+			return SSX_EMESH;
+		}
+	}
+	ssx_batch *b = calloc(1, sizeof(*b));
+	if (!b)
+		return SSX_ENOMEM;
+	b->np = count;
+	b->ctx = malloc(sizeof(ssx_ctx *) * count);
+	b->op = malloc(sizeof(int) * count);
+	b->var = malloc(sizeof(int) * count);
+	if (!b->ctx || !b->op || !b->var) {
+		ssx_batch_destroy(b);
+		return SSX_ENOMEM;
+	}
+	memcpy(b->ctx, ctx, sizeof(ssx_ctx *) * count);
+	memcpy(b->op, op, sizeof(int) * count);
+	memcpy(b->var, var, sizeof(int) * count);
+	hipStream_t s;
+	if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+		ssx_set_error("hipStreamCreate (batch)");
+		ssx_batch_destroy(b);
+		return SSX_EHIP;
+	}
+	b->own_stream = b->stream = s;
+	int rc = build(b);
+	if (rc != SSX_OK) {
+		ssx_batch_destroy(b);
+		return rc;
+	}
+	*out = b;
+	return SSX_OK;
+}
+
+int ssx_batch_set_stream(ssx_batch *b, void *s)
+{
+	if (!b)
+		return SSX_EINVAL;
+	b->stream = s;
+	if (b->have_graph) {             /* recaptured on the new stream */
+		hipGraphExecDestroy(b->gexec);
+		b->have_graph = 0;
+	}
+	return SSX_OK;
+}
+
+/* one iteration: operator per order, DSS, alpha, update, beta + stop */
+static int iteration(ssx_batch *b, int pupd)
+{
+	void *st = b->stream;
+	for (int k = 0; k < b->norder; ++k)
+		TRYB(ssxk_bax(st, b->order_n[k], b->order_v[k], pupd, b->d_prob,
+				b->d_axtab[k], b->nax[k]));
+	TRYB(ssxk_bdss(st, b->d_prob, b->d_dsstab, b->ndss));
+	TRYB(ssxk_breduce(st, 0, b->d_prob, b->np));
+	TRYB(ssxk_bupdate(st, b->d_prob, b->d_updtab, b->nupd));
+	return ssxk_breduce(st, 1, b->d_prob, b->np);
+}
+
+static int capture(ssx_batch *b)
+{
+	hipStream_t st = (hipStream_t) b->stream;
+	hipGraph_t g;
+	HIPB(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+	int rc = SSX_OK;
+	for (int k = 0; k < GRAPH_ITERS && rc == SSX_OK; ++k)
+		rc = iteration(b, 1);
+	hipError_t e = hipStreamEndCapture(st, &g);
+	if (rc != SSX_OK)
+		return rc;
+	if (e != hipSuccess) {
+		ssx_set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+		return SSX_EHIP;
+	}
+	e = hipGraphInstantiate(&b->gexec, g, NULL, NULL, 0);
+	hipGraphDestroy(g);
+	if (e != hipSuccess) {
+		ssx_set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+		return SSX_EHIP;
+	}
+	b->have_graph = 1;
+	return SSX_OK;
+}
+
+/* per-problem set-up: mask f, r = f - A u, null space, z = D^-1 r,
+ * p = z, <r,z>_w (solver.c:46-56) */
+static int setup_problem(ssx_batch *b, int i, double *f, double *u)
+{
+	const ssx_ctx *c = b->ctx[i];
+	const ssx_level *L = &c->lev[0];
+	const ssx_bprob *d = &b->h_prob[i];
+	void *st = b->stream;
+	const size_t nt = L->ntot;
+	double *sc = b->scal + CG_NSLOT * i;
+	if (!d->pressure)
+		TRYB(ssxk_mul(st, nt, f, L->mask, f));
+	TRYB(ssxk_ax(st, L->n, L->axv[SSX_PREC_FP64], c->nel, d->op, u, d->r,
+			L->gf, L->mass, L->dmat, c->lambda, L->bc, !d->pressure, NULL,
+			NULL));
+	TRYB(ssxk_dss_all(st, L->grp, L->ngrp, d->r, NULL, NULL, 0, NULL));
+	TRYB(ssxk_axpby(st, nt, 1.0, f, NULL, -1.0, d->r));
+	for (int pass = 0; pass < 2; ++pass) {
+		double *x = pass == 0 ? d->r : d->z;
+		if (pass == 1)
+			TRYB(ssxk_mul(st, nt, d->r, d->dinv, d->z));
+		if (d->pressure) {      /* solver.c:200-213 */
+			TRYB(ssxk_dot(st, nt, x, L->scale, NULL, b->bpart, 0));
+			TRYB(ssxk_reduce(st, b->bpart, 0, sc, CG_NULLC));
+			TRYB(ssxk_shift_s(st, nt, sc, CG_NULLC, (double) L->nunique,
+					x));
+		}
+	}
+	TRYB(ssxk_copy(st, d->p, d->z, nt));
+	TRYB(ssxk_dot(st, nt, d->r, L->scale, d->z, b->bpart, 0));
+	return ssxk_reduce(st, b->bpart, 0, sc, CG_RDZ);
+}
+
+int ssx_batch_cg(ssx_batch *b, double *const *f, double *const *u,
+		const double *eps, int max_iter, int use_graph, ssx_stats *stats)
+{
+	if (!b || !f || !u || !eps || max_iter < 1) {
+		ssx_set_error("ssx_batch_cg: bad arguments");
+		return SSX_EINVAL;
+	}
+	if (use_graph && !b->stream) {
+		ssx_set_error("ssx_batch_cg: graph replay needs a non-null stream");
+		return SSX_EINVAL;
+	}
+	const int np = b->np;
+	hipStream_t st = (hipStream_t) b->stream;
+	struct timespec t0, t1;
+	clock_gettime(CLOCK_MONOTONIC, &t0);
+	for (int i = 0; i < np; ++i) {
+		if (!f[i] || !u[i]) {
+			ssx_set_error("ssx_batch_cg: null vector for problem %d", i);
+			return SSX_EINVAL;
+		}
+		b->h_prob[i].u = u[i];
+		b->h_prob[i].eps = eps[i];
+		b->h_prob[i].maxit = max_iter;
+	}
+	HIPB(hipMemcpyAsync(b->d_prob, b->h_prob, sizeof(ssx_bprob) * np,
+			hipMemcpyHostToDevice, st));
+	for (int i = 0; i < np; ++i)
+		TRYB(setup_problem(b, i, f[i], u[i]));
+	TRYB(ssxk_set_ints(st, b->flags, 0, 2 * np < 64 ? 2 * np : 64));
+	for (int q = 64; q < 2 * np; q += 64)
+		TRYB(ssxk_set_ints(st, b->flags + q, 0,
+				2 * np - q < 64 ? 2 * np - q : 64));
+	/* first iteration: p = z already */
+	TRYB(iteration(b, 0));
+	int launched = 1;
+	if (use_graph && !b->have_graph)
+		TRYB(capture(b));
+	for (;;) {
+		int all = 1;
+		HIPB(hipMemcpyAsync(b->h_flags, b->flags, sizeof(int) * 2 * np,
+				hipMemcpyDeviceToHost, st));
+		HIPB(hipStreamSynchronize(st));
+		for (int i = 0; i < np; ++i)
+			all &= b->h_flags[2 * i] != 0;
+		if (all || launched >= max_iter)
+			break;
+		if (use_graph) {
+			HIPB(hipGraphLaunch(b->gexec, st));
+			launched += GRAPH_ITERS;
+		} else {
+			for (int k = 0; k < GRAPH_ITERS; ++k)
+				TRYB(iteration(b, 1));
+			launched += GRAPH_ITERS;
+		}
+	}
+	HIPB(hipMemcpyAsync(b->h_scal, b->scal, sizeof(double) * CG_NSLOT * np,
+			hipMemcpyDeviceToHost, st));
+	HIPB(hipStreamSynchronize(st));
+	clock_gettime(CLOCK_MONOTONIC, &t1);
+	const double secs = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec
+			- t0.tv_nsec);
+	int rc = SSX_OK;
+	for (int i = 0; i < np; ++i) {
+		const double err = sqrt(b->h_scal[CG_NSLOT * i + CG_RR]);
+		if (stats) {
+			memset(&stats[i], 0, sizeof(ssx_stats));
+			stats[i].iterations = b->h_flags[2 * i + 1];
+			stats[i].residual = err;
+			stats[i].converged = err <= eps[i];
+			stats[i].fine_ax = b->h_flags[2 * i + 1] + 1;
+			stats[i].seconds = secs;
+		}
+		if (isnan(err) || isinf(err)) {
+			ssx_set_error("batch problem %d: residual is %g", i, err);
+			rc = SSX_ENUMERIC;
+		} else if (rc == SSX_OK && !(err <= eps[i])) {
+			rc = SSX_ENOTCONV;
+		}
+	}
+	return rc;
+}

@@ -1,0 +1,243 @@
+// ssx_bkernels.hip -- one launch per CG stage for a batch of independent
+// problems of any element count and order (conjugate_gradient,
+// solver.c:33-81, Jacobi preconditioner, lock-step iterations).
+//
+// Work tables map each workgroup to (problem, local workgroup); every
+// problem has its own done flag, scalars and partial sums, so a converged
+// problem's workgroups return at once while the others go on.  The
+// operator stage is one launch per distinct order (the kernel shape
+// depends on n); DSS, update and the two reductions are one launch each.
+// Per-problem partials are summed in a fixed order: results do not depend
+// on the other problems of the batch.
+#include <hip/hip_runtime.h>
+#include "ssx_internal.h"
+#include "ssx_device.h"
+#include "ssx_ax.h"
+#include "ssx_batch.h"
+
+#define LAUNCH_OK(name)                                                    \
+	do {                                                               \
+		hipError_t e_ = hipGetLastError();                         \
+		if (e_ != hipSuccess) {                                    \
+			ssx_set_error("%s: %s", name, hipGetErrorString(e_)); \
+			return SSX_EHIP;                                   \
+		}                                                          \
+	} while (0)
+
+// operator (PUPD: with the direction update of the previous iteration)
+template <int N, bool PUPD, int V>
+__global__ void __launch_bounds__((AxShape<N, V>::BLOCK))
+k_bax(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
+{
+	const ssx_bwork w = tab[blockIdx.x];
+	const ssx_bprob *d = pr + w.prob;
+	if (d->flags[0])
+		return;
+	const int ic = d->pressure ? CG_NULLC : -1;
+	// PUPD reads z and p and writes p; otherwise p is the operand
+	const double *u = PUPD ? nullptr : d->p;
+	double *pio = PUPD ? d->p : nullptr;
+	double pd;
+	if (d->op == SSX_OP_HELMHOLTZ)
+		pd = ax_team<N, 1, double, double, PUPD, V>(w.blk, d->nel, u, d->ap,
+				d->gf, d->mass, d->dmat, d->lambda, d->bc, d->apply_mask,
+				d->z, pio, d->scal, CG_BETA, ic);
+	else
+		pd = ax_team<N, 0, double, double, PUPD, V>(w.blk, d->nel, u, d->ap,
+				d->gf, d->mass, d->dmat, d->lambda, d->bc, d->apply_mask,
+				d->z, pio, d->scal, CG_BETA, ic);
+	ax_block_partial(pd, d->part, w.blk);
+}
+
+// DSS of Ap with the shared part of <p, Ap>_w; local workgroups number the
+// pairs, then the quadruples, then the octuples of the problem
+__global__ void __launch_bounds__(256)
+k_bdss(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
+{
+	const ssx_bwork w = tab[blockIdx.x];
+	const ssx_bprob *d = pr + w.prob;
+	if (d->flags[0])
+		return;
+	const long long b2 = (d->G2 + 255) / 256, b4 = (d->G4 + 255) / 256;
+	const long long b = w.blk;
+	double pd = 0.0;
+	if (b < b2) {
+		const long long g = b * 256 + threadIdx.x;
+		if (g < d->G2)
+			pd = dss_group<2, double>(d->G2, g, d->g2, d->ap, d->p, 0);
+	} else if (b < b2 + b4) {
+		const long long g = (b - b2) * 256 + threadIdx.x;
+		if (g < d->G4)
+			pd = dss_group<4, double>(d->G4, g, d->g4, d->ap, d->p, 0);
+	} else {
+		const long long g = (b - b2 - b4) * 256 + threadIdx.x;
+		if (g < d->G8)
+			pd = dss_group<8, double>(d->G8, g, d->g8, d->ap, d->p, 0);
+	}
+	ax_block_partial(pd, d->part, d->nba + b);
+}
+
+// u += alpha p; r -= alpha Ap; z = D^-1 r; partials of <r,z>_w, <r,r>_w,
+// <z,1>_w, <r,1>_w over this workgroup's chunk
+__global__ void __launch_bounds__(256)
+k_bupdate(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
+{
+	const ssx_bwork w = tab[blockIdx.x];
+	const ssx_bprob *d = pr + w.prob;
+	if (d->flags[0])
+		return;
+	__shared__ double sh[256];
+	const double a = d->scal[CG_ALPHA];
+	double acc[4] = { 0.0, 0.0, 0.0, 0.0 };
+	const long long q0 = (long long) w.blk * SSX_BATCH_CHUNK;
+	for (int t = 0; t < SSX_BATCH_CHUNK / 256; ++t) {
+		const long long q = q0 + t * 256 + threadIdx.x;
+		if (q >= d->ntot)
+			break;
+		d->u[q] = a * d->p[q] + d->u[q];
+		const double rv = -a * d->ap[q] + d->r[q];
+		d->r[q] = rv;
+		const double zv = rv * d->dinv[q];
+		d->z[q] = zv;
+		const double s = d->scale[q];
+		acc[0] += (rv * s) * zv;
+		acc[1] += (rv * s) * rv;
+		if (d->pressure) {
+			acc[2] += zv * s;
+			acc[3] += rv * s;
+		}
+	}
+	double *base = d->part + d->nba + d->nbd;
+	for (int c = 0; c < 4; ++c) {
+		sh[threadIdx.x] = acc[c];
+		__syncthreads();
+		for (int st = 128; st > 0; st >>= 1) {
+			if ((int) threadIdx.x < st)
+				sh[threadIdx.x] += sh[threadIdx.x + st];
+			__syncthreads();
+		}
+		if (threadIdx.x == 0)
+			base[(long long) c * d->nbu + w.blk] = sh[0];
+		__syncthreads();
+	}
+}
+
+// one workgroup per problem.  mode 0: <p,Ap>_w -> alpha.  mode 1: the
+// null-space constant, rdz, beta, the iteration count and the stopping
+// test sqrt(<r,r>_w) <= eps or count >= maxit (or NaN), as k_cg_reduce
+__global__ void __launch_bounds__(1024)
+k_breduce(int mode, const ssx_bprob *__restrict__ pr)
+{
+	const ssx_bprob *d = pr + blockIdx.x;
+	if (d->flags[0])
+		return;
+	__shared__ double sh[4][1024];
+	const int nq = mode == 0 ? 1 : (d->pressure ? 4 : 2);
+	const double *base = d->part + (mode == 0 ? 0 : d->nba + d->nbd);
+	const long long m = mode == 0 ? d->nba + d->nbd : d->nbu;
+	for (int c = 0; c < 4; ++c) {
+		double acc = 0.0;
+		if (c < nq)
+			for (long long q = threadIdx.x; q < m; q += 1024)
+				acc += base[(long long) c * m + q];
+		sh[c][threadIdx.x] = acc;
+	}
+	__syncthreads();
+	for (int st = 512; st > 0; st >>= 1) {
+		if ((int) threadIdx.x < st)
+			for (int c = 0; c < 4; ++c)
+				sh[c][threadIdx.x] += sh[c][threadIdx.x + st];
+		__syncthreads();
+	}
+	if (threadIdx.x != 0)
+		return;
+	double *s = d->scal;
+	if (mode == 0) {
+		s[CG_PAP] = sh[0][0];
+		s[CG_ALPHA] = s[CG_RDZ] / sh[0][0];
+		return;
+	}
+	double rz = sh[0][0], rr = sh[1][0], c = 0.0;
+	if (d->pressure) {
+		c = sh[2][0] / d->nunique;
+		rz = rz - c * sh[3][0];
+	}
+	s[CG_NULLC] = c;
+	s[CG_BETA] = rz / s[CG_RDZ];
+	s[CG_RDZ] = rz;
+	s[CG_RR] = rr;
+	const int it = d->flags[1] + 1;
+	d->flags[1] = it;
+	if (sqrt(rr) <= d->eps || it >= d->maxit || !(rr == rr))
+		d->flags[0] = 1;
+}
+
+template <int N, bool PU, int V>
+static void launch_bax(hipStream_t st, const ssx_bprob *pr,
+		const ssx_bwork *tab, long long ntab)
+{
+	hipLaunchKernelGGL((k_bax<N, PU, V>), dim3((unsigned) ntab),
+			dim3(AxShape<N, V>::BLOCK), 0, st, pr, tab);
+}
+
+extern "C" int ssxk_bax(void *stream, int n, int v, int pupd,
+		const ssx_bprob *pr, const ssx_bwork *tab, long long ntab)
+{
+	if (ntab <= 0)
+		return SSX_OK;
+	hipStream_t st = (hipStream_t) stream;
+	switch ((n * SSX_AX_NVARIANTS + v) * 2 + (pupd != 0)) {
+#define CASEV(NN, VV) \
+	case (NN * SSX_AX_NVARIANTS + VV) * 2: \
+		launch_bax<NN, false, VV>(st, pr, tab, ntab); break; \
+	case (NN * SSX_AX_NVARIANTS + VV) * 2 + 1: \
+		launch_bax<NN, true, VV>(st, pr, tab, ntab); break;
+#define CASE(NN) CASEV(NN, 0) CASEV(NN, 1) CASEV(NN, 2) CASEV(NN, 3) \
+		CASEV(NN, 4) CASEV(NN, 5) CASEV(NN, 6) CASEV(NN, 7)
+	CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9)
+	CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+#undef CASEV
+	default:
+		ssx_set_error("batched operator: n=%d variant %d unsupported", n, v);
+		return SSX_EINVAL;
+	}
+	LAUNCH_OK("k_bax");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_bax_teams(int n, int v)
+{
+	return ssx_ax_teams(n, v);
+}
+
+extern "C" int ssxk_bdss(void *stream, const ssx_bprob *pr,
+		const ssx_bwork *tab, long long ntab)
+{
+	if (ntab <= 0)
+		return SSX_OK;
+	hipLaunchKernelGGL(k_bdss, dim3((unsigned) ntab), dim3(256), 0,
+			(hipStream_t) stream, pr, tab);
+	LAUNCH_OK("k_bdss");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_bupdate(void *stream, const ssx_bprob *pr,
+		const ssx_bwork *tab, long long ntab)
+{
+	if (ntab <= 0)
+		return SSX_OK;
+	hipLaunchKernelGGL(k_bupdate, dim3((unsigned) ntab), dim3(256), 0,
+			(hipStream_t) stream, pr, tab);
+	LAUNCH_OK("k_bupdate");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_breduce(void *stream, int mode, const ssx_bprob *pr,
+		int np)
+{
+	hipLaunchKernelGGL(k_breduce, dim3(np), dim3(1024), 0,
+			(hipStream_t) stream, mode, pr);
+	LAUNCH_OK("k_breduce");
+	return SSX_OK;
+}

@@ -101,7 +101,8 @@ class Context:
 
     def ax_variant(self, level, prec=0):
         """(variant in use, measured us per launch of each variant)"""
-        us = np.zeros(4)
+        from .native import AX_NVARIANTS
+        us = np.zeros(AX_NVARIANTS)
         v = self._lib.ssx_ax_variant(self.h, level, prec, _dp(us))
         return v, us
 
@@ -216,3 +217,49 @@ class Context:
         check(self._lib.ssx_particles_step_host(self.h, C.byref(ps), t_now,
                                                 dt, C.byref(mp)))
         return mp.value
+
+
+class Batch:
+    """Independent Jacobi-PCG problems solved together (ssx_batch_*):
+    problems = [(context, op, var), ...]; a context may repeat."""
+
+    def __init__(self, problems):
+        self._lib = load()
+        self._ctxs = [p[0] for p in problems]     # keep them alive
+        n = len(problems)
+        hs = (C.c_void_p * n)(*[p[0].h.value for p in problems])
+        ops = (C.c_int * n)(*[p[1] for p in problems])
+        var = (C.c_int * n)(*[p[2] for p in problems])
+        h = C.c_void_p()
+        check(self._lib.ssx_batch_create(C.byref(h), hs, ops, var, n))
+        self.h = h
+        self.n = n
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.ssx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_stream(self, stream):
+        handle = None if stream is None else C.c_void_p(stream.cuda_stream)
+        check(self._lib.ssx_batch_set_stream(self.h, handle))
+
+    def cg(self, fs, us, eps, max_iter, use_graph=True, allow_notconv=False):
+        """fs, us: lists of float64 device tensors; returns a list of
+        Stats.  Synchronises the device first (the batch runs on its own
+        stream)."""
+        import torch
+        torch.cuda.synchronize()
+        n = self.n
+        fp = (C.c_void_p * n)(*[f.data_ptr() for f in fs])
+        up = (C.c_void_p * n)(*[u.data_ptr() for u in us])
+        ep = (C.c_double * n)(*[float(e) for e in eps])
+        st = (Stats * n)()
+        rc = self._lib.ssx_batch_cg(self.h, fp, up, ep, max_iter,
+                                    1 if use_graph else 0, st)
+        if not (allow_notconv and rc == 4):
+            check(rc)
+        return list(st)

@@ -10,7 +10,7 @@ PC_JACOBI, PC_MG = 0, 1
 OPT_FUSED_CG = 1
 OPT_SMOOTHER_PRECISION = 2
 OPT_PERSISTENT_CG = 3
-AX_NVARIANTS = 4
+AX_NVARIANTS = 8
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 (DATA_GFAC, DATA_MASS, DATA_DIAG_P, DATA_DIAG_H, DATA_SCALE, DATA_MASK,
  DATA_INVMASS, DATA_X, DATA_Y, DATA_Z) = range(10)
@@ -32,7 +32,8 @@ EXPORTED_SYMBOLS = [
     "ssx_solve_helmholtz_cg", "ssx_particles_set_velocity",
     "ssx_particles_step", "ssx_particles_set_velocity_host",
     "ssx_particles_step_host", "ssx_autotune", "ssx_ax_variant",
-    "ssx_set_ax_variant",
+    "ssx_set_ax_variant", "ssx_batch_create", "ssx_batch_destroy",
+    "ssx_batch_set_stream", "ssx_batch_cg",
 ]
 
 
@@ -132,6 +133,11 @@ def load():
         "ssx_particles_set_velocity": (i, [vp, vp, vp, vp]),
         "ssx_particles_set_velocity_host": (i, [vp, dp, dp, dp]),
         "ssx_autotune": (i, [vp, i]),
+        "ssx_batch_create": (i, [C.POINTER(vp), C.POINTER(vp), ip, ip, i]),
+        "ssx_batch_destroy": (None, [vp]),
+        "ssx_batch_set_stream": (i, [vp, vp]),
+        "ssx_batch_cg": (i, [vp, C.POINTER(vp), C.POINTER(vp), dp, i, i,
+                             C.POINTER(Stats)]),
         "ssx_ax_variant": (i, [vp, i, i, dp]),
         "ssx_set_ax_variant": (i, [vp, i, i, i]),
         "ssx_particles_step_host": (i, [vp, C.POINTER(ParticlesStruct), d, d,

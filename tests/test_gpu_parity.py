@@ -562,3 +562,89 @@ def test_operator_launch_variants_bit_identical(name):
             its.append((st.iterations, host(x)))
         assert all(i == its[0][0] for i, _ in its)
         assert all(np.array_equal(x, its[0][1]) for _, x in its)
+
+
+def _batch_problems():
+    """golden meshes of different orders and element counts, each with the
+    reference's Jacobi-PCG Helmholtz solve, plus a pressure Poisson"""
+    names = ["box2_n5", "cube2_n8", "periodic3_n6", "channel_n6",
+             "cube2_n12"]
+    return [(nm, golden(nm)) for nm in names]
+
+
+def test_batch_cg_matches_single_solves_and_reference():
+    """one launch per CG stage for five meshes of orders 4..11 (Helmholtz,
+    x velocity) plus a pressure Poisson problem: iteration counts within
+    one of the single-problem solver and of the reference, solutions within
+    the solve tolerance of the reference's"""
+    cases = _batch_problems()
+    ctxs = [make_ctx(golden_mesh(Z)) for _, Z in cases]
+    try:
+        probs = [(c, S.OP_HELMHOLTZ, S.X_VELOCITY) for c in ctxs]
+        probs.append((ctxs[1], S.OP_POISSON, S.PRESSURE))
+        rhs = [Z["in_f"] for _, Z in cases] + [cases[1][1]["in_f"]]
+        eps = [Z["ref_scalars"][0] for _, Z in cases]
+        eps.append(cases[1][1]["ref_scalars"][0])
+        batch = S.Batch(probs)
+        try:
+            fs = [dev(r) for r in rhs]
+            us = [torch.zeros_like(f) for f in fs]
+            stats = batch.cg(fs, us, eps, 5000)
+        finally:
+            batch.close()
+        for k, (c, op, var) in enumerate(probs):
+            f = dev(rhs[k])
+            u = torch.zeros_like(f)
+            st1 = c.cg(0, op, var, f, u, eps[k], 5000)
+            name = cases[k][0] if k < len(cases) else "cube2_n8 pressure"
+            print(f"[gpu] batch {name}: {stats[k].iterations} its "
+                  f"(single {st1.iterations})")
+            assert stats[k].converged
+            assert abs(stats[k].iterations - st1.iterations) <= 1
+            assert rel(host(us[k]), host(u)) <= 1e-8
+            if k < len(cases):
+                Z = cases[k][1]
+                assert abs(stats[k].iterations - int(Z["ref_scalars"][1])) <= 1
+                assert rel(host(us[k]), Z["ref_cg_helm"]) <= 1e-8
+    finally:
+        for c in ctxs:
+            c.close()
+
+
+def test_batch_graph_replays_match_eager_launches():
+    """the captured-graph iterations give bit for bit what eager launches
+    give, replay after replay, with the program's own eager work and
+    synchronisations in between; one context three times (three velocity
+    components with different right-hand sides)"""
+    Z = golden("channel_n6")
+    with make_ctx(golden_mesh(Z)) as ctx:
+        rng = np.random.default_rng(11)
+        rhs = [Z["in_f"]] + [rng.standard_normal(ctx.ntot(0)) for _ in
+                             range(2)]
+        for r in rhs[1:]:
+            t = dev(r)
+            ctx.dss(0, t)
+            r[:] = host(t)
+        probs = [(ctx, S.OP_HELMHOLTZ, v) for v in (S.X_VELOCITY,
+                                                     S.Y_VELOCITY,
+                                                     S.Z_VELOCITY)]
+        eps = [1e-10 * np.linalg.norm(r) for r in rhs]
+        batch = S.Batch(probs)
+        try:
+            res = []
+            for use_graph in (False, True, True, True):
+                fs = [dev(r) for r in rhs]
+                us = [torch.zeros_like(f) for f in fs]
+                st = batch.cg(fs, us, eps, 5000, use_graph=use_graph)
+                res.append(([s.iterations for s in st],
+                            [host(u) for u in us]))
+                junk = torch.randn(1 << 20, device="cuda").sum()   # eager
+                torch.cuda.synchronize()
+                del junk
+        finally:
+            batch.close()
+        its0, sol0 = res[0]
+        print(f"[gpu] batch graph replays: iterations {its0}")
+        for its, sol in res[1:]:
+            assert its == its0
+            assert all(np.array_equal(a, b) for a, b in zip(sol, sol0))
