@@ -180,6 +180,10 @@ int ssx_set_diag_choice(ssx_ctx *ctx, int choice);
  * co-resident).  Same algorithm and stopping test; inner products are
  * summed in a different fixed order. */
 #define SSX_OPT_PERSISTENT_CG 3
+/* SSX_OPT_LP_COARSE_RTOL: k in [1,15]; with a reduced-precision smoother
+ * the FP64 coarsest solve stops at max(eps, 10^-k |f|_w) (default k = 3):
+ * its correction feeds FP32 levels, so digits beyond theirs are lost. */
+#define SSX_OPT_LP_COARSE_RTOL 4
 #define SSX_PREC_FP64 0
 #define SSX_PREC_FP32 1
 #define SSX_PREC_BF16 2

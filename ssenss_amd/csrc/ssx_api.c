@@ -370,6 +370,7 @@ int ssx_create(ssx_ctx **out, const ssx_mesh *m)
 	c->diag_choice = -1;
 	c->fused = 1;
 	c->persist = 0;
+	c->lp_coarse_rtol = pow(10.0, -SSX_LP_COARSE_RTOL_EXP);
 	c->lambda = (11.0 / 6.0) / (m->dt * m->visc);
 	int nl = SSX_LMAX;
 	for (int i = 0; i < SSX_LMAX; ++i) {     /* precon.c:62-70 */
@@ -577,6 +578,14 @@ int ssx_set_option(ssx_ctx *c, int option, int value)
 	switch (option) {
 	case SSX_OPT_FUSED_CG:
 		c->fused = value != 0;
+		return SSX_OK;
+	case SSX_OPT_LP_COARSE_RTOL:
+		if (value < 1 || value > 15) {
+			ssx_set_error("coarse tolerance exponent %d out of [1,15]",
+					value);
+			return SSX_EINVAL;
+		}
+		c->lp_coarse_rtol = pow(10.0, -value);
 		return SSX_OK;
 	case SSX_OPT_PERSISTENT_CG:
 		if (value < 0 || value > 1024) {
@@ -1167,13 +1176,13 @@ static int mg_recurse_lp(ssx_ctx *c, int l, int op, int var, double eps)
 		int it;
 		/* Right-hand side: FP64 and DSS-averaged by the caller.  The
 		 * correction is handed to FP32 levels, so the solve stops at
-		 * max(eps, SSX_LP_COARSE_RTOL * |f|_w): below that its digits are
+		 * max(eps, lp_coarse_rtol * |f|_w): below that its digits are
 		 * lost in the conversion, and FP32 rounding of the fine residual
 		 * leaves a component in the null space of singular problems
 		 * (periodic velocity) that CG cannot remove. */
 		TRY(ssxk_fill(st, L->mg_e, 0.0, L->ntot));
 		TRY(cg_level_fused(c, l, op, var, L->mg_fn, L->mg_e, eps,
-				SSX_LP_COARSE_RTOL, 500, 1, &it, &err));
+				c->lp_coarse_rtol, 500, 1, &it, &err));
 		c->cnt[1] += it;
 		return ssxk_cvt(st, SSX_PREC_FP64, SSX_PREC_FP32, L->mg_e, L->e_lp,
 				L->ntot);

@@ -20,8 +20,8 @@ extern "C" {
 #define SSX_NSCAL 256        /* device scalar slots                        */
 #define SSX_FCG_M 10         /* FCG history length (solver.c:90-92)        */
 /* mixed-precision V-cycle: the FP64 coarsest solve stops at
- * max(eps, SSX_LP_COARSE_RTOL * |f|_w), 16 FP32 rounding units */
-#define SSX_LP_COARSE_RTOL 1e-6
+ * max(eps, 10^-k * |f|_w), default k = 3 */
+#define SSX_LP_COARSE_RTOL_EXP 3
 
 /* device scalar slots */
 enum {
@@ -88,6 +88,7 @@ struct ssx_ctx {
 	int diag_choice;
 	int fused;                   /* SSX_OPT_FUSED_CG                     */
 	int smooth_prec;             /* SSX_OPT_SMOOTHER_PRECISION           */
+	double lp_coarse_rtol;       /* SSX_OPT_LP_COARSE_RTOL               */
 	double *inv_mass;            /* device, level 0 */
 	void *stream;                /* hipStream_t in use */
 	void *own_stream;
