@@ -184,6 +184,13 @@ int ssx_set_diag_choice(ssx_ctx *ctx, int choice);
  * the FP64 coarsest solve stops at max(eps, 10^-k |f|_w) (default k = 3):
  * its correction feeds FP32 levels, so digits beyond theirs are lost. */
 #define SSX_OPT_LP_COARSE_RTOL 4
+/* SSX_OPT_FUSED_REDUCE: 1 lets the last workgroup of the DSS and of the
+ * update kernel form alpha and beta (three launches per fused CG
+ * iteration); 0 (default) uses separate single-workgroup reductions
+ * (five).  Same summation order: bit-identical results.  Off by default:
+ * every workgroup's device-scope release (an L2 write-back per XCD) costs
+ * more than the two launches it saves. */
+#define SSX_OPT_FUSED_REDUCE 5
 #define SSX_PREC_FP64 0
 #define SSX_PREC_FP32 1
 #define SSX_PREC_BF16 2

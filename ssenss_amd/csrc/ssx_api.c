@@ -370,6 +370,7 @@ int ssx_create(ssx_ctx **out, const ssx_mesh *m)
 	c->diag_choice = -1;
 	c->fused = 1;
 	c->persist = 0;
+	c->fused_red = 0;
 	c->lp_coarse_rtol = pow(10.0, -SSX_LP_COARSE_RTOL_EXP);
 	c->lambda = (11.0 / 6.0) / (m->dt * m->visc);
 	int nl = SSX_LMAX;
@@ -455,6 +456,10 @@ int ssx_create(ssx_ctx **out, const ssx_mesh *m)
 		if (!rc && hipMalloc((void **) &c->pbar, 4 * sizeof(unsigned))
 				!= hipSuccess)
 			rc = SSX_ENOMEM;
+		if (!rc && (hipMalloc((void **) &c->redctr, 4 * sizeof(unsigned))
+				!= hipSuccess || hipMemset(c->redctr, 0,
+				4 * sizeof(unsigned)) != hipSuccess))
+			rc = SSX_ENOMEM;
 		if (!rc && hipMalloc((void **) &c->cg_flags, sizeof(int) * 2 * SSX_LMAX)
 				!= hipSuccess) {
 			rc = SSX_ENOMEM;
@@ -510,6 +515,7 @@ void ssx_destroy(ssx_ctx *c)
 	hipFree(c->inv_mass);
 	hipFree(c->cg_flags);
 	hipFree(c->pbar);
+	hipFree(c->redctr);
 	if (c->h_flags)
 		hipHostFree(c->h_flags);
 	hipFree(c->scal);
@@ -578,6 +584,9 @@ int ssx_set_option(ssx_ctx *c, int option, int value)
 	switch (option) {
 	case SSX_OPT_FUSED_CG:
 		c->fused = value != 0;
+		return SSX_OK;
+	case SSX_OPT_FUSED_REDUCE:
+		c->fused_red = value != 0;
 		return SSX_OK;
 	case SSX_OPT_LP_COARSE_RTOL:
 		if (value < 1 || value > 15) {
@@ -840,10 +849,12 @@ static int cg_level_plain(ssx_ctx *c, int l, int op, int var, double *f,
 }
 
 /*
- * The same iteration with six launches: Ax with <p,Ap> over unshared nodes,
- * DSS with the shared part, a reduce that forms alpha, the fused update
- * (u, r, z and the <r,z>, <r,r> partials), a reduce that forms beta and the
- * stopping decision, and the p update.  The stopping test of solver.c:73-74
+ * The same iteration fused: Ax (from the second iteration on with the
+ * direction update p = (z - c) + beta p of the previous one) and the sum of
+ * p (A_e p) over all local copies, which for p equal on the copies of each
+ * shared node is <p, DSS(Ap)>_w; the DSS; a reduce that forms alpha; the
+ * fused update (u, r, z and the <r,z>, <r,r> partials); a reduce that
+ * forms beta and the stopping decision.  The stopping test of solver.c:73-74
  * runs on the device: once it fires every later kernel of the solve returns
  * at once, so the host enqueues iterations in chunks and synchronises once
  * per chunk.  x_zero: the initial guess is known to be zero (A*0 = 0 is
@@ -911,6 +922,10 @@ static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
 	} else {
 	const long long nba = ssxk_ax_blocks(L->n, c->nel, L->axv[0]);
 	const long long nbd = ssxk_dss_blocks(L->ngrp);
+	const ssx_lastred lr0 = { c->redctr, c->part, nba, c->scal, o,
+		max_iter, pres, flags, eps_dev, (double) L->nunique };
+	const ssx_lastred lr1 = { c->redctr + 1, c->part, SSX_NPART, c->scal, o,
+		max_iter, pres, flags, eps_dev, (double) L->nunique };
 	int launched = 0;
 	int chunk = c->cg_hint[l] > 0 ? c->cg_hint[l] + 1 : 8;
 	for (;;) {
@@ -929,9 +944,15 @@ static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
 						o + CG_BETA, pres ? o + CG_NULLC : -1, ap, L->gf,
 						L->mass, L->dmat, c->lambda, L->bc, !pres, flags,
 						c->part));
-			TRY(ssxk_dss_all(st, L->grp, L->ngrp, ap, p, c->part + nba, 0,
-					flags));
-			TRY(ssxk_cg_reduce(st, 0, c->part, nba + nbd, c->scal, o, flags,
+			if (c->fused_red && nbd > 0) {
+				TRY(ssxk_dss_all_red(st, L->grp, L->ngrp, ap, NULL, NULL,
+						flags, &lr0));
+				TRY(ssxk_cg_update_red(st, nt, c->scal, o + CG_ALPHA, p, ap,
+						u, r, z, dinv, L->scale, pres, c->part, flags, &lr1));
+				continue;
+			}
+			TRY(ssxk_dss_all(st, L->grp, L->ngrp, ap, NULL, NULL, 0, flags));
+			TRY(ssxk_cg_reduce(st, 0, c->part, nba, c->scal, o, flags,
 					eps_dev, max_iter, L->nunique, pres));
 			TRY(ssxk_cg_update(st, nt, c->scal, o + CG_ALPHA, p, ap, u, r, z,
 					dinv, L->scale, pres, c->part, flags));
@@ -1146,10 +1167,20 @@ static int cg_level_lp(ssx_ctx *c, int l, int op, int var, float *f,
 					c->scal, o + CG_BETA, pres ? o + CG_NULLC : -1, ap,
 					L->gf_lp, L->mass_lp, L->dmat_lp, c->lambda, L->bc, !pres,
 					flags, c->part));
-		TRY(ssxk_dss_all_lp(st, L->grp, L->ngrp, ap, p, c->part + nba, 0,
-				flags));
-		TRY(ssxk_cg_reduce(st, 0, c->part, nba + ssxk_dss_blocks(L->ngrp),
-				c->scal, o, flags, eps, max_iter, L->nunique, pres));
+		if (c->fused_red && ssxk_dss_blocks(L->ngrp) > 0) {
+			const ssx_lastred lr0 = { c->redctr, c->part, nba, c->scal, o,
+				max_iter, pres, flags, eps, (double) L->nunique };
+			const ssx_lastred lr1 = { c->redctr + 1, c->part, SSX_NPART,
+				c->scal, o, max_iter, pres, flags, eps, (double) L->nunique };
+			TRY(ssxk_dss_all_red_lp(st, L->grp, L->ngrp, ap, NULL, NULL,
+					flags, &lr0));
+			TRY(ssxk_cg_update_red_lp(st, nt, c->scal, o + CG_ALPHA, p, ap, u,
+					r, z, dinv, L->scale_lp, pres, c->part, flags, &lr1));
+			continue;
+		}
+		TRY(ssxk_dss_all_lp(st, L->grp, L->ngrp, ap, NULL, NULL, 0, flags));
+		TRY(ssxk_cg_reduce(st, 0, c->part, nba, c->scal, o, flags, eps,
+				max_iter, L->nunique, pres));
 		TRY(ssxk_cg_update_lp(st, nt, c->scal, o + CG_ALPHA, p, ap, u, r, z,
 				dinv, L->scale_lp, pres, c->part, flags));
 		TRY(ssxk_cg_reduce(st, 1, c->part, 0, c->scal, o, flags, eps,

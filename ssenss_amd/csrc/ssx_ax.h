@@ -56,7 +56,7 @@ struct PUpd {
 // and the operator is applied to the new p (the k_cg_pupdate arithmetic);
 // u is not read.
 // One workgroup's elements [eblk*TEAMS, eblk*TEAMS + TEAMS): Au (masked)
-// written, and each lane's part of <p,Ap> over unshared nodes returned.
+// written, and each lane's part of sum p (A_e p) over its nodes returned.
 template <int N, int OP, typename T, typename G, bool PUPD, int V>
 __device__ __forceinline__ double ax_team(int eblk, int nel,
 		const T *__restrict__ u, T *__restrict__ au,
@@ -183,8 +183,6 @@ __device__ __forceinline__ double ax_team(int eblk, int nel,
 			acc[kk] += (kk == k ? r : T(0)) + gDs[k * N + kk] * t2;
 		__syncthreads();
 	}
-	const bool si = ((eb & 256) && i == 0) || ((eb & 512) && i == N - 1)
-		|| ((eb & 1024) && j == 0) || ((eb & 2048) && j == N - 1);
 	double pd = 0.0;
 	if (act) {
 #pragma unroll
@@ -200,12 +198,10 @@ __device__ __forceinline__ double ax_team(int eblk, int nel,
 				__builtin_nontemporal_store(v, au + eoff + q);
 			else
 				au[eoff + q] = v;
-			// <p, Ap>_w over nodes no other element shares (weight 1);
-			// the shared ones are added by the DSS kernel after assembly
-			const bool sh = si || ((eb & 4096) && k == 0)
-				|| ((eb & 8192) && k == N - 1);
-			if (!sh)
-				pd += (double) uc[k] * (double) v;
+			// <p, DSS(Ap)>_w: for p equal on the copies of each shared
+			// node, sum_w p DSS(Ap) is the sum over all local copies of
+			// p (A_e p), so it is complete before the assembly
+			pd += (double) uc[k] * (double) v;
 		}
 	}
 	return pd;

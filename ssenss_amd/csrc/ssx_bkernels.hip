@@ -49,8 +49,8 @@ k_bax(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 	ax_block_partial(pd, d->part, w.blk);
 }
 
-// DSS of Ap with the shared part of <p, Ap>_w; local workgroups number the
-// pairs, then the quadruples, then the octuples of the problem
+// DSS of Ap; local workgroups number the pairs, then the quadruples, then
+// the octuples of the problem
 __global__ void __launch_bounds__(256)
 k_bdss(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 {
@@ -60,21 +60,19 @@ k_bdss(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 		return;
 	const long long b2 = (d->G2 + 255) / 256, b4 = (d->G4 + 255) / 256;
 	const long long b = w.blk;
-	double pd = 0.0;
 	if (b < b2) {
 		const long long g = b * 256 + threadIdx.x;
 		if (g < d->G2)
-			pd = dss_group<2, double>(d->G2, g, d->g2, d->ap, d->p, 0);
+			dss_group<2, double>(d->G2, g, d->g2, d->ap, nullptr, 0);
 	} else if (b < b2 + b4) {
 		const long long g = (b - b2) * 256 + threadIdx.x;
 		if (g < d->G4)
-			pd = dss_group<4, double>(d->G4, g, d->g4, d->ap, d->p, 0);
+			dss_group<4, double>(d->G4, g, d->g4, d->ap, nullptr, 0);
 	} else {
 		const long long g = (b - b2 - b4) * 256 + threadIdx.x;
 		if (g < d->G8)
-			pd = dss_group<8, double>(d->G8, g, d->g8, d->ap, d->p, 0);
+			dss_group<8, double>(d->G8, g, d->g8, d->ap, nullptr, 0);
 	}
-	ax_block_partial(pd, d->part, d->nba + b);
 }
 
 // u += alpha p; r -= alpha Ap; z = D^-1 r; partials of <r,z>_w, <r,r>_w,
@@ -122,7 +120,8 @@ k_bupdate(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 	}
 }
 
-// one workgroup per problem.  mode 0: <p,Ap>_w -> alpha.  mode 1: the
+// one workgroup per problem.  mode 0: <p,Ap>_w (the operator partials)
+// -> alpha.  mode 1: the
 // null-space constant, rdz, beta, the iteration count and the stopping
 // test sqrt(<r,r>_w) <= eps or count >= maxit (or NaN), as k_cg_reduce
 __global__ void __launch_bounds__(1024)
@@ -134,7 +133,7 @@ k_breduce(int mode, const ssx_bprob *__restrict__ pr)
 	__shared__ double sh[4][1024];
 	const int nq = mode == 0 ? 1 : (d->pressure ? 4 : 2);
 	const double *base = d->part + (mode == 0 ? 0 : d->nba + d->nbd);
-	const long long m = mode == 0 ? d->nba + d->nbd : d->nbu;
+	const long long m = mode == 0 ? d->nba : d->nbu;
 	for (int c = 0; c < 4; ++c) {
 		double acc = 0.0;
 		if (c < nq)

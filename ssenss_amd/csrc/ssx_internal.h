@@ -99,6 +99,8 @@ struct ssx_ctx {
 	int *h_flags;                /* pinned host mirror                   */
 	int cg_hint[SSX_LMAX];       /* last iteration count per level       */
 	int persist;                 /* SSX_OPT_PERSISTENT_CG                */
+	int fused_red;               /* SSX_OPT_FUSED_REDUCE                 */
+	unsigned *redctr;            /* device: last-workgroup counter       */
 	int persist_grid;            /* blocks of the cooperative CG launch  */
 	unsigned *pbar;              /* device: grid-barrier words [4]       */
 	size_t npart;                /* doubles in part                      */
@@ -164,6 +166,34 @@ int ssxk_cg_update(void *stream, size_t n, const double *scal, int ia,
 int ssxk_cg_pupdate(void *stream, size_t n, const double *z, double *p,
 		const double *scal, int ib, int ic, const double *mask,
 		const int *done);
+/* The two reductions of a fused CG iteration done by the last workgroup
+ * of the DSS kernel (mode 0: <p,Ap>_w -> alpha) and of the update kernel
+ * (mode 1: beta, null-space constant, stopping test) instead of two
+ * single-workgroup launches; sums in the order of ssxk_cg_reduce, so the
+ * results are the same bit for bit.  counter: zero between launches. */
+typedef struct ssx_lastred {
+	unsigned *counter;
+	const double *part;      /* mode 0: [0, cnt); mode 1: c*SSX_NPART + q */
+	long long cnt;
+	double *scal;
+	int o, maxit, pressure;
+	int *flags;
+	double eps, nunique;
+} ssx_lastred;
+int ssxk_dss_all_red(void *stream, int *const grp[3], const long long ng[3],
+		double *u, const double *p, double *part, const int *done,
+		const ssx_lastred *lr);
+int ssxk_dss_all_red_lp(void *stream, int *const grp[3],
+		const long long ng[3], void *u, const void *p, double *part,
+		const int *done, const ssx_lastred *lr);
+int ssxk_cg_update_red(void *stream, size_t n, const double *scal, int ia,
+		const double *p, const double *ap, double *u, double *r, double *z,
+		const double *dinv, const double *scale, int pressure, double *part,
+		const int *done, const ssx_lastred *lr);
+int ssxk_cg_update_red_lp(void *stream, size_t n, const double *scal, int ia,
+		const void *p, const void *ap, void *u, void *r, void *z,
+		const void *dinv, const void *scale, int pressure, double *part,
+		const int *done, const ssx_lastred *lr);
 /* eps < 0: the stopping tolerance is read from scal[o + CG_TOL] */
 int ssxk_cg_reduce(void *stream, int mode, const double *part, long long cnt,
 		double *scal, int o, int *flags, double eps, int maxit,

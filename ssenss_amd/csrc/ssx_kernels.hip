@@ -120,17 +120,101 @@ extern "C" int ssxk_ax_pupd_lp(void *stream, int prec, int n, int v, int nel,
 }
 
 // ---------------------------------------------------------------------
+// CG scalar logic shared by k_cg_reduce and the last-workgroup reductions
+// ---------------------------------------------------------------------
+
+// mode 0: s[o+PAP] = pap; s[o+ALPHA] = s[o+RDZ] / pap
+__device__ __forceinline__ void cg_scalars0(double pap, double *scal, int o)
+{
+	scal[o + CG_PAP] = pap;
+	scal[o + CG_ALPHA] = scal[o + CG_RDZ] / pap;
+}
+
+// mode 1: null-space constant c = zs/nunique; rdz = rz - c rs;
+// beta = rdz / rdz_old; iteration count ++, done when sqrt(rr) <= eps
+// (eps < 0: s[o+TOL]) or count >= maxit or NaN
+__device__ __forceinline__ void cg_scalars1(double rz, double rr, double zs,
+		double rs, double *scal, int o, int *flags, double eps, int maxit,
+		double nunique, int pressure)
+{
+	double c = 0.0;
+	if (pressure) {
+		c = zs / nunique;
+		rz = rz - c * rs;
+	}
+	scal[o + CG_NULLC] = c;
+	scal[o + CG_BETA] = rz / scal[o + CG_RDZ];
+	scal[o + CG_RDZ] = rz;
+	scal[o + CG_RR] = rr;
+	const int it = flags[1] + 1;
+	flags[1] = it;
+	const double tol = eps >= 0.0 ? eps : scal[o + CG_TOL];
+	if (sqrt(rr) <= tol || it >= maxit || !(rr == rr))
+		flags[0] = 1;
+}
+
+// In a 256-lane workgroup, the sum of pp[0..m) in exactly the order of a
+// 1024-lane k_cg_reduce: lane t holds the accumulators of virtual lanes
+// t, t+256, t+512, t+768 and folds them as the first two tree levels do.
+__device__ double sum_as_1024(const double *pp, long long m, double *sh)
+{
+	double a[4];
+#pragma unroll
+	for (int v = 0; v < 4; ++v) {
+		double acc = 0.0;
+		for (long long q = threadIdx.x + 256 * v; q < m; q += 1024)
+			acc += __hip_atomic_load(pp + q, __ATOMIC_RELAXED,
+					__HIP_MEMORY_SCOPE_AGENT);
+		a[v] = acc;
+	}
+	a[0] += a[2];
+	a[1] += a[3];
+	a[0] += a[1];
+	sh[threadIdx.x] = a[0];
+	__syncthreads();
+	for (int st = 128; st > 0; st >>= 1) {
+		if ((int) threadIdx.x < st)
+			sh[threadIdx.x] += sh[threadIdx.x + st];
+		__syncthreads();
+	}
+	const double t = sh[0];
+	__syncthreads();
+	return t;
+}
+
+// After thread 0 stored this workgroup's partials: true in every lane of
+// the workgroup that arrives last (its partials and everyone else's are
+// then visible to it).  The last one resets the counter.
+__device__ bool last_workgroup(unsigned *counter)
+{
+	__shared__ int s_last;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		__threadfence();
+		const unsigned prev = atomicAdd(counter, 1u);
+		s_last = prev == gridDim.x - 1;
+		if (s_last) {
+			atomicExch(counter, 0u);
+			__threadfence();
+		}
+	}
+	__syncthreads();
+	return s_last != 0;
+}
+
+// ---------------------------------------------------------------------
 // Direct stiffness summation over shared-node groups
 // ---------------------------------------------------------------------
 
 // One launch for all three multiplicities: blocks [0,b2) take the pairs,
 // [b2,b2+b4) the quadruples, the rest the octuples.
-template <typename T>
+template <typename T, bool RED>
 __global__ void __launch_bounds__(256)
 k_dss_all(long long G2, long long G4, long long G8, const int *__restrict__ i2,
 		const int *__restrict__ i4, const int *__restrict__ i8,
 		T *__restrict__ u, const T *__restrict__ p,
-		double *__restrict__ part, int average, const int *__restrict__ done)
+		double *__restrict__ part, int average, const int *__restrict__ done,
+		ssx_lastred lr)
 {
 	if (done && *done)
 		return;
@@ -163,6 +247,12 @@ k_dss_all(long long G2, long long G4, long long G8, const int *__restrict__ i2,
 		if (threadIdx.x == 0)
 			part[blockIdx.x] = sh[0];
 	}
+	if (RED && last_workgroup(lr.counter)) {
+		__shared__ double sr[256];
+		const double pap = sum_as_1024(lr.part, lr.cnt, sr);
+		if (threadIdx.x == 0)
+			cg_scalars0(pap, lr.scal, lr.o);
+	}
 }
 
 extern "C" long long ssxk_dss_blocks(const long long ng[3])
@@ -172,16 +262,38 @@ extern "C" long long ssxk_dss_blocks(const long long ng[3])
 
 template <typename T>
 static int launch_dss(hipStream_t st, int *const grp[3], const long long ng[3],
-		void *u, const void *p, double *part, int average, const int *done)
+		void *u, const void *p, double *part, int average, const int *done,
+		const ssx_lastred *lr = nullptr)
 {
 	long long nb = ssxk_dss_blocks(ng);
 	if (nb == 0)
 		return SSX_OK;
-	hipLaunchKernelGGL(k_dss_all<T>, dim3((unsigned) nb), dim3(256), 0, st,
-			ng[0], ng[1], ng[2], grp[0], grp[1], grp[2], (T *) u,
-			(const T *) p, part, average, done);
+	if (lr)
+		hipLaunchKernelGGL((k_dss_all<T, true>), dim3((unsigned) nb),
+				dim3(256), 0, st, ng[0], ng[1], ng[2], grp[0], grp[1], grp[2],
+				(T *) u, (const T *) p, part, average, done, *lr);
+	else
+		hipLaunchKernelGGL((k_dss_all<T, false>), dim3((unsigned) nb),
+				dim3(256), 0, st, ng[0], ng[1], ng[2], grp[0], grp[1], grp[2],
+				(T *) u, (const T *) p, part, average, done, ssx_lastred{});
 	LAUNCH_CHECK("k_dss_all");
 	return SSX_OK;
+}
+
+extern "C" int ssxk_dss_all_red(void *stream, int *const grp[3],
+		const long long ng[3], double *u, const double *p, double *part,
+		const int *done, const ssx_lastred *lr)
+{
+	return launch_dss<double>((hipStream_t) stream, grp, ng, u, p, part, 0,
+			done, lr);
+}
+
+extern "C" int ssxk_dss_all_red_lp(void *stream, int *const grp[3],
+		const long long ng[3], void *u, const void *p, double *part,
+		const int *done, const ssx_lastred *lr)
+{
+	return launch_dss<float>((hipStream_t) stream, grp, ng, u, p, part, 0,
+			done, lr);
 }
 
 extern "C" int ssxk_dss_all(void *stream, int *const grp[3],
@@ -223,13 +335,14 @@ __device__ __forceinline__ void block_sum4(double *v, double (*sh)[256])
 
 // u += a p; r -= a ap; z = dinv r; partial <r,z>_w, <r,r>_w and, for the
 // pressure null space, <z,1>_w and <r,1>_w (FP64 accumulation)
-template <typename T>
+template <typename T, bool RED>
 __global__ void __launch_bounds__(256)
 k_cg_update(size_t n, const double *__restrict__ scal, int ia,
 		const T *__restrict__ p, const T *__restrict__ ap,
 		T *__restrict__ u, T *__restrict__ r, T *__restrict__ z,
 		const T *__restrict__ dinv, const T *__restrict__ scale, int pressure,
-		double *__restrict__ part, const int *__restrict__ done)
+		double *__restrict__ part, const int *__restrict__ done,
+		ssx_lastred lr)
 {
 	if (done && *done)
 		return;
@@ -253,6 +366,19 @@ k_cg_update(size_t n, const double *__restrict__ scal, int ia,
 	block_sum4(acc, sh);
 	if (threadIdx.x < 4)
 		part[(size_t) threadIdx.x * SSX_NPART + blockIdx.x] = sh[threadIdx.x][0];
+	if (RED && last_workgroup(lr.counter)) {
+		// the last workgroup: every lane's stores of all four partials
+		// happened before the counter (threads 1-3 included: the fence in
+		// last_workgroup follows the block barrier)
+		double v[4] = { 0.0, 0.0, 0.0, 0.0 };
+		const int nq = lr.pressure ? 4 : 2;
+		for (int c = 0; c < nq; ++c)
+			v[c] = sum_as_1024(lr.part + (size_t) c * SSX_NPART, SSX_NPART,
+					sh[0]);
+		if (threadIdx.x == 0)
+			cg_scalars1(v[0], v[1], v[2], v[3], lr.scal, lr.o, lr.flags,
+					lr.eps, lr.maxit, lr.nunique, lr.pressure);
+	}
 }
 
 // p = (z - c) + beta p, masked (c: pressure null-space constant)
@@ -308,25 +434,11 @@ k_cg_reduce(int mode, const double *__restrict__ part, long long cnt,
 	}
 	if (threadIdx.x != 0)
 		return;
-	if (mode == 0) {
-		scal[o + CG_PAP] = sh[0][0];
-		scal[o + CG_ALPHA] = scal[o + CG_RDZ] / sh[0][0];
-		return;
-	}
-	double rz = sh[0][0], rr = sh[1][0], c = 0.0;
-	if (pressure) {
-		c = sh[2][0] / nunique;
-		rz = rz - c * sh[3][0];
-	}
-	scal[o + CG_NULLC] = c;
-	scal[o + CG_BETA] = rz / scal[o + CG_RDZ];
-	scal[o + CG_RDZ] = rz;
-	scal[o + CG_RR] = rr;
-	const int it = flags[1] + 1;
-	flags[1] = it;
-	const double tol = eps >= 0.0 ? eps : scal[o + CG_TOL];
-	if (sqrt(rr) <= tol || it >= maxit || !(rr == rr))
-		flags[0] = 1;
+	if (mode == 0)
+		cg_scalars0(sh[0][0], scal, o);
+	else
+		cg_scalars1(sh[0][0], sh[1][0], sh[2][0], sh[3][0], scal, o, flags,
+				eps, maxit, nunique, pressure);
 }
 
 __global__ void k_cg_tol(double *scal, int dst, int src, double eps,
@@ -349,14 +461,38 @@ template <typename T>
 static int launch_cg_update(hipStream_t st, size_t n, const double *scal,
 		int ia, const void *p, const void *ap, void *u, void *r, void *z,
 		const void *dinv, const void *scale, int pressure, double *part,
-		const int *done)
+		const int *done, const ssx_lastred *lr = nullptr)
 {
-	hipLaunchKernelGGL(k_cg_update<T>, dim3(SSX_NPART), dim3(256), 0, st, n,
-			scal, ia, (const T *) p, (const T *) ap, (T *) u, (T *) r,
-			(T *) z, (const T *) dinv, (const T *) scale, pressure, part,
-			done);
+	if (lr)
+		hipLaunchKernelGGL((k_cg_update<T, true>), dim3(SSX_NPART), dim3(256),
+				0, st, n, scal, ia, (const T *) p, (const T *) ap, (T *) u,
+				(T *) r, (T *) z, (const T *) dinv, (const T *) scale,
+				pressure, part, done, *lr);
+	else
+		hipLaunchKernelGGL((k_cg_update<T, false>), dim3(SSX_NPART),
+				dim3(256), 0, st, n, scal, ia, (const T *) p, (const T *) ap,
+				(T *) u, (T *) r, (T *) z, (const T *) dinv,
+				(const T *) scale, pressure, part, done, ssx_lastred{});
 	LAUNCH_CHECK("k_cg_update");
 	return SSX_OK;
+}
+
+extern "C" int ssxk_cg_update_red(void *stream, size_t n, const double *scal,
+		int ia, const double *p, const double *ap, double *u, double *r,
+		double *z, const double *dinv, const double *scale, int pressure,
+		double *part, const int *done, const ssx_lastred *lr)
+{
+	return launch_cg_update<double>((hipStream_t) stream, n, scal, ia, p, ap,
+			u, r, z, dinv, scale, pressure, part, done, lr);
+}
+
+extern "C" int ssxk_cg_update_red_lp(void *stream, size_t n,
+		const double *scal, int ia, const void *p, const void *ap, void *u,
+		void *r, void *z, const void *dinv, const void *scale, int pressure,
+		double *part, const int *done, const ssx_lastred *lr)
+{
+	return launch_cg_update<float>((hipStream_t) stream, n, scal, ia, p, ap,
+			u, r, z, dinv, scale, pressure, part, done, lr);
 }
 
 template <typename T>

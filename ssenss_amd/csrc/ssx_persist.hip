@@ -8,12 +8,11 @@
 // the iteration loop and the phases are separated by grid-wide barriers:
 //
 //   A  p = (z - c) + beta p (masked), Ap_e = A_e p per element, mask,
-//      <p,Ap> over nodes of multiplicity one          -> part[blk]
+//      sum of p Ap_e over the local copies (= <p, DSS(Ap)>_w) -> part[blk]
 //   -- barrier
-//   B  DSS of Ap over the shared-node groups (the reference's add order),
-//      shared-node part of <p,Ap>_w                    -> part[G + blk]
+//   B  DSS of Ap over the shared-node groups (the reference's add order)
 //   -- barrier
-//   C  every block sums the 2G partials in the same fixed order (so all
+//   C  every block sums the G partials in the same fixed order (so all
 //      blocks hold the same alpha), u += alpha p, r -= alpha Ap, z = D^-1 r,
 //      partials of <r,z>_w, <r,r>_w (and <z,1>_w, <r,1>_w for the pressure
 //      null space)                                      -> part[2G + 4 blk]
@@ -140,7 +139,7 @@ k_cg_persist(PersistArgs a)
 	const bool lane_ok = le < EPB;
 	const int ngroups = (a.nel + EPB - 1) / EPB;
 	const size_t nt = (size_t) a.nel * N3;
-	double *partA = a.part, *partB = a.part + G, *partC = a.part + 2 * G;
+	double *partA = a.part, *partC = a.part + 2 * G;
 
 	double rdz = a.scal[a.o + CG_RDZ];
 	const double tol = a.eps >= 0.0 ? a.eps : a.scal[a.o + CG_TOL];
@@ -213,12 +212,7 @@ k_cg_persist(PersistArgs a)
 					v += a.lambda * (a.mass[q] * pv);
 				v = m ? 0.0 : v;
 				a.ap[q] = v;
-				const bool shd = ((eb & 256) && i == 0)
-					|| ((eb & 512) && i == N - 1)
-					|| ((eb & 1024) && j == 0) || ((eb & 2048) && j == N - 1)
-					|| ((eb & 4096) && k == 0) || ((eb & 8192) && k == N - 1);
-				if (!shd)
-					pd += pv * v;
+				pd += pv * v;       // all local copies: <p, DSS(Ap)>_w
 			}
 			__syncthreads();
 		}
@@ -228,28 +222,23 @@ k_cg_persist(PersistArgs a)
 		if (!(ok = grid_sync(a.bar, G, phase)))
 			break;
 
-		// ---- B: DSS of Ap, shared part of <p,Ap>_w
-		pd = 0.0;
+		// ---- B: DSS of Ap
 		const long long ng = a.G2 + a.G4 + a.G8;
 		for (long long t = (long long) blockIdx.x * PB + threadIdx.x; t < ng;
 				t += (long long) G * PB) {
 			if (t < a.G2)
-				pd += dss_group<2, double>(a.G2, t, a.g2, a.ap, a.p, 0);
+				dss_group<2, double>(a.G2, t, a.g2, a.ap, nullptr, 0);
 			else if (t < a.G2 + a.G4)
-				pd += dss_group<4, double>(a.G4, t - a.G2, a.g4, a.ap, a.p,
-						0);
+				dss_group<4, double>(a.G4, t - a.G2, a.g4, a.ap, nullptr, 0);
 			else
-				pd += dss_group<8, double>(a.G8, t - a.G2 - a.G4, a.g8, a.ap,
-						a.p, 0);
+				dss_group<8, double>(a.G8, t - a.G2 - a.G4, a.g8, a.ap,
+						nullptr, 0);
 		}
-		bs = block_sum(pd, sh);
-		if (threadIdx.x == 0)
-			partB[blockIdx.x] = bs;
 		if (!(ok = grid_sync(a.bar, G, phase)))
 			break;
 
 		// ---- C: alpha, vector update, partials of the next inner products
-		pap = sum_parts(a.part, 2 * G, sh);
+		pap = sum_parts(a.part, G, sh);
 		const double alpha = rdz / pap;
 		double acc[4] = { 0.0, 0.0, 0.0, 0.0 };
 		for (size_t q = (size_t) blockIdx.x * PB + threadIdx.x; q < nt;

@@ -648,3 +648,36 @@ def test_batch_graph_replays_match_eager_launches():
         for its, sol in res[1:]:
             assert its == its0
             assert all(np.array_equal(a, b) for a, b in zip(sol, sol0))
+
+
+@pytest.mark.parametrize("name", ["cube2_n8", "periodic3_n6", "channel_n6"])
+def test_last_workgroup_reductions_bit_identical(name):
+    """alpha and beta formed by the last workgroup of the DSS and update
+    kernels (three launches per CG iteration) give bit for bit what the
+    separate reduction launches give: Jacobi-PCG Helmholtz, FCG + p-MG
+    pressure, and the FP32 mixed-precision solve"""
+    Z = golden(name)
+    eps = Z["ref_scalars"][0]
+    out = {}
+    with make_ctx(golden_mesh(Z)) as ctx:
+        for fr in (0, 1):
+            ctx.set_option(S.OPT_FUSED_REDUCE, fr)
+            runs = []
+            f = dev(Z["in_f"])
+            u = torch.zeros_like(f)
+            st = ctx.cg(0, S.OP_HELMHOLTZ, S.X_VELOCITY, f, u, eps, 5000)
+            runs.append((st.iterations, host(u)))
+            for prec in (S.PREC_FP64, S.PREC_FP32):
+                ctx.set_option(S.OPT_SMOOTHER_PRECISION, prec)
+                f = dev(Z["in_f"])
+                v = torch.zeros_like(f)
+                st = ctx.fcg(S.OP_POISSON, S.PRESSURE, S.PC_MG, f, v, eps, 5000)
+                runs.append((st.iterations, host(v)))
+            ctx.set_option(S.OPT_SMOOTHER_PRECISION, S.PREC_FP64)
+            out[fr] = runs
+        ctx.set_option(S.OPT_FUSED_REDUCE, 0)
+    print(f"[gpu] last-workgroup reductions {name}: iterations "
+          f"{[r[0] for r in out[1]]} / {[r[0] for r in out[0]]}")
+    for a, b in zip(out[1], out[0]):
+        assert a[0] == b[0]
+        assert np.array_equal(a[1], b[1])
