@@ -151,8 +151,9 @@ def main():
                     help="skip the N = 5 / 7 / 11 mixed-precision sweep")
     ap.add_argument("--profile", action="store_true",
                     help="kernels only, for rocprofv3 counter runs: the "
-                    "Poisson Ax on the workload, then a 1 GiB device copy "
-                    "(known bytes, to calibrate FETCH_SIZE/WRITE_SIZE)")
+                    "Poisson Ax on the workload, then its DSS, then a 1 GiB "
+                    "device copy (known bytes, to calibrate FETCH_SIZE/"
+                    "WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this "
@@ -193,6 +194,8 @@ def main():
     if args.profile:
         for _ in range(args.steps):
             ctx.ax(0, S.OP_POISSON, u, au)
+        for _ in range(args.steps):
+            ctx.dss(0, au)
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
