@@ -109,6 +109,125 @@ def solve_case(S, torch, ctx, mesh, max_iter, precs=("fp64", "fp32", "bf16"),
     return out
 
 
+def particle_case(S, torch, stream, npart=1_000_000, seed=7):
+    """Inertial point particles (integrate_particle_equations) on the
+    channel of north-star configuration 5: 24x16x24 hexahedra at n = 10 on
+    [0,4pi] x [-1,1] x [0,4pi/3], tanh wall stretching 2.0, periodic in x
+    and z, walls in y.  Velocity: Poiseuille u = 1 - y^2 plus a seeded
+    divergence-free perturbation of 10% amplitude, set three times (the
+    3-level history).  npart particles at uniform random element and
+    reference position, velocity 0, response time tau ~ U[0.05, 0.5],
+    advanced over one step dt = 2e-3 (one element is about 0.2 long in x,
+    so few particles change element)."""
+    from ssenss_amd.mesh import box_mesh, PERIODIC, NOSLIP
+    mesh = box_mesh(10, (24, 16, 24), lo=(0.0, -1.0, 0.0),
+                    hi=(4 * np.pi, 1.0, 4 * np.pi / 3),
+                    bc=(PERIODIC, PERIODIC, NOSLIP, NOSLIP, PERIODIC,
+                        PERIODIC), deform=2, amp=2.0)
+    ctx = S.Context(mesh)
+    ctx.set_stream(stream)
+    X, Y, Z = (a.ravel() for a in (mesh.x, mesh.y, mesh.z))
+    kx, kz = 2 * np.pi / (4 * np.pi), 2 * np.pi / (4 * np.pi / 3)
+    pert = 0.1 * (1 - Y ** 2)
+    u = (1 - Y ** 2) + pert * np.cos(kx * X) * np.sin(kz * Z)
+    v = 0.1 * (1 - Y ** 2) ** 2 * np.sin(kx * X) * np.sin(kz * Z)
+    w = -pert * np.sin(kx * X) * np.cos(kz * Z)
+    uv = [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (u, v, w)]
+    for _ in range(3):
+        ctx.particles_set_velocity(*uv)
+    rng = np.random.default_rng(seed)
+    dt, T = 2e-3, 1.0
+    ip = lambda a: torch.from_numpy(a.astype(np.int32)).cuda()
+    fp = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    el = rng.integers(0, mesh.nel, npart)
+    xyz = rng.uniform(-1, 1, (3, npart))
+    zero = np.zeros(npart)
+    base = {"el": el, "xyz": xyz, "tau": rng.uniform(0.05, 0.5, npart)}
+
+    def fresh():
+        return {"el": ip(base["el"]), "old_el": ip(base["el"]),
+                "x": fp(base["xyz"][0]), "y": fp(base["xyz"][1]),
+                "z": fp(base["xyz"][2]), "u": fp(zero), "v": fp(zero),
+                "w": fp(zero), "t": fp(zero + (T - dt)),
+                "tau": fp(base["tau"]), "reinit": ip(np.zeros(npart))}
+    ctx.particles_step(fresh(), T, dt)              # warm-up
+    times = []
+    for _ in range(3):
+        pr = fresh()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        passes = ctx.particles_step(pr, T, dt)      # synchronous
+        times.append(time.perf_counter() - t0)
+    moved = int((pr["el"].cpu().numpy() != base["el"]).sum())
+    reinit = int(pr["reinit"].sum().item())
+    ctx.close()
+    t = min(times)
+    return {"particles": npart, "local_dofs": mesh.ntot, "seconds": t,
+            "particles_per_s": npart / t, "max_passes": passes,
+            "changed_element": moved, "reseeded": reinit,
+            "mesh": "channel 24x16x24, n=10, tanh 2.0", "dt": dt}
+
+
+def batch_case(S, torch, nprob=16, seed=11):
+    """Uneven batch (north-star configuration 4, reduced to nprob problems
+    of at most 16^3 elements so the bench stays within minutes):
+    independent Helmholtz problems on randomly deformed boxes, element
+    counts m^3 with m in [8, 16], orders N in {4,5,7,9,11}, lambda ~
+    U[1, 100], Jacobi-PCG to 1e-10 relative.  The batch (one launch per CG
+    stage, graph-replayed) against the same problems solved one by one."""
+    from ssenss_amd.mesh import box_mesh
+    rng = np.random.default_rng(seed)
+    ctxs, rhs = [], []
+    for _ in range(nprob):
+        m = int(rng.integers(8, 17))
+        n = int(rng.choice([5, 6, 8, 10, 12]))
+        lam = rng.uniform(1, 100)
+        mesh = box_mesh(n, (m, m, m), deform=1, amp=rng.uniform(0, 0.15),
+                        dt=(11.0 / 6.0) / lam, visc=1.0)
+        c = S.Context(mesh)
+        c.set_stream(torch.cuda.current_stream())
+        f = smooth_source(mesh.x.ravel(), mesh.y.ravel(), mesh.z.ravel(),
+                          seed=int(rng.integers(1 << 30)))
+        massv = torch.from_numpy(c.level_data(0, S.DATA_MASS)).cuda()
+        b = torch.from_numpy(f).cuda() * massv
+        c.dss(0, b)
+        ctxs.append(c)
+        rhs.append(b)
+    eps = [1e-10 * float(np.sqrt(c.dot(0, b, b))) for c, b in zip(ctxs, rhs)]
+    dofs = sum(c.ntot(0) for c in ctxs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    single_its = []
+    for c, b, e in zip(ctxs, rhs, eps):
+        f = b.clone()
+        u = torch.zeros_like(f)
+        st = c.cg(0, S.OP_HELMHOLTZ, S.X_VELOCITY, f, u, e, 5000)
+        single_its.append(st.iterations)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    batch = S.Batch([(c, S.OP_HELMHOLTZ, S.X_VELOCITY) for c in ctxs])
+    res = {}
+    for graph in (False, True):
+        fs = [b.clone() for b in rhs]
+        us = [torch.zeros_like(f) for f in fs]
+        batch.cg(fs, us, eps, 5000, use_graph=graph)        # warm-up
+        fs = [b.clone() for b in rhs]
+        us = [torch.zeros_like(f) for f in fs]
+        t0 = time.perf_counter()
+        st = batch.cg(fs, us, eps, 5000, use_graph=graph)
+        res[graph] = (time.perf_counter() - t0, [s.iterations for s in st])
+    batch.close()
+    for c in ctxs:
+        c.close()
+    return {"problems": nprob, "local_dofs": dofs,
+            "orders": sorted({c.mesh.n - 1 for c in ctxs}),
+            "sequential_s": t_seq, "batched_s": res[False][0],
+            "batched_graph_s": res[True][0],
+            "speedup_batched_graph": t_seq / res[True][0],
+            "iterations_single": single_its,
+            "iterations_batched": res[True][1]}
+
+
 def cpu_baseline(mesh, seconds):
     """The oracle (C restatement of the reference CPU path) on a bounded
     sample: Poisson Ax on a 12^3-element piece of the same deformed cube
@@ -147,6 +266,8 @@ def main():
     ap.add_argument("--max-iter", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-particles", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the N = 5 / 7 / 11 mixed-precision sweep")
     ap.add_argument("--profile", action="store_true",
@@ -233,6 +354,19 @@ def main():
             "variant": v, "us_per_variant": [round(x, 2) for x in us]}
     ctx.close()
 
+    particles = None if args.no_particles else particle_case(S, torch,
+                                                              stream)
+    if particles:
+        log(f"[bench] {particles['particles']} particles: "
+            f"{particles['seconds']*1e3:.2f} ms per step, "
+            f"{particles['max_passes']} passes")
+    batchres = None if args.no_batch else batch_case(S, torch)
+    if batchres:
+        log(f"[bench] batch of {batchres['problems']}: sequential "
+            f"{batchres['sequential_s']:.3f} s, batched "
+            f"{batchres['batched_s']:.3f} s, with graphs "
+            f"{batchres['batched_graph_s']:.3f} s")
+
     sweep = {}
     if not args.no_sweep:
         # the same problem at N = 5, 7, 11 on 48^3, 32^3, 20^3 elements
@@ -274,6 +408,8 @@ def main():
         "solve_mixed_fp32": solves.get("fp32"),
         "solve_mixed_bf16": solves.get("bf16"),
         "ax_launch_variants": variants,
+        "particles": particles,
+        "batch_uneven": batchres,
         "mixed_precision_sweep": sweep or None,
     }
     print(json.dumps(out), flush=True)

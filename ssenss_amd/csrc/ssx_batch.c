@@ -49,12 +49,17 @@ struct ssx_batch {
 	int *flags;
 	size_t *poff;              /* partial offsets per problem */
 	ssx_bprob *h_prob, *d_prob;
-	/* work tables */
-	int norder, order_n[SSX_NMAX + 1], order_v[SSX_NMAX + 1];
-	ssx_bwork *d_axtab[SSX_NMAX + 1];
-	long long nax[SSX_NMAX + 1];
+	/* work tables: one operator table per (order, operator) group, rebuilt
+	 * for the problems still iterating whenever that set changes */
+	int norder, order_n[2 * (SSX_NMAX + 1)], order_op[2 * (SSX_NMAX + 1)];
+	int order_v[2 * (SSX_NMAX + 1)];
+	ssx_bwork *d_axtab[2 * (SSX_NMAX + 1)];
+	long long nax[2 * (SSX_NMAX + 1)];
 	ssx_bwork *d_dsstab, *d_updtab;
 	long long ndss, nupd;
+	ssx_bwork *htab;           /* host staging of the tables */
+	long long htab_n;
+	unsigned char *active;     /* problems in the current tables */
 	double *bpart;             /* set-up reductions [SSX_NPART] */
 	int *h_flags;
 	double *h_scal;
@@ -108,18 +113,9 @@ void ssx_batch_destroy(ssx_batch *b)
 	if (b->own_stream)
 		hipStreamDestroy((hipStream_t) b->own_stream);
 	free(b->r); free(b->z); free(b->p); free(b->ap);
-	free(b->poff); free(b->h_prob);
+	free(b->poff); free(b->h_prob); free(b->htab); free(b->active);
 	free(b->ctx); free(b->op); free(b->var);
 	free(b);
-}
-
-static int upload_table(ssx_bwork **d, const ssx_bwork *h, long long n)
-{
-	TRYB(balloc((void **) d, sizeof(ssx_bwork) * (size_t) n));
-	if (n)
-		HIPB(hipMemcpy(*d, h, sizeof(ssx_bwork) * (size_t) n,
-				hipMemcpyHostToDevice));
-	return SSX_OK;
 }
 
 static int build(ssx_batch *b)
@@ -184,85 +180,124 @@ static int build(ssx_batch *b)
 		d->scal = b->scal + CG_NSLOT * i;
 		d->flags = b->flags + 2 * i;
 	}
-	/* operator tables, one per (order, launch shape) */
-	ssx_bwork *tmp;
-	long long tot = 0;
-	for (int i = 0; i < np; ++i)       /* nel bounds any operator table */
-		tot += b->h_prob[i].nel + b->h_prob[i].nbd + b->h_prob[i].nbu;
-	tmp = malloc(sizeof(ssx_bwork) * (size_t) (tot ? tot : 1));
-	if (!tmp) {
-		ssx_set_error("out of host memory (batch)");
-		return SSX_ENOMEM;
-	}
+	/* (order, operator) groups and the launch shape the autotuner chose for
+	 * the order; a problem's operator block count follows its group's
+	 * shape */
 	b->norder = 0;
 	for (int i = 0; i < np; ++i) {
-		const int n = b->h_prob[i].n;
+		const int n = b->h_prob[i].n, op = b->h_prob[i].op;
 		int k;
 		for (k = 0; k < b->norder; ++k)
-			if (b->order_n[k] == n)
+			if (b->order_n[k] == n && b->order_op[k] == op)
 				break;
 		if (k == b->norder) {
 			b->order_n[k] = n;
-			/* the launch shape the autotuner chose for this order */
+			b->order_op[k] = op;
 			b->order_v[k] = b->ctx[i]->lev[0].axv[SSX_PREC_FP64];
 			b->norder++;
 		}
+		const int teams = ssxk_bax_teams(n, b->order_v[k]);
+		b->h_prob[i].nba = (b->h_prob[i].nel + teams - 1) / teams;
 	}
 	int rc = SSX_OK;
+	for (int i = 0; i < np; ++i) {
+		const ssx_bprob *d = &b->h_prob[i];
+		b->poff[i + 1] = b->poff[i] + (size_t) (d->nba + d->nbd + 4 * d->nbu);
+	}
+	hipFree(b->part);
+	rc = balloc((void **) &b->part, sizeof(double) * b->poff[np]);
+	for (int i = 0; i < np && rc == SSX_OK; ++i)
+		b->h_prob[i].part = b->part + b->poff[i];
+	/* device tables at full size; host staging for rebuilds */
+	long long tot = 0;
 	for (int k = 0; k < b->norder && rc == SSX_OK; ++k) {
 		long long m = 0;
-		for (int i = 0; i < np; ++i) {
-			if (b->h_prob[i].n != b->order_n[k])
+		for (int i = 0; i < np; ++i)
+			if (b->h_prob[i].n == b->order_n[k]
+					&& b->h_prob[i].op == b->order_op[k])
+				m += b->h_prob[i].nba;
+		rc = balloc((void **) &b->d_axtab[k], sizeof(ssx_bwork) * m);
+		tot += m;
+	}
+	long long md = 0, mu = 0;
+	for (int i = 0; i < np; ++i) {
+		md += b->h_prob[i].nbd;
+		mu += b->h_prob[i].nbu;
+	}
+	if (rc == SSX_OK)
+		rc = balloc((void **) &b->d_dsstab, sizeof(ssx_bwork) * md);
+	if (rc == SSX_OK)
+		rc = balloc((void **) &b->d_updtab, sizeof(ssx_bwork) * mu);
+	tot += md + mu;
+	b->htab = malloc(sizeof(ssx_bwork) * (size_t) (tot ? tot : 1));
+	b->htab_n = tot;
+	b->active = calloc(np, 1);
+	if (rc == SSX_OK && (!b->htab || !b->active)) {
+		ssx_set_error("out of host memory (batch)");
+		rc = SSX_ENOMEM;
+	}
+	return rc;
+}
+
+/* the work tables of the problems with active[i] != 0, uploaded on the
+ * batch stream (ordered after the kernels that read the old ones) */
+static int fill_tables(ssx_batch *b, const unsigned char *active)
+{
+	hipStream_t st = (hipStream_t) b->stream;
+	ssx_bwork *t = b->htab;
+	long long off = 0;
+	for (int k = 0; k < b->norder; ++k) {
+		long long m = 0;
+		for (int i = 0; i < b->np; ++i) {
+			const ssx_bprob *d = &b->h_prob[i];
+			if (!active[i] || d->n != b->order_n[k] || d->op != b->order_op[k])
 				continue;
-			/* a problem with another shape for the same order gets this
-			 * order's shape: its block count follows */
-			const int teams = ssxk_bax_teams(b->order_n[k], b->order_v[k]);
-			b->h_prob[i].nba = (b->h_prob[i].nel + teams - 1) / teams;
-			for (long long q = 0; q < b->h_prob[i].nba; ++q) {
-				tmp[m].prob = i;
-				tmp[m].blk = (int) q;
+			for (long long q = 0; q < d->nba; ++q) {
+				t[off + m].prob = i;
+				t[off + m].blk = (int) q;
 				++m;
 			}
 		}
 		b->nax[k] = m;
-		rc = upload_table(&b->d_axtab[k], tmp, m);
+		if (m)
+			HIPB(hipMemcpyAsync(b->d_axtab[k], t + off,
+					sizeof(ssx_bwork) * m, hipMemcpyHostToDevice, st));
+		off += m;
 	}
-	/* the operator blocks may have changed: recompute partial offsets */
-	for (int i = 0; i < np && rc == SSX_OK; ++i) {
-		const ssx_bprob *d = &b->h_prob[i];
-		b->poff[i + 1] = b->poff[i] + (size_t) (d->nba + d->nbd + 4 * d->nbu);
-	}
-	if (rc == SSX_OK) {
-		hipFree(b->part);
-		rc = balloc((void **) &b->part, sizeof(double) * b->poff[np]);
-		for (int i = 0; i < np && rc == SSX_OK; ++i)
-			b->h_prob[i].part = b->part + b->poff[i];
-	}
-	if (rc == SSX_OK) {
+	for (int pass = 0; pass < 2; ++pass) {
 		long long m = 0;
-		for (int i = 0; i < np; ++i)
-			for (long long q = 0; q < b->h_prob[i].nbd; ++q) {
-				tmp[m].prob = i;
-				tmp[m].blk = (int) q;
+		for (int i = 0; i < b->np; ++i) {
+			if (!active[i])
+				continue;
+			const long long nb = pass == 0 ? b->h_prob[i].nbd
+				: b->h_prob[i].nbu;
+			for (long long q = 0; q < nb; ++q) {
+				t[off + m].prob = i;
+				t[off + m].blk = (int) q;
 				++m;
 			}
-		b->ndss = m;
-		rc = upload_table(&b->d_dsstab, tmp, m);
+		}
+		if (m)
+			HIPB(hipMemcpyAsync(pass == 0 ? b->d_dsstab : b->d_updtab,
+					t + off, sizeof(ssx_bwork) * m, hipMemcpyHostToDevice,
+					st));
+		if (pass == 0)
+			b->ndss = m;
+		else
+			b->nupd = m;
+		off += m;
 	}
-	if (rc == SSX_OK) {
-		long long m = 0;
-		for (int i = 0; i < np; ++i)
-			for (long long q = 0; q < b->h_prob[i].nbu; ++q) {
-				tmp[m].prob = i;
-				tmp[m].blk = (int) q;
-				++m;
-			}
-		b->nupd = m;
-		rc = upload_table(&b->d_updtab, tmp, m);
+	/* the staging buffer is reused by the next rebuild: let the copies
+	 * finish first */
+	HIPB(hipStreamSynchronize(st));
+	memcpy(b->active, active, b->np);
+	if (b->have_graph) {
+		hipGraphExecDestroy(b->gexec);
+		b->have_graph = 0;
 	}
-	free(tmp);
-	return rc;
+	return SSX_OK;
 }
+
 
 int ssx_batch_create(ssx_batch **out, ssx_ctx *const *ctx, const int *op,
 		const int *var, int count)
@@ -333,8 +368,8 @@ static int iteration(ssx_batch *b, int pupd)
 {
 	void *st = b->stream;
 	for (int k = 0; k < b->norder; ++k)
-		TRYB(ssxk_bax(st, b->order_n[k], b->order_v[k], pupd, b->d_prob,
-				b->d_axtab[k], b->nax[k]));
+		TRYB(ssxk_bax(st, b->order_n[k], b->order_op[k], b->order_v[k], pupd,
+				b->d_prob, b->d_axtab[k], b->nax[k]));
 	TRYB(ssxk_bdss(st, b->d_prob, b->d_dsstab, b->ndss));
 	TRYB(ssxk_breduce(st, 0, b->d_prob, b->np));
 	TRYB(ssxk_bupdate(st, b->d_prob, b->d_updtab, b->nupd));
@@ -431,29 +466,55 @@ int ssx_batch_cg(ssx_batch *b, double *const *f, double *const *u,
 	for (int q = 64; q < 2 * np; q += 64)
 		TRYB(ssxk_set_ints(st, b->flags + q, 0,
 				2 * np - q < 64 ? 2 * np - q : 64));
+	unsigned char *act = malloc(np);
+	if (!act)
+		return SSX_ENOMEM;
+	memset(act, 1, np);
+	int rc0 = SSX_OK;
+	if (memcmp(act, b->active, np) != 0)
+		rc0 = fill_tables(b, act);
 	/* first iteration: p = z already */
-	TRYB(iteration(b, 0));
+	if (rc0 == SSX_OK)
+		rc0 = iteration(b, 0);
 	int launched = 1;
-	if (use_graph && !b->have_graph)
-		TRYB(capture(b));
-	for (;;) {
-		int all = 1;
-		HIPB(hipMemcpyAsync(b->h_flags, b->flags, sizeof(int) * 2 * np,
-				hipMemcpyDeviceToHost, st));
-		HIPB(hipStreamSynchronize(st));
-		for (int i = 0; i < np; ++i)
-			all &= b->h_flags[2 * i] != 0;
+	for (; rc0 == SSX_OK;) {
+		int all = 1, changed = 0;
+		if ((rc0 = hipMemcpyAsync(b->h_flags, b->flags, sizeof(int) * 2 * np,
+				hipMemcpyDeviceToHost, st)) != hipSuccess
+				|| (rc0 = hipStreamSynchronize(st)) != hipSuccess) {
+			ssx_set_error("batch flags: %s",
+					hipGetErrorString((hipError_t) rc0));
+			rc0 = SSX_EHIP;
+			break;
+		}
+		for (int i = 0; i < np; ++i) {
+			act[i] = b->h_flags[2 * i] == 0;
+			all &= !act[i];
+			changed |= act[i] != b->active[i];
+		}
 		if (all || launched >= max_iter)
 			break;
+		/* drop converged problems from the launches */
+		if (changed && (rc0 = fill_tables(b, act)) != SSX_OK)
+			break;
+		if (use_graph && !b->have_graph && (rc0 = capture(b)) != SSX_OK)
+			break;
 		if (use_graph) {
-			HIPB(hipGraphLaunch(b->gexec, st));
+			if (hipGraphLaunch(b->gexec, st) != hipSuccess) {
+				ssx_set_error("hipGraphLaunch failed (batch)");
+				rc0 = SSX_EHIP;
+				break;
+			}
 			launched += GRAPH_ITERS;
 		} else {
-			for (int k = 0; k < GRAPH_ITERS; ++k)
-				TRYB(iteration(b, 1));
+			for (int k = 0; k < GRAPH_ITERS && rc0 == SSX_OK; ++k)
+				rc0 = iteration(b, 1);
 			launched += GRAPH_ITERS;
 		}
 	}
+	free(act);
+	if (rc0 != SSX_OK)
+		return rc0;
 	HIPB(hipMemcpyAsync(b->h_scal, b->scal, sizeof(double) * CG_NSLOT * np,
 			hipMemcpyDeviceToHost, st));
 	HIPB(hipStreamSynchronize(st));

@@ -31,8 +31,8 @@ typedef struct ssx_bwork {
 extern "C" {
 #endif
 /* launchers (ssx_bkernels.hip); all return SSX_OK or SSX_EHIP */
-int ssxk_bax(void *stream, int n, int v, int pupd, const ssx_bprob *pr,
-		const ssx_bwork *tab, long long ntab);
+int ssxk_bax(void *stream, int n, int op, int v, int pupd,
+		const ssx_bprob *pr, const ssx_bwork *tab, long long ntab);
 int ssxk_bdss(void *stream, const ssx_bprob *pr, const ssx_bwork *tab,
 		long long ntab);
 int ssxk_bupdate(void *stream, const ssx_bprob *pr, const ssx_bwork *tab,

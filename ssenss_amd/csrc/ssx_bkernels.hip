@@ -6,7 +6,8 @@
 // problem has its own done flag, scalars and partial sums, so a converged
 // problem's workgroups return at once while the others go on.  The
 // operator stage is one launch per distinct order (the kernel shape
-// depends on n); DSS, update and the two reductions are one launch each.
+// depends on n, and one per operator); DSS, update and the two reductions
+// are one launch each.  The host drops converged problems from the tables.
 // Per-problem partials are summed in a fixed order: results do not depend
 // on the other problems of the batch.
 #include <hip/hip_runtime.h>
@@ -24,8 +25,9 @@
 		}                                                          \
 	} while (0)
 
-// operator (PUPD: with the direction update of the previous iteration)
-template <int N, bool PUPD, int V>
+// operator (PUPD: with the direction update of the previous iteration);
+// one table per (order, operator) group
+template <int N, int OP, bool PUPD, int V>
 __global__ void __launch_bounds__((AxShape<N, V>::BLOCK))
 k_bax(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 {
@@ -37,15 +39,9 @@ k_bax(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 	// PUPD reads z and p and writes p; otherwise p is the operand
 	const double *u = PUPD ? nullptr : d->p;
 	double *pio = PUPD ? d->p : nullptr;
-	double pd;
-	if (d->op == SSX_OP_HELMHOLTZ)
-		pd = ax_team<N, 1, double, double, PUPD, V>(w.blk, d->nel, u, d->ap,
-				d->gf, d->mass, d->dmat, d->lambda, d->bc, d->apply_mask,
-				d->z, pio, d->scal, CG_BETA, ic);
-	else
-		pd = ax_team<N, 0, double, double, PUPD, V>(w.blk, d->nel, u, d->ap,
-				d->gf, d->mass, d->dmat, d->lambda, d->bc, d->apply_mask,
-				d->z, pio, d->scal, CG_BETA, ic);
+	const double pd = ax_team<N, OP, double, double, PUPD, V>(w.blk, d->nel,
+			u, d->ap, d->gf, d->mass, d->dmat, d->lambda, d->bc,
+			d->apply_mask, d->z, pio, d->scal, CG_BETA, ic);
 	ax_block_partial(pd, d->part, w.blk);
 }
 
@@ -171,26 +167,40 @@ k_breduce(int mode, const ssx_bprob *__restrict__ pr)
 		d->flags[0] = 1;
 }
 
-template <int N, bool PU, int V>
+template <int N, int OP, bool PU, int V>
 static void launch_bax(hipStream_t st, const ssx_bprob *pr,
 		const ssx_bwork *tab, long long ntab)
 {
-	hipLaunchKernelGGL((k_bax<N, PU, V>), dim3((unsigned) ntab),
+	hipLaunchKernelGGL((k_bax<N, OP, PU, V>), dim3((unsigned) ntab),
 			dim3(AxShape<N, V>::BLOCK), 0, st, pr, tab);
 }
 
-extern "C" int ssxk_bax(void *stream, int n, int v, int pupd,
+template <int N, int V>
+static void launch_bax_nv(hipStream_t st, int op, int pupd,
+		const ssx_bprob *pr, const ssx_bwork *tab, long long ntab)
+{
+	if (op == SSX_OP_HELMHOLTZ) {
+		if (pupd)
+			launch_bax<N, 1, true, V>(st, pr, tab, ntab);
+		else
+			launch_bax<N, 1, false, V>(st, pr, tab, ntab);
+	} else {
+		if (pupd)
+			launch_bax<N, 0, true, V>(st, pr, tab, ntab);
+		else
+			launch_bax<N, 0, false, V>(st, pr, tab, ntab);
+	}
+}
+
+extern "C" int ssxk_bax(void *stream, int n, int op, int v, int pupd,
 		const ssx_bprob *pr, const ssx_bwork *tab, long long ntab)
 {
 	if (ntab <= 0)
 		return SSX_OK;
 	hipStream_t st = (hipStream_t) stream;
-	switch ((n * SSX_AX_NVARIANTS + v) * 2 + (pupd != 0)) {
-#define CASEV(NN, VV) \
-	case (NN * SSX_AX_NVARIANTS + VV) * 2: \
-		launch_bax<NN, false, VV>(st, pr, tab, ntab); break; \
-	case (NN * SSX_AX_NVARIANTS + VV) * 2 + 1: \
-		launch_bax<NN, true, VV>(st, pr, tab, ntab); break;
+	switch (n * SSX_AX_NVARIANTS + v) {
+#define CASEV(NN, VV) case NN * SSX_AX_NVARIANTS + VV: \
+		launch_bax_nv<NN, VV>(st, op, pupd, pr, tab, ntab); break;
 #define CASE(NN) CASEV(NN, 0) CASEV(NN, 1) CASEV(NN, 2) CASEV(NN, 3) \
 		CASEV(NN, 4) CASEV(NN, 5) CASEV(NN, 6) CASEV(NN, 7)
 	CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9)
