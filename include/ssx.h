@@ -191,6 +191,12 @@ int ssx_set_diag_choice(ssx_ctx *ctx, int choice);
  * every workgroup's device-scope release (an L2 write-back per XCD) costs
  * more than the two launches it saves. */
 #define SSX_OPT_FUSED_REDUCE 5
+/* SSX_OPT_COARSE_ASSEMBLED: 1 (default) runs the p-multigrid coarsest-level
+ * Jacobi-PCG over unique nodes with the level's operator assembled once
+ * (CSR; at n = 2 a 27-point stencil) instead of on E-vectors with the
+ * element operator and DSS; 0 keeps the E-vector CG.  Same algorithm and
+ * stopping test; sums in another order (equal to rounding). */
+#define SSX_OPT_COARSE_ASSEMBLED 6
 #define SSX_PREC_FP64 0
 #define SSX_PREC_FP32 1
 #define SSX_PREC_BF16 2

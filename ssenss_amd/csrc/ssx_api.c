@@ -268,6 +268,11 @@ static int setup_level(ssx_ctx *c, const ssx_mesh *m, int l, double *hx,
 	rc = up(&L->mask, one, nt);
 	if (rc)
 		goto out;
+	if (l > 0 && l == c->nlev - 1) {   /* coarsest level: assembled form */
+		rc = ssx_coarse_build(c, l, gf, mass, dp, dh, one);
+		if (rc)
+			goto out;
+	}
 	{
 		double dm[2 * SSX_NMAX * SSX_NMAX];
 		memcpy(dm, L->b.d, sizeof(double) * n * n);
@@ -371,6 +376,7 @@ int ssx_create(ssx_ctx **out, const ssx_mesh *m)
 	c->fused = 1;
 	c->persist = 0;
 	c->fused_red = 0;
+	c->coarse_csr = 1;
 	c->lp_coarse_rtol = pow(10.0, -SSX_LP_COARSE_RTOL_EXP);
 	c->lambda = (11.0 / 6.0) / (m->dt * m->visc);
 	int nl = SSX_LMAX;
@@ -504,6 +510,7 @@ void ssx_destroy(ssx_ctx *c)
 			hipFree(L->grp[mi]);
 		hipFree(L->r); hipFree(L->z); hipFree(L->p); hipFree(L->ap);
 		hipFree(L->mg_r); hipFree(L->mg_fn); hipFree(L->mg_e);
+		ssx_coarse_free(&L->cs);
 		free(L->h_x); free(L->h_y); free(L->h_z);
 		hipFree(L->gf_lp); hipFree(L->mass_lp); hipFree(L->diagp_lp);
 		hipFree(L->diagh_lp); hipFree(L->scale_lp); hipFree(L->mask_lp);
@@ -584,6 +591,9 @@ int ssx_set_option(ssx_ctx *c, int option, int value)
 	switch (option) {
 	case SSX_OPT_FUSED_CG:
 		c->fused = value != 0;
+		return SSX_OK;
+	case SSX_OPT_COARSE_ASSEMBLED:
+		c->coarse_csr = value != 0;
 		return SSX_OK;
 	case SSX_OPT_FUSED_REDUCE:
 		c->fused_red = value != 0;
@@ -1000,7 +1010,11 @@ static int mg_recurse(ssx_ctx *c, int l, int op, int var, double *f,
 	double err;
 	int it;
 	if (l == c->nlev - 1) {          /* precon.c:242-244 */
-		TRY(cg_level(c, l, op, var, f, x, eps, 500, l > 0, &it, &err));
+		if (c->coarse_csr && c->lev[l].cs.nu > 0)
+			TRY(ssx_coarse_cg(c, l, op, var, f, x, eps, 0.0, 500, l > 0, &it,
+					&err));
+		else
+			TRY(cg_level(c, l, op, var, f, x, eps, 500, l > 0, &it, &err));
 		c->cnt[1] += it;
 		return SSX_OK;
 	}
@@ -1212,8 +1226,12 @@ static int mg_recurse_lp(ssx_ctx *c, int l, int op, int var, double eps)
 		 * leaves a component in the null space of singular problems
 		 * (periodic velocity) that CG cannot remove. */
 		TRY(ssxk_fill(st, L->mg_e, 0.0, L->ntot));
-		TRY(cg_level_fused(c, l, op, var, L->mg_fn, L->mg_e, eps,
-				c->lp_coarse_rtol, 500, 1, &it, &err));
+		if (c->coarse_csr && L->cs.nu > 0)
+			TRY(ssx_coarse_cg(c, l, op, var, L->mg_fn, L->mg_e, eps,
+					c->lp_coarse_rtol, 500, 1, &it, &err));
+		else
+			TRY(cg_level_fused(c, l, op, var, L->mg_fn, L->mg_e, eps,
+					c->lp_coarse_rtol, 500, 1, &it, &err));
 		c->cnt[1] += it;
 		return ssxk_cvt(st, SSX_PREC_FP64, SSX_PREC_FP32, L->mg_e, L->e_lp,
 				L->ntot);
@@ -1288,7 +1306,15 @@ int ssx_cg(ssx_ctx *c, int l, int op, int var, double *f, double *u,
 	memcpy(c0, c->cnt, sizeof(c0));
 	double t0 = now_s(), err;
 	int it = 0;
-	int rc = cg_level(c, l, op, var, f, u, eps, max_iter, 0, &it, &err);
+	int rc;
+	if (c->coarse_csr && l == c->nlev - 1 && c->lev[l].cs.nu > 0) {
+		rc = ssx_mask(c, l, var, f);               /* solver.c:46 */
+		if (rc == SSX_OK)
+			rc = ssx_coarse_cg(c, l, op, var, f, u, eps, 0.0, max_iter, 0,
+					&it, &err);
+	} else {
+		rc = cg_level(c, l, op, var, f, u, eps, max_iter, 0, &it, &err);
+	}
 	if (rc == SSX_OK)
 		rc = ssx_synchronize(c);
 	if (stats) {

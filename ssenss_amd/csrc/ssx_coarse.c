@@ -1,0 +1,421 @@
+/*
+ * ssx_coarse.c -- the coarsest p-multigrid level solved over unique nodes.
+ *
+ * At the coarsest level (n = 2: trilinear elements, every node a vertex
+ * shared by up to 8 elements) an E-vector holds each unknown up to 8
+ * times, and every CG iteration pays the element operator, the DSS and
+ * vector updates on that redundant storage.  Here the level's Poisson and
+ * Helmholtz operators are assembled once at set-up into one CSR structure
+ * over the unique nodes (27 entries per row at n = 2), and the coarse
+ * Jacobi-PCG (conjugate_gradient, solver.c:33-81, as called from
+ * multigrid_recurse, precon.c:242-244) runs on unique-node vectors:
+ *
+ *   set-up: f_u = f at one copy of each node (f is averaged over copies),
+ *           mask, r = f - A x, null space, z = D^-1 r, p = z, <r,z>
+ *   loop:   ap = mask (A p) with p = mask((z - c) + beta p) formed on the
+ *           fly (ping-pong p buffers), reduce -> alpha, update u, r, z and
+ *           the partials, reduce -> beta and the stopping test
+ *
+ * On continuous vectors the weighted inner product <x,y>_w of the E-vector
+ * form is the plain sum over unique nodes, so this is the same algorithm
+ * with the same stopping test; only the order of the sums differs.  The
+ * solution is written back to every copy.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "ssx_internal.h"
+
+#define HIPQ(x)                                                              \
+	do {                                                                 \
+		hipError_t e_ = (x);                                         \
+		if (e_ != hipSuccess) {                                      \
+			ssx_set_error("%s: %s", #x, hipGetErrorString(e_));   \
+			return SSX_EHIP;                                     \
+		}                                                            \
+	} while (0)
+#define TRYQ(x)                                                              \
+	do {                                                                 \
+		int rc_ = (x);                                               \
+		if (rc_ != SSX_OK)                                           \
+			return rc_;                                          \
+	} while (0)
+
+/* the element operator of the kernels (poisson.c:34-56, helmholtz.c:33-49)
+ * on one host element: D_il at d[i*n+l]; ds = SVV matrix for Helmholtz */
+static void host_ax_elem(int n, int op, const double *d, const double *ds,
+		const double *g, const double *mass, double lambda,
+		const double *u, double *au, double *t)
+{
+	const int n3 = n * n * n;
+	const double *Dx = op == SSX_OP_HELMHOLTZ ? ds : d;
+	double *t0 = t, *t1 = t + n3, *t2 = t + 2 * n3;
+	for (int k = 0; k < n; ++k)
+		for (int j = 0; j < n; ++j)
+			for (int i = 0; i < n; ++i) {
+				double gx = 0.0, gy = 0.0, gz = 0.0;
+				for (int l = 0; l < n; ++l) {
+					gx += Dx[i * n + l] * u[(k * n + j) * n + l];
+					gy += d[j * n + l] * u[(k * n + l) * n + i];
+					gz += d[k * n + l] * u[(l * n + j) * n + i];
+				}
+				const int q = (k * n + j) * n + i;
+				const double gxx = g[q], gxy = g[n3 + q], gxz = g[2 * n3 + q];
+				const double gyy = g[3 * n3 + q], gyz = g[4 * n3 + q];
+				const double gzz = g[5 * n3 + q];
+				t0[q] = gxx * gx + gxy * gy + gxz * gz;
+				t1[q] = gxy * gx + gyy * gy + gyz * gz;
+				t2[q] = gxz * gx + gyz * gy + gzz * gz;
+			}
+	const double *Dt = op == SSX_OP_HELMHOLTZ ? ds : d;
+	for (int k = 0; k < n; ++k)
+		for (int j = 0; j < n; ++j)
+			for (int i = 0; i < n; ++i) {
+				double v = 0.0;
+				for (int l = 0; l < n; ++l)
+					v += d[l * n + i] * t0[(k * n + j) * n + l]
+						+ Dt[l * n + j] * t1[(k * n + l) * n + i];
+				for (int l = 0; l < n; ++l)
+					v += Dt[l * n + k] * t2[(l * n + j) * n + i];
+				const int q = (k * n + j) * n + i;
+				if (op == SSX_OP_HELMHOLTZ)
+					v += lambda * (mass[q] * u[q]);
+				au[q] = v;
+			}
+}
+
+void ssx_coarse_free(ssx_coarse *cs)
+{
+	hipFree(cs->rowptr); hipFree(cs->col);
+	hipFree(cs->valp); hipFree(cs->valh);
+	hipFree(cs->rep); hipFree(cs->cmap);
+	hipFree(cs->dinvp); hipFree(cs->dinvh); hipFree(cs->mask);
+	hipFree(cs->ones);
+	hipFree(cs->f); hipFree(cs->x); hipFree(cs->r); hipFree(cs->z);
+	hipFree(cs->p0); hipFree(cs->p1); hipFree(cs->ap);
+	memset(cs, 0, sizeof(*cs));
+}
+
+static int dup(void **d, const void *h, size_t bytes)
+{
+	if (hipMalloc(d, bytes ? bytes : 1) != hipSuccess) {
+		*d = NULL;
+		ssx_set_error("hipMalloc of %zu bytes (coarse level)", bytes);
+		return SSX_ENOMEM;
+	}
+	if (h && bytes)
+		HIPQ(hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice));
+	return SSX_OK;
+}
+
+/*
+ * Build at set-up, while the level's groups, factors, mass, inverse
+ * diagonals and mask are still on the host.
+ */
+int ssx_coarse_build(ssx_ctx *c, int l, const double *gf, const double *mass,
+		const double *dp, const double *dh, const double *mask)
+{
+	ssx_level *L = &c->lev[l];
+	ssx_coarse *cs = &L->cs;
+	const int n = L->n, n3 = L->n3, nel = c->nel;
+	const size_t nt = L->ntot;
+	int *cmap = malloc(sizeof(int) * nt);
+	long long *gid = malloc(sizeof(long long) * nt);
+	if (!cmap || !gid) {
+		free(cmap); free(gid);
+		ssx_set_error("out of host memory (coarse level)");
+		return SSX_ENOMEM;
+	}
+	/* copy -> group, then unique ids in order of the smallest copy */
+	for (size_t q = 0; q < nt; ++q)
+		gid[q] = -1;
+	for (int mi = 0; mi < 3; ++mi) {
+		const int M = 2 << mi;
+		const long long G = L->ngrp[mi];
+		for (long long g = 0; g < G; ++g)
+			for (int t = 0; t < M; ++t)
+				gid[L->grp[mi][t * G + g]] = ((long long) mi << 40) | g;
+	}
+	for (size_t q = 0; q < nt; ++q)
+		cmap[q] = -1;
+	int nu = 0;
+	int *rep = malloc(sizeof(int) * nt);
+	if (!rep) {
+		free(cmap); free(gid);
+		return SSX_ENOMEM;
+	}
+	for (size_t q = 0; q < nt; ++q) {
+		if (cmap[q] >= 0)
+			continue;
+		rep[nu] = (int) q;
+		if (gid[q] < 0) {
+			cmap[q] = nu;
+		} else {
+			const int mi = (int) (gid[q] >> 40);
+			const long long g = gid[q] & ((1LL << 40) - 1);
+			const int M = 2 << mi;
+			const long long G = L->ngrp[mi];
+			for (int t = 0; t < M; ++t)
+				cmap[L->grp[mi][t * G + g]] = nu;
+		}
+		++nu;
+	}
+	free(gid);
+	/* element matrices, both operators, rows gathered per unique node */
+	const int per = 8 * n3;                /* row entries: <= 8 elements */
+	int *cnt = calloc(nu, sizeof(int));
+	int *rc_col = malloc(sizeof(int) * (size_t) nu * per);
+	double *rvp = malloc(sizeof(double) * (size_t) nu * per);
+	double *rvh = malloc(sizeof(double) * (size_t) nu * per);
+	double *ae = malloc(sizeof(double) * 2 * (size_t) n3 * n3);
+	double *ub = malloc(sizeof(double) * n3 * 5);
+	int rc = SSX_OK;
+	if (!cnt || !rc_col || !rvp || !rvh || !ae || !ub) {
+		ssx_set_error("out of host memory (coarse level)");
+		rc = SSX_ENOMEM;
+	}
+	for (int e = 0; e < nel && rc == SSX_OK; ++e) {
+		const double *g = gf + (size_t) e * 6 * n3;
+		const double *ms = mass + (size_t) e * n3;
+		for (int o = 0; o < 2; ++o)
+			for (int b = 0; b < n3; ++b) {
+				memset(ub, 0, sizeof(double) * n3);
+				ub[b] = 1.0;
+				host_ax_elem(n, o, L->b.d, L->b.dsvv, g, ms, c->lambda, ub,
+						ub + n3, ub + 2 * n3);
+				for (int a = 0; a < n3; ++a)
+					ae[((size_t) o * n3 + a) * n3 + b] = ub[n3 + a];
+			}
+		for (int a = 0; a < n3; ++a) {
+			const int row = cmap[(size_t) e * n3 + a];
+			for (int b = 0; b < n3; ++b) {
+				const int k = cnt[row]++;
+				if (k >= per) {
+					ssx_set_error("coarse level: node in more than 8 "
+							"elements");
+					rc = SSX_EMESH;
+					break;
+				}
+				rc_col[(size_t) row * per + k] = cmap[(size_t) e * n3 + b];
+				rvp[(size_t) row * per + k] = ae[(size_t) a * n3 + b];
+				rvh[(size_t) row * per + k] = ae[((size_t) n3 + a) * n3 + b];
+			}
+			if (rc)
+				break;
+		}
+	}
+	/* sort each row by column (stable: contributions keep element order)
+	 * and sum duplicates */
+	int *rowptr = malloc(sizeof(int) * (nu + 1));
+	long long nnz = 0;
+	if (rc == SSX_OK && !rowptr)
+		rc = SSX_ENOMEM;
+	for (int u = 0; u < nu && rc == SSX_OK; ++u) {
+		int *cc = rc_col + (size_t) u * per;
+		double *vp = rvp + (size_t) u * per, *vh = rvh + (size_t) u * per;
+		const int m = cnt[u];
+		for (int a = 1; a < m; ++a) {         /* insertion sort */
+			const int kc = cc[a];
+			const double kp = vp[a], kh = vh[a];
+			int b = a - 1;
+			while (b >= 0 && cc[b] > kc) {
+				cc[b + 1] = cc[b];
+				vp[b + 1] = vp[b];
+				vh[b + 1] = vh[b];
+				--b;
+			}
+			cc[b + 1] = kc;
+			vp[b + 1] = kp;
+			vh[b + 1] = kh;
+		}
+		int w = 0;
+		for (int a = 0; a < m; ++a) {
+			if (w > 0 && cc[w - 1] == cc[a]) {
+				vp[w - 1] += vp[a];
+				vh[w - 1] += vh[a];
+			} else {
+				cc[w] = cc[a];
+				vp[w] = vp[a];
+				vh[w] = vh[a];
+				++w;
+			}
+		}
+		cnt[u] = w;
+		rowptr[u] = (int) nnz;
+		nnz += w;
+	}
+	int *col = NULL;
+	double *valp = NULL, *valh = NULL;
+	if (rc == SSX_OK) {
+		rowptr[nu] = (int) nnz;
+		col = malloc(sizeof(int) * nnz);
+		valp = malloc(sizeof(double) * nnz);
+		valh = malloc(sizeof(double) * nnz);
+		if (!col || !valp || !valh)
+			rc = SSX_ENOMEM;
+	}
+	for (int u = 0; u < nu && rc == SSX_OK; ++u) {
+		memcpy(col + rowptr[u], rc_col + (size_t) u * per,
+				sizeof(int) * cnt[u]);
+		memcpy(valp + rowptr[u], rvp + (size_t) u * per,
+				sizeof(double) * cnt[u]);
+		memcpy(valh + rowptr[u], rvh + (size_t) u * per,
+				sizeof(double) * cnt[u]);
+	}
+	free(cnt); free(rc_col); free(rvp); free(rvh); free(ae); free(ub);
+	/* per-unique-node diagonals, mask, ones */
+	double *tmp = rc == SSX_OK ? malloc(sizeof(double) * 4 * (size_t) nu)
+		: NULL;
+	if (rc == SSX_OK && !tmp)
+		rc = SSX_ENOMEM;
+	if (rc == SSX_OK) {
+		for (int u = 0; u < nu; ++u) {
+			tmp[u] = dp[rep[u]];
+			tmp[nu + u] = dh[rep[u]];
+			tmp[2 * nu + u] = mask[rep[u]];
+			tmp[3 * nu + u] = 1.0;
+		}
+		cs->nu = nu;
+		cs->nnz = nnz;
+		const size_t vb = sizeof(double) * (size_t) nu;
+		rc = dup((void **) &cs->rowptr, rowptr, sizeof(int) * (nu + 1));
+		if (!rc) rc = dup((void **) &cs->col, col, sizeof(int) * nnz);
+		if (!rc) rc = dup((void **) &cs->valp, valp, sizeof(double) * nnz);
+		if (!rc) rc = dup((void **) &cs->valh, valh, sizeof(double) * nnz);
+		if (!rc) rc = dup((void **) &cs->rep, rep, sizeof(int) * nu);
+		if (!rc) rc = dup((void **) &cs->cmap, cmap, sizeof(int) * nt);
+		if (!rc) rc = dup((void **) &cs->dinvp, tmp, vb);
+		if (!rc) rc = dup((void **) &cs->dinvh, tmp + nu, vb);
+		if (!rc) rc = dup((void **) &cs->mask, tmp + 2 * nu, vb);
+		if (!rc) rc = dup((void **) &cs->ones, tmp + 3 * nu, vb);
+		double **vec[7] = { &cs->f, &cs->x, &cs->r, &cs->z, &cs->p0, &cs->p1,
+			&cs->ap };
+		for (int k = 0; k < 7 && !rc; ++k)
+			rc = dup((void **) vec[k], NULL, vb);
+	}
+	free(tmp); free(rowptr); free(col); free(valp); free(valh);
+	free(cmap); free(rep);
+	if (rc != SSX_OK)
+		ssx_coarse_free(cs);
+	return rc;
+}
+
+/* dot of unique-node vectors into scal[dst] (weight 1) */
+static int udot(ssx_ctx *c, const double *x, const double *y, int dst)
+{
+	ssx_coarse *cs = &c->lev[c->nlev - 1].cs;
+	TRYQ(ssxk_dot(c->stream, (size_t) cs->nu, x, cs->ones, y, c->part, 0));
+	return ssxk_reduce(c->stream, c->part, 0, c->scal, dst);
+}
+
+/* u -= <u,1>/nu (subtract_nullspace, solver.c:200-213, unique nodes) */
+static int unull(ssx_ctx *c, double *x)
+{
+	ssx_coarse *cs = &c->lev[c->nlev - 1].cs;
+	TRYQ(ssxk_dot(c->stream, (size_t) cs->nu, x, cs->ones, NULL, c->part, 0));
+	TRYQ(ssxk_reduce(c->stream, c->part, 0, c->scal, SC_NULLC));
+	return ssxk_shift_s(c->stream, (size_t) cs->nu, c->scal, SC_NULLC,
+			(double) cs->nu, x);
+}
+
+int ssx_coarse_cg(ssx_ctx *c, int l, int op, int var, const double *f,
+		double *x, double eps, double rtol, int max_iter, int x_zero,
+		int *iters, double *err_out)
+{
+	ssx_level *L = &c->lev[l];
+	ssx_coarse *cs = &L->cs;
+	if (cs->nu <= 0) {
+		ssx_set_error("level %d has no assembled operator", l);
+		return SSX_EINVAL;
+	}
+	void *st = c->stream;
+	const int nu = cs->nu;
+	const int pres = var == SSX_PRESSURE;
+	const double *val = op == SSX_OP_HELMHOLTZ ? cs->valh : cs->valp;
+	int ch = c->diag_choice;
+	if (ch < 0)
+		ch = pres ? 0 : 1;                /* precon.c:211-225 */
+	const double *dinv = ch == 0 ? cs->dinvp : cs->dinvh;
+	const double *mask = pres ? NULL : cs->mask;
+	const int o = SC_CG(l);
+	int *flags = c->cg_flags + 2 * l;
+	TRYQ(ssxk_gather_u(st, nu, cs->rep, f, cs->f));
+	if (mask)
+		TRYQ(ssxk_mul(st, (size_t) nu, cs->f, mask, cs->f));
+	if (x_zero) {
+		TRYQ(ssxk_fill(st, cs->x, 0.0, (size_t) nu));
+		TRYQ(ssxk_copy(st, cs->r, cs->f, (size_t) nu));
+	} else {
+		TRYQ(ssxk_gather_u(st, nu, cs->rep, x, cs->x));
+		TRYQ(ssxk_csr_pupd(st, nu, cs->rowptr, cs->col, val, cs->x, cs->p1,
+				NULL, mask, c->scal, 0, -1, 1, cs->ap, NULL, NULL));
+		TRYQ(ssxk_copy(st, cs->r, cs->f, (size_t) nu));
+		TRYQ(ssxk_axpby(st, (size_t) nu, -1.0, cs->ap, NULL, 1.0, cs->r));
+	}
+	if (pres)
+		TRYQ(unull(c, cs->r));
+	TRYQ(ssxk_mul(st, (size_t) nu, cs->r, dinv, cs->z));
+	if (pres)
+		TRYQ(unull(c, cs->z));
+	TRYQ(ssxk_copy(st, cs->p0, cs->z, (size_t) nu));
+	TRYQ(udot(c, cs->r, cs->z, o + CG_RDZ));
+	TRYQ(ssxk_set_ints(st, flags, 0, 2));
+	double eps_dev = eps;
+	if (rtol > 0.0) {
+		TRYQ(udot(c, cs->r, cs->r, o + CG_RR));
+		TRYQ(ssxk_cg_tol(st, c->scal, o + CG_TOL, o + CG_RR, eps, rtol));
+		eps_dev = -1.0;
+	}
+	const long long nbs = ssxk_csr_blocks(nu);
+	/* one node per lane: no idle workgroups on the small level */
+	const int nbu = (nu + 255) / 256 < SSX_NPART ? (nu + 255) / 256
+		: SSX_NPART;
+	double *pa = cs->p0, *pb = cs->p1;      /* p before / after update */
+	int launched = 0;
+	int chunk = c->cg_hint[l] > 0 ? c->cg_hint[l] + 1 : 8;
+	for (;;) {
+		if (chunk > max_iter - launched)
+			chunk = max_iter - launched;
+		for (int k = 0; k < chunk; ++k) {
+			const int first = launched + k == 0;
+			TRYQ(ssxk_csr_pupd(st, nu, cs->rowptr, cs->col, val, pa, pb,
+					cs->z, mask, c->scal, o + CG_BETA,
+					pres ? o + CG_NULLC : -1, first, cs->ap, c->part, flags));
+			TRYQ(ssxk_cg_reduce(st, 0, c->part, nbs, c->scal, o, flags,
+					eps_dev, max_iter, (double) nu, pres));
+			TRYQ(ssxk_cg_update_nb(st, (size_t) nu, c->scal, o + CG_ALPHA,
+					pb, cs->ap, cs->x, cs->r, cs->z, dinv, cs->ones, pres,
+					c->part, flags, nbu));
+			TRYQ(ssxk_cg_reduce(st, 1, c->part, nbu, c->scal, o, flags,
+					eps_dev, max_iter, (double) nu, pres));
+			double *t = pa;
+			pa = pb;
+			pb = t;
+		}
+		launched += chunk;
+		hipStream_t hs = (hipStream_t) st;
+		HIPQ(hipMemcpyAsync(c->h_flags, flags, 2 * sizeof(int),
+				hipMemcpyDeviceToHost, hs));
+		HIPQ(hipMemcpyAsync(c->h_scal + o + CG_RR, c->scal + o + CG_RR,
+				sizeof(double), hipMemcpyDeviceToHost, hs));
+		HIPQ(hipStreamSynchronize(hs));
+		if (c->h_flags[0] || launched >= max_iter)
+			break;
+		chunk = 4;
+	}
+	TRYQ(ssxk_scatter_u(st, L->ntot, cs->cmap, cs->x, x));
+	const int it = c->h_flags[1];
+	const double err = sqrt(c->h_scal[o + CG_RR]);
+	c->cg_hint[l] = it;
+	if (iters)
+		*iters = it;
+	*err_out = err;
+	if (isnan(err) || isinf(err)) {
+		ssx_set_error("coarse CG level %d: residual is %g", l, err);
+		return SSX_ENUMERIC;
+	}
+	return SSX_OK;
+}

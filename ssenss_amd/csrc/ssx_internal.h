@@ -46,6 +46,19 @@ typedef struct {
 	double dsvv[SSX_NMAX * SSX_NMAX];  /* same layout                   */
 } ssx_basis;
 
+/* the coarsest multigrid level's operators assembled over unique nodes
+ * (CSR, FP64) and the work vectors of its CG in that space */
+typedef struct ssx_coarse {
+	int nu;                      /* unique nodes                          */
+	long long nnz;
+	int *rowptr, *col;           /* device CSR structure                  */
+	double *valp, *valh;         /* Poisson / Helmholtz values            */
+	int *rep;                    /* [nu] one local copy of each node      */
+	int *cmap;                   /* [ntot] unique node of each copy       */
+	double *dinvp, *dinvh, *mask, *ones;
+	double *f, *x, *r, *z, *p0, *p1, *ap;
+} ssx_coarse;
+
 /* one multigrid level on the device */
 typedef struct {
 	int n, n3;
@@ -63,6 +76,7 @@ typedef struct {
 	long long ngrp[3];
 	int *bc;                     /* per element Dirichlet bits           */
 	double nunique;              /* <1,1>_w                              */
+	ssx_coarse cs;               /* coarsest level (l > 0) only: nu > 0   */
 	int axv[3];                  /* operator launch variant per precision
 	                                (SSX_PREC_*), set by the autotuner   */
 	int fused_ok;                /* multiplicity > 1 exactly on faces with
@@ -100,6 +114,7 @@ struct ssx_ctx {
 	int cg_hint[SSX_LMAX];       /* last iteration count per level       */
 	int persist;                 /* SSX_OPT_PERSISTENT_CG                */
 	int fused_red;               /* SSX_OPT_FUSED_REDUCE                 */
+	int coarse_csr;              /* SSX_OPT_COARSE_ASSEMBLED             */
 	unsigned *redctr;            /* device: last-workgroup counter       */
 	int persist_grid;            /* blocks of the cooperative CG launch  */
 	unsigned *pbar;              /* device: grid-barrier words [4]       */
@@ -117,6 +132,28 @@ struct ssx_ctx {
 	long long cnt[4];
 	double ax_us[SSX_LMAX][3][SSX_AX_NVARIANTS];  /* autotuner timings  */
 };
+
+/* ---- assembled coarsest level (ssx_coarse.c) ---- */
+int ssx_coarse_build(ssx_ctx *c, int l, const double *gf, const double *mass,
+		const double *dp, const double *dh, const double *mask);
+void ssx_coarse_free(ssx_coarse *cs);
+/* Jacobi-PCG (solver.c:33-81) of level l in unique-node space: f, x
+ * E-vectors of the level (f averaged over copies; x_zero: x is zero on
+ * entry); stops at max(eps, rtol |r0|) or max_iter */
+int ssx_coarse_cg(ssx_ctx *c, int l, int op, int var, const double *f,
+		double *x, double eps, double rtol, int max_iter, int x_zero,
+		int *iters, double *err);
+int ssxk_gather_u(void *stream, int nu, const int *rep, const double *e,
+		double *u);
+int ssxk_scatter_u(void *stream, size_t ntot, const int *cmap,
+		const double *u, double *e);
+/* ap = mask (A pn), pn = first ? pold : mask ((z - c) + beta pold) written
+ * to pnew; part[blk] = partial sum of pn ap (unique nodes: weight 1) */
+int ssxk_csr_pupd(void *stream, int nu, const int *rowptr, const int *col,
+		const double *val, const double *pold, double *pnew,
+		const double *z, const double *mask, const double *scal, int ib,
+		int ic, int first, double *ap, double *part, const int *done);
+int ssxk_csr_blocks(int nu);
 
 /* ---- host setup (ssx_setup.c) ---- */
 void ssx_basis_init(ssx_basis *b, int n, int svvn, double svva);
@@ -194,7 +231,14 @@ int ssxk_cg_update_red_lp(void *stream, size_t n, const double *scal, int ia,
 		const void *p, const void *ap, void *u, void *r, void *z,
 		const void *dinv, const void *scale, int pressure, double *part,
 		const int *done, const ssx_lastred *lr);
-/* eps < 0: the stopping tolerance is read from scal[o + CG_TOL] */
+/* ssxk_cg_update on a grid of nblocks <= SSX_NPART workgroups (the mode-1
+ * reduce then takes cnt = nblocks) */
+int ssxk_cg_update_nb(void *stream, size_t n, const double *scal, int ia,
+		const double *p, const double *ap, double *u, double *r, double *z,
+		const double *dinv, const double *scale, int pressure, double *part,
+		const int *done, int nblocks);
+/* eps < 0: the stopping tolerance is read from scal[o + CG_TOL]; mode 1
+ * sums cnt partials per quantity (0: SSX_NPART) */
 int ssxk_cg_reduce(void *stream, int mode, const double *part, long long cnt,
 		double *scal, int o, int *flags, double eps, int maxit,
 		double nunique, int pressure);

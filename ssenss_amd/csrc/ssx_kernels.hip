@@ -419,7 +419,7 @@ k_cg_reduce(int mode, const double *__restrict__ part, long long cnt,
 		if (c < nq) {
 			const double *pp = mode == 0 ? part
 				: part + (size_t) c * SSX_NPART;
-			const long long m = mode == 0 ? cnt : SSX_NPART;
+			const long long m = (mode == 0 || cnt > 0) ? cnt : SSX_NPART;
 			for (long long q = threadIdx.x; q < m; q += 1024)
 				acc += pp[q];
 		}
@@ -461,7 +461,8 @@ template <typename T>
 static int launch_cg_update(hipStream_t st, size_t n, const double *scal,
 		int ia, const void *p, const void *ap, void *u, void *r, void *z,
 		const void *dinv, const void *scale, int pressure, double *part,
-		const int *done, const ssx_lastred *lr = nullptr)
+		const int *done, const ssx_lastred *lr = nullptr,
+		int nblocks = SSX_NPART)
 {
 	if (lr)
 		hipLaunchKernelGGL((k_cg_update<T, true>), dim3(SSX_NPART), dim3(256),
@@ -469,12 +470,25 @@ static int launch_cg_update(hipStream_t st, size_t n, const double *scal,
 				(T *) r, (T *) z, (const T *) dinv, (const T *) scale,
 				pressure, part, done, *lr);
 	else
-		hipLaunchKernelGGL((k_cg_update<T, false>), dim3(SSX_NPART),
+		hipLaunchKernelGGL((k_cg_update<T, false>), dim3(nblocks),
 				dim3(256), 0, st, n, scal, ia, (const T *) p, (const T *) ap,
 				(T *) u, (T *) r, (T *) z, (const T *) dinv,
 				(const T *) scale, pressure, part, done, ssx_lastred{});
 	LAUNCH_CHECK("k_cg_update");
 	return SSX_OK;
+}
+
+extern "C" int ssxk_cg_update_nb(void *stream, size_t n, const double *scal,
+		int ia, const double *p, const double *ap, double *u, double *r,
+		double *z, const double *dinv, const double *scale, int pressure,
+		double *part, const int *done, int nblocks)
+{
+	if (nblocks < 1 || nblocks > SSX_NPART) {
+		ssx_set_error("ssxk_cg_update_nb: %d workgroups", nblocks);
+		return SSX_EINVAL;
+	}
+	return launch_cg_update<double>((hipStream_t) stream, n, scal, ia, p, ap,
+			u, r, z, dinv, scale, pressure, part, done, nullptr, nblocks);
 }
 
 extern "C" int ssxk_cg_update_red(void *stream, size_t n, const double *scal,
@@ -913,6 +927,114 @@ extern "C" int ssxk_fcg_step(void *stream, size_t n, const double *scal,
 	hipLaunchKernelGGL(k_fcg_step, dim3(SSX_NPART), dim3(256), 0,
 			(hipStream_t) stream, n, scal, ia, p, ap, u, r, s, part);
 	LAUNCH_CHECK("k_fcg_step");
+	return SSX_OK;
+}
+
+// ---------------------------------------------------------------------
+// Coarsest level over unique nodes (ssx_coarse.c)
+// ---------------------------------------------------------------------
+
+__global__ void k_gather_u(int nu, const int *__restrict__ rep,
+		const double *__restrict__ e, double *__restrict__ u)
+{
+	GRID_STRIDE(q, (size_t) nu) u[q] = e[rep[q]];
+}
+
+__global__ void k_scatter_u(size_t ntot, const int *__restrict__ cmap,
+		const double *__restrict__ u, double *__restrict__ e)
+{
+	GRID_STRIDE(q, ntot) e[q] = u[cmap[q]];
+}
+
+// CSR_LPR lanes per row: pn = first ? pold : mask ((z - c) + beta pold) at
+// the row and, recomputed with the same arithmetic, at every column read
+// (the k_cg_pupdate formula); ap = mask (A pn); part = block sum of pn ap
+#define CSR_LPR 8
+__device__ __forceinline__ double csr_pn(const double *pold,
+		const double *z, const double *mask, int j, int first, double b,
+		double cc, int ic)
+{
+	double pj = pold[j];
+	if (!first) {
+		const double zv = ic >= 0 ? cc * 1.0 + z[j] : z[j];
+		pj = b * pj + zv;
+		if (mask)
+			pj = pj * mask[j];
+	}
+	return pj;
+}
+
+__global__ void __launch_bounds__(256)
+k_csr_pupd(int nu, const int *__restrict__ rowptr,
+		const int *__restrict__ col, const double *__restrict__ val,
+		const double *__restrict__ pold, double *__restrict__ pnew,
+		const double *__restrict__ z, const double *__restrict__ mask,
+		const double *__restrict__ scal, int ib, int ic, int first,
+		double *__restrict__ ap, double *__restrict__ part,
+		const int *__restrict__ done)
+{
+	if (done && *done)
+		return;
+	__shared__ double sh[256];
+	const double b = first ? 0.0 : scal[ib];
+	const double cc = (!first && ic >= 0) ? -scal[ic] : 0.0;
+	const int lane = threadIdx.x % CSR_LPR;
+	const int u = (blockIdx.x * 256 + threadIdx.x) / CSR_LPR;
+	double pd = 0.0;
+	double y = 0.0;
+	if (u < nu)
+		for (int k = rowptr[u] + lane; k < rowptr[u + 1]; k += CSR_LPR)
+			y += val[k] * csr_pn(pold, z, mask, col[k], first, b, cc, ic);
+	// lanes of a row are adjacent in the wavefront: fixed-order fold
+	for (int o = 1; o < CSR_LPR; o <<= 1)
+		y += __shfl_xor(y, o, CSR_LPR);
+	if (u < nu && lane == 0) {
+		const double pu = csr_pn(pold, z, mask, u, first, b, cc, ic);
+		pnew[u] = pu;
+		if (mask && mask[u] == 0.0)
+			y = 0.0;
+		ap[u] = y;
+		pd = pu * y;
+	}
+	if (part) {
+		const double t = block_sum256(pd, sh);
+		if (threadIdx.x == 0)
+			part[blockIdx.x] = t;
+	}
+}
+
+extern "C" int ssxk_csr_blocks(int nu)
+{
+	return (int) (((long long) nu * CSR_LPR + 255) / 256);
+}
+
+extern "C" int ssxk_gather_u(void *stream, int nu, const int *rep,
+		const double *e, double *u)
+{
+	hipLaunchKernelGGL(k_gather_u, vgrid((size_t) nu), dim3(256), 0,
+			(hipStream_t) stream, nu, rep, e, u);
+	LAUNCH_CHECK("k_gather_u");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_scatter_u(void *stream, size_t ntot, const int *cmap,
+		const double *u, double *e)
+{
+	hipLaunchKernelGGL(k_scatter_u, vgrid(ntot), dim3(256), 0,
+			(hipStream_t) stream, ntot, cmap, u, e);
+	LAUNCH_CHECK("k_scatter_u");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_csr_pupd(void *stream, int nu, const int *rowptr,
+		const int *col, const double *val, const double *pold, double *pnew,
+		const double *z, const double *mask, const double *scal, int ib,
+		int ic, int first, double *ap, double *part, const int *done)
+{
+	hipLaunchKernelGGL(k_csr_pupd, dim3(ssxk_csr_blocks(nu)), dim3(256), 0,
+			(hipStream_t) stream, nu, rowptr, col, val, pold, pnew, z, mask,
+			scal, ib, ic, first, ap, part, done);
+	LAUNCH_CHECK("k_csr_pupd");
 	return SSX_OK;
 }
 

@@ -12,6 +12,7 @@ OPT_SMOOTHER_PRECISION = 2
 OPT_PERSISTENT_CG = 3
 OPT_LP_COARSE_RTOL = 4
 OPT_FUSED_REDUCE = 5
+OPT_COARSE_ASSEMBLED = 6
 AX_NVARIANTS = 8
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 (DATA_GFAC, DATA_MASS, DATA_DIAG_P, DATA_DIAG_H, DATA_SCALE, DATA_MASK,

@@ -455,6 +455,7 @@ def test_persistent_coarse_cg(name, grid):
     sc = Z["ref_scalars"]
     eps = sc[0]
     with make_ctx(mesh) as ctx:
+        ctx.set_option(S.OPT_COARSE_ASSEMBLED, 0)     # the E-vector CG
         lc = ctx.nlevels - 1
         rng = np.random.default_rng(5)
         fc = rng.standard_normal(ctx.ntot(lc))
@@ -681,3 +682,60 @@ def test_last_workgroup_reductions_bit_identical(name):
     for a, b in zip(out[1], out[0]):
         assert a[0] == b[0]
         assert np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("name", ["cube2_n8", "periodic3_n6", "channel_n6",
+                                  "cube2_n12"])
+def test_assembled_coarse_level(name):
+    """the coarsest level solved over unique nodes with its operator
+    assembled once (CSR) against the E-vector CG with element operator and
+    DSS: coarse Jacobi-PCG for both operators and variable kinds (iteration
+    counts within one, solutions within the tolerance, the same value on
+    every copy of a node), and the full FCG + p-MG solves against the
+    reference's iteration counts and solutions, in FP64 and FP32"""
+    Z = golden(name)
+    mesh = golden_mesh(Z)
+    sc = Z["ref_scalars"]
+    eps = sc[0]
+    with make_ctx(mesh) as ctx:
+        lc = ctx.nlevels - 1
+        rng = np.random.default_rng(8)
+        fc = rng.standard_normal(ctx.ntot(lc))
+        res = {}
+        for opt in (0, 1):
+            ctx.set_option(S.OPT_COARSE_ASSEMBLED, opt)
+            runs = []
+            for op, var in ((S.OP_POISSON, S.PRESSURE),
+                            (S.OP_HELMHOLTZ, S.X_VELOCITY)):
+                f = dev(fc)
+                ctx.dss_average(lc, f)
+                u = torch.zeros_like(f)
+                st = ctx.cg(lc, op, var, f, u, 1e-9 * float(f.norm()), 500)
+                runs.append((st.iterations, host(u)))
+            for var, diag in ((S.PRESSURE, -1), (S.X_VELOCITY, 0)):
+                ctx.set_diag_choice(diag)
+                f = dev(Z["in_f"])
+                v = torch.zeros_like(f)
+                st = ctx.fcg(S.OP_POISSON, var, S.PC_MG, f, v, eps, 5000)
+                runs.append((st.iterations, host(v), st.coarse_iters))
+            ctx.set_diag_choice(-1)
+            res[opt] = runs
+        ctx.set_option(S.OPT_SMOOTHER_PRECISION, S.PREC_FP32)
+        ctx.set_diag_choice(0)
+        f = dev(Z["in_f"])
+        v = torch.zeros_like(f)
+        st32 = ctx.fcg(S.OP_POISSON, S.X_VELOCITY, S.PC_MG, f, v, eps, 5000)
+        v32 = host(v)
+    a, b = res[1], res[0]
+    print(f"[gpu] assembled coarse {name}: coarse cg {a[0][0]}/{b[0][0]}, "
+          f"{a[1][0]}/{b[1][0]}; fcg {a[2][0]}/{b[2][0]} (coarse "
+          f"{a[2][2]}/{b[2][2]}), {a[3][0]}/{b[3][0]}; fp32 {st32.iterations}")
+    for k in range(4):
+        assert abs(a[k][0] - b[k][0]) <= 1, k
+    for k in range(2):
+        assert rel(a[k][1], b[k][1]) <= 1e-7, k
+    assert abs(a[2][0] - int(sc[3])) <= 1 and abs(a[3][0] - int(sc[5])) <= 1
+    assert rel(a[2][1], Z["ref_fcg_pressure"]) <= 1e-8
+    assert rel(a[3][1], Z["ref_fcg_dirichlet"]) <= 1e-8
+    assert st32.converged and st32.iterations <= int(sc[5]) + 3
+    assert rel(v32, Z["ref_fcg_dirichlet"]) <= 1e-7
