@@ -137,6 +137,11 @@ int ssx_mesh_box(int n, int mx, int my, int mz, const double lo[3],
  * layout of SSX_DATA_GFAC) and w*detJ (nel*n^3) of a mesh (reference
  * geometry.c:102-135, poisson.c:58-66). */
 int ssx_dss_host(const ssx_mesh *mesh, int n, double *u);
+/* ssx_dss_gather_host: the same sum in the gather form the fused CG update
+ * evaluates on the device (each copy adds its partners' values, read from
+ * per-copy partner lists, in its own add order); equal to ssx_dss_host bit
+ * for bit. */
+int ssx_dss_gather_host(const ssx_mesh *mesh, int n, double *u);
 int ssx_geometry_host(const ssx_mesh *mesh, double *gfac, double *mass);
 
 /* ---- context ---- */
@@ -161,8 +166,8 @@ int ssx_level_data(ssx_ctx *ctx, int level, int which, double *host);
 int ssx_dss_groups(const ssx_ctx *ctx, int level, long long counts[3]);
 /* Jacobi diagonal choice: 0 Poisson, 1 Helmholtz, -1 by var (default) */
 int ssx_set_diag_choice(ssx_ctx *ctx, int choice);
-/* options: SSX_OPT_FUSED_CG 1 (default) runs each CG iteration as six fused
- * kernels with the stopping test on the device; 0 runs the plain
+/* options: SSX_OPT_FUSED_CG 1 (default) runs each CG iteration as four or
+ * five fused kernels with the stopping test on the device; 0 runs the plain
  * one-kernel-per-vector-operation loop with a host check per iteration */
 #define SSX_OPT_FUSED_CG 1
 /* SSX_OPT_SMOOTHER_PRECISION: precision of the p-multigrid smoother and
@@ -197,6 +202,11 @@ int ssx_set_diag_choice(ssx_ctx *ctx, int choice);
  * element operator and DSS; 0 keeps the E-vector CG.  Same algorithm and
  * stopping test; sums in another order (equal to rounding). */
 #define SSX_OPT_COARSE_ASSEMBLED 6
+/* SSX_OPT_GATHER_DSS: 1 (default) assembles A p inside the fused CG update:
+ * every copy adds its partners' values in its own add order, read from
+ * per-copy partner lists, so the DSS launch and its write-back go away;
+ * 0 runs the group DSS as a launch of its own.  Bit-identical results. */
+#define SSX_OPT_GATHER_DSS 7
 #define SSX_PREC_FP64 0
 #define SSX_PREC_FP32 1
 #define SSX_PREC_BF16 2

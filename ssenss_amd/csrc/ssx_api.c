@@ -342,6 +342,29 @@ static int setup_level(ssx_ctx *c, const ssx_mesh *m, int l, double *hx,
 		}
 		free(bcb);
 	}
+	if (L->fused_ok) {       /* partner lists for the gather-form DSS */
+		int *tab = NULL, *soff = NULL, P = 0;
+		rc = ssx_build_gather(m, n, &tab, &soff, &P);
+		if (rc)
+			goto out;
+		const size_t tb = sizeof(int) * (size_t) c->nel * (P ? P : 1);
+		if (hipMalloc((void **) &L->gtab, tb) != hipSuccess
+				|| hipMalloc((void **) &L->gsoff, sizeof(int) * n3)
+					!= hipSuccess
+				|| (P && hipMemcpy(L->gtab, tab, sizeof(int) * (size_t)
+					c->nel * P, hipMemcpyHostToDevice) != hipSuccess)
+				|| hipMemcpy(L->gsoff, soff, sizeof(int) * n3,
+					hipMemcpyHostToDevice) != hipSuccess) {
+			free(tab);
+			free(soff);
+			rc = SSX_EHIP;
+			ssx_set_error("upload gather tables");
+			goto out;
+		}
+		free(tab);
+		free(soff);
+		L->gP = P;
+	}
 	rc = dalloc(&L->r, nt);
 	if (!rc) rc = dalloc(&L->z, nt);
 	if (!rc) rc = dalloc(&L->p, nt);
@@ -377,6 +400,7 @@ int ssx_create(ssx_ctx **out, const ssx_mesh *m)
 	c->persist = 0;
 	c->fused_red = 0;
 	c->coarse_csr = 1;
+	c->gather_upd = 1;
 	c->lp_coarse_rtol = pow(10.0, -SSX_LP_COARSE_RTOL_EXP);
 	c->lambda = (11.0 / 6.0) / (m->dt * m->visc);
 	int nl = SSX_LMAX;
@@ -451,7 +475,8 @@ int ssx_create(ssx_ctx **out, const ssx_mesh *m)
 		const size_t nt = c->lev[0].ntot;
 		size_t np = (size_t) 12 * SSX_NPART;   /* FCG: 10 dots at once */
 		for (int l = 0; l < nl; ++l) {
-			size_t q = (size_t) ssxk_ax_blocks(c->lev[l].n, c->nel, 1)
+			/* one operator partial per element, then DSS partials */
+			size_t q = (size_t) c->nel
 				+ (size_t) ssxk_dss_blocks(c->lev[l].ngrp);
 			if (q > np)
 				np = q;
@@ -508,6 +533,7 @@ void ssx_destroy(ssx_ctx *c)
 		hipFree(L->dmat); hipFree(L->prol); hipFree(L->bc);
 		for (int mi = 0; mi < 3; ++mi)
 			hipFree(L->grp[mi]);
+		hipFree(L->gtab); hipFree(L->gsoff);
 		hipFree(L->r); hipFree(L->z); hipFree(L->p); hipFree(L->ap);
 		hipFree(L->mg_r); hipFree(L->mg_fn); hipFree(L->mg_e);
 		ssx_coarse_free(&L->cs);
@@ -594,6 +620,9 @@ int ssx_set_option(ssx_ctx *c, int option, int value)
 		return SSX_OK;
 	case SSX_OPT_COARSE_ASSEMBLED:
 		c->coarse_csr = value != 0;
+		return SSX_OK;
+	case SSX_OPT_GATHER_DSS:
+		c->gather_upd = value != 0;
 		return SSX_OK;
 	case SSX_OPT_FUSED_REDUCE:
 		c->fused_red = value != 0;
@@ -930,7 +959,7 @@ static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
 			return SSX_EHIP;
 		}
 	} else {
-	const long long nba = ssxk_ax_blocks(L->n, c->nel, L->axv[0]);
+	const long long nba = c->nel;   /* operator partials: one per element */
 	const long long nbd = ssxk_dss_blocks(L->ngrp);
 	const ssx_lastred lr0 = { c->redctr, c->part, nba, c->scal, o,
 		max_iter, pres, flags, eps_dev, (double) L->nunique };
@@ -961,11 +990,22 @@ static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
 						u, r, z, dinv, L->scale, pres, c->part, flags, &lr1));
 				continue;
 			}
-			TRY(ssxk_dss_all(st, L->grp, L->ngrp, ap, NULL, NULL, 0, flags));
-			TRY(ssxk_cg_reduce(st, 0, c->part, nba, c->scal, o, flags,
-					eps_dev, max_iter, L->nunique, pres));
-			TRY(ssxk_cg_update(st, nt, c->scal, o + CG_ALPHA, p, ap, u, r, z,
-					dinv, L->scale, pres, c->part, flags));
+			if (c->gather_upd && L->gP > 0) {
+				/* alpha needs only the operator's partials; the
+				 * update then assembles Ap as it reads it */
+				TRY(ssxk_cg_reduce(st, 0, c->part, nba, c->scal, o, flags,
+						eps_dev, max_iter, L->nunique, pres));
+				TRY(ssxk_cg_update_g(st, nt, L->n3, L->gP, L->gsoff,
+						L->gtab, c->scal, o + CG_ALPHA, p, ap, u, r, z,
+						dinv, pres, c->part, flags));
+			} else {
+				TRY(ssxk_dss_all(st, L->grp, L->ngrp, ap, NULL, NULL, 0,
+						flags));
+				TRY(ssxk_cg_reduce(st, 0, c->part, nba, c->scal, o, flags,
+						eps_dev, max_iter, L->nunique, pres));
+				TRY(ssxk_cg_update(st, nt, c->scal, o + CG_ALPHA, p, ap, u,
+						r, z, dinv, L->scale, pres, c->part, flags));
+			}
 			TRY(ssxk_cg_reduce(st, 1, c->part, 0, c->scal, o, flags,
 					eps_dev, max_iter, L->nunique, pres));
 		}
@@ -1168,7 +1208,7 @@ static int cg_level_lp(ssx_ctx *c, int l, int op, int var, float *f,
 	TRY(ssxk_dot_lp(st, nt, r, L->scale_lp, z, c->part, 0));
 	TRY(ssxk_reduce(st, c->part, 0, c->scal, o + CG_RDZ));
 	TRY(ssxk_set_ints(st, flags, 0, 2));
-	const long long nba = ssxk_ax_blocks(L->n, c->nel, L->axv[L->lp_prec]);
+	const long long nba = c->nel;   /* operator partials: one per element */
 	for (int k = 0; k < max_iter; ++k) {
 		if (k == 0)
 			TRY(ssxk_ax_lp(st, L->lp_prec, L->n, L->axv[L->lp_prec], c->nel,
@@ -1192,11 +1232,20 @@ static int cg_level_lp(ssx_ctx *c, int l, int op, int var, float *f,
 					r, z, dinv, L->scale_lp, pres, c->part, flags, &lr1));
 			continue;
 		}
-		TRY(ssxk_dss_all_lp(st, L->grp, L->ngrp, ap, NULL, NULL, 0, flags));
-		TRY(ssxk_cg_reduce(st, 0, c->part, nba, c->scal, o, flags, eps,
-				max_iter, L->nunique, pres));
-		TRY(ssxk_cg_update_lp(st, nt, c->scal, o + CG_ALPHA, p, ap, u, r, z,
-				dinv, L->scale_lp, pres, c->part, flags));
+		if (c->gather_upd && L->gP > 0) {
+			TRY(ssxk_cg_reduce(st, 0, c->part, nba, c->scal, o, flags, eps,
+					max_iter, L->nunique, pres));
+			TRY(ssxk_cg_update_g_lp(st, nt, L->n3, L->gP, L->gsoff, L->gtab,
+					c->scal, o + CG_ALPHA, p, ap, u, r, z, dinv, pres,
+					c->part, flags));
+		} else {
+			TRY(ssxk_dss_all_lp(st, L->grp, L->ngrp, ap, NULL, NULL, 0,
+					flags));
+			TRY(ssxk_cg_reduce(st, 0, c->part, nba, c->scal, o, flags, eps,
+					max_iter, L->nunique, pres));
+			TRY(ssxk_cg_update_lp(st, nt, c->scal, o + CG_ALPHA, p, ap, u, r,
+					z, dinv, L->scale_lp, pres, c->part, flags));
+		}
 		TRY(ssxk_cg_reduce(st, 1, c->part, 0, c->scal, o, flags, eps,
 				max_iter, L->nunique, pres));
 	}
@@ -1449,6 +1498,35 @@ int ssx_dss_host(const ssx_mesh *m, int n, double *u)
 	ssx_host_dss(grp, ng, u);
 	for (int i = 0; i < 3; ++i)
 		free(grp[i]);
+	return SSX_OK;
+}
+
+int ssx_dss_gather_host(const ssx_mesh *m, int n, double *u)
+{
+	if (!m || !u || n < SSX_NMIN || n > SSX_NMAX || m->nel < 1)
+		return SSX_EINVAL;
+	const int n3 = n * n * n;
+	const size_t nt = (size_t) m->nel * n3;
+	int *tab, *soff, P;
+	TRY(ssx_build_gather(m, n, &tab, &soff, &P));
+	double *v = malloc(sizeof(double) * nt);
+	if (!v) {
+		free(tab);
+		free(soff);
+		return SSX_ENOMEM;
+	}
+	memcpy(v, u, sizeof(double) * nt);
+	for (size_t q = 0; q < nt; ++q) {
+		const int code = soff[q % n3];
+		const int *pt = tab + (q / n3) * (size_t) P + (code >> 3);
+		double s = v[q];
+		for (int k = 0; k < (code & 7) && pt[k] >= 0; ++k)
+			s = s + v[pt[k]];
+		u[q] = s;
+	}
+	free(v);
+	free(tab);
+	free(soff);
 	return SSX_OK;
 }
 

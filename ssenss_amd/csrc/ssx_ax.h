@@ -207,24 +207,41 @@ __device__ __forceinline__ double ax_team(int eblk, int nel,
 	return pd;
 }
 
-// block sum of one value per lane (any block size <= 256), to part[slot]
-__device__ __forceinline__ void ax_block_partial(double pd, double *part,
-		long long slot)
+// One partial per element: the N*N lanes of each team are summed by a
+// fixed tree over a power-of-two segment padded with zeros, and element e's
+// sum goes to part[e].  The tree does not depend on how many teams share a
+// workgroup, so every launch shape gives the same partials (the reductions
+// that follow, and with them the whole solve, do not depend on the shape
+// the autotuner picked).
+template <int N, int V>
+__device__ __forceinline__ void ax_elem_partial(double pd, double *part,
+		int eblk, int nel)
 {
-	__shared__ double s_red[256];
-	for (int q = threadIdx.x; q < 256; q += blockDim.x)
-		s_red[q] = 0.0;
-	__syncthreads();
-	s_red[threadIdx.x] = pd;
+	using S = AxShape<N, V>;
+	constexpr int N2 = S::N2, TEAMS = S::TEAMS, BLOCK = S::BLOCK;
+	constexpr int P2 = N2 <= 4 ? 4 : N2 <= 16 ? 16 : N2 <= 32 ? 32
+		: N2 <= 64 ? 64 : N2 <= 128 ? 128 : 256;
+	static_assert(N2 <= 256, "team larger than the partial tree");
+	__shared__ double s_red[TEAMS * P2];
+	const int team = threadIdx.x / N2;
+	const int tid = threadIdx.x - team * N2;
+	s_red[team * P2 + tid] = pd;
+	if (tid < P2 - N2)
+		s_red[team * P2 + N2 + tid] = 0.0;
 	__syncthreads();
 #pragma unroll
-	for (int st = 128; st > 0; st >>= 1) {
-		if ((int) threadIdx.x < st)
-			s_red[threadIdx.x] += s_red[threadIdx.x + st];
+	for (int st = P2 / 2; st > 0; st >>= 1) {
+		for (int q = threadIdx.x; q < TEAMS * st; q += BLOCK) {
+			const int t = q / st, l = q - t * st;
+			s_red[t * P2 + l] += s_red[t * P2 + l + st];
+		}
 		__syncthreads();
 	}
-	if (threadIdx.x == 0)
-		part[slot] = s_red[0];
+	if ((int) threadIdx.x < TEAMS) {
+		const int e = eblk * TEAMS + threadIdx.x;
+		if (e < nel)
+			part[e] = s_red[threadIdx.x * P2];
+	}
 }
 
 template <int N, int OP, typename T, typename G, bool PUPD, int V>
@@ -242,7 +259,7 @@ k_ax(int nel, const T *__restrict__ u, T *__restrict__ au,
 	const double pd = ax_team<N, OP, T, G, PUPD, V>(blockIdx.x, nel, u, au,
 			gf, mass, dmat, lambda, bc, apply_mask, zin, pio, scal, ib, ic);
 	if (part)
-		ax_block_partial(pd, part, blockIdx.x);
+		ax_elem_partial<N, V>(pd, part, blockIdx.x, nel);
 }
 
 template <int N, typename T, typename G, bool PU, int V>

@@ -158,7 +158,7 @@ static int build(ssx_batch *b)
 		d->nba = (c->nel + teams - 1) / teams;
 		d->nbd = ssxk_dss_blocks(L->ngrp);
 		d->nbu = (d->ntot + SSX_BATCH_CHUNK - 1) / SSX_BATCH_CHUNK;
-		b->poff[i + 1] = b->poff[i] + (size_t) (d->nba + d->nbd
+		b->poff[i + 1] = b->poff[i] + (size_t) (d->nel + d->nbd
 				+ 4 * d->nbu);
 		TRYB(balloc((void **) &b->r[i], sizeof(double) * L->ntot));
 		TRYB(balloc((void **) &b->z[i], sizeof(double) * L->ntot));
@@ -202,7 +202,7 @@ static int build(ssx_batch *b)
 	int rc = SSX_OK;
 	for (int i = 0; i < np; ++i) {
 		const ssx_bprob *d = &b->h_prob[i];
-		b->poff[i + 1] = b->poff[i] + (size_t) (d->nba + d->nbd + 4 * d->nbu);
+		b->poff[i + 1] = b->poff[i] + (size_t) (d->nel + d->nbd + 4 * d->nbu);
 	}
 	hipFree(b->part);
 	rc = balloc((void **) &b->part, sizeof(double) * b->poff[np]);

@@ -17,7 +17,7 @@ typedef struct ssx_bprob {
 	long long G2, G4, G8;
 	long long nba, nbd, nbu;   /* operator, DSS, update workgroups */
 	double *u, *r, *z, *p, *ap;
-	double *part;              /* [nba][nbd][4][nbu] partial sums */
+	double *part;              /* [nel][nbd][4][nbu] partial sums */
 	double *scal;              /* CG_NSLOT scalars (CG_RDZ, ...)   */
 	int *flags;                /* [0] done, [1] iterations          */
 } ssx_bprob;

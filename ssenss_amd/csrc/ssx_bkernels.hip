@@ -42,7 +42,7 @@ k_bax(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 	const double pd = ax_team<N, OP, double, double, PUPD, V>(w.blk, d->nel,
 			u, d->ap, d->gf, d->mass, d->dmat, d->lambda, d->bc,
 			d->apply_mask, d->z, pio, d->scal, CG_BETA, ic);
-	ax_block_partial(pd, d->part, w.blk);
+	ax_elem_partial<N, V>(pd, d->part, w.blk, d->nel);
 }
 
 // DSS of Ap; local workgroups number the pairs, then the quadruples, then
@@ -101,7 +101,7 @@ k_bupdate(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 			acc[3] += rv * s;
 		}
 	}
-	double *base = d->part + d->nba + d->nbd;
+	double *base = d->part + d->nel + d->nbd;
 	for (int c = 0; c < 4; ++c) {
 		sh[threadIdx.x] = acc[c];
 		__syncthreads();
@@ -128,8 +128,8 @@ k_breduce(int mode, const ssx_bprob *__restrict__ pr)
 		return;
 	__shared__ double sh[4][1024];
 	const int nq = mode == 0 ? 1 : (d->pressure ? 4 : 2);
-	const double *base = d->part + (mode == 0 ? 0 : d->nba + d->nbd);
-	const long long m = mode == 0 ? d->nba : d->nbu;
+	const double *base = d->part + (mode == 0 ? 0 : d->nel + d->nbd);
+	const long long m = mode == 0 ? d->nel : d->nbu;
 	for (int c = 0; c < 4; ++c) {
 		double acc = 0.0;
 		if (c < nq)

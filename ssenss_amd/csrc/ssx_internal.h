@@ -81,6 +81,9 @@ typedef struct {
 	                                (SSX_PREC_*), set by the autotuner   */
 	int fused_ok;                /* multiplicity > 1 exactly on faces with
 	                                a neighbour: fused inner products OK */
+	int *gtab, *gsoff;           /* gather DSS: partners [nel][gP] in add
+	                                order, per-node offset<<3|capacity   */
+	int gP;                      /* 0: no gather tables on this level    */
 	/* solver scratch */
 	double *r, *z, *p, *ap, *mg_r, *mg_fn, *mg_e;
 	/* host copies kept for ssx_level_data */
@@ -115,6 +118,7 @@ struct ssx_ctx {
 	int persist;                 /* SSX_OPT_PERSISTENT_CG                */
 	int fused_red;               /* SSX_OPT_FUSED_REDUCE                 */
 	int coarse_csr;              /* SSX_OPT_COARSE_ASSEMBLED             */
+	int gather_upd;              /* SSX_OPT_GATHER_UPDATE                */
 	unsigned *redctr;            /* device: last-workgroup counter       */
 	int persist_grid;            /* blocks of the cooperative CG launch  */
 	unsigned *pbar;              /* device: grid-barrier words [4]       */
@@ -165,18 +169,21 @@ void ssx_interp(const double *xfrom, int nfrom, const double *xto, int nto,
 int ssx_build_groups(const ssx_mesh *m, int n, int *grp[3], long long ng[3]);
 /* host DSS with the groups (same order as the device kernel) */
 void ssx_host_dss(int *grp[3], const long long ng[3], double *u);
+/* per-copy partner lists in add order for the gather form of the DSS:
+ * tab [nel][P], soff [n^3] = offset << 3 | capacity (host arrays) */
+int ssx_build_gather(const ssx_mesh *m, int n, int **tab, int **soff,
+		int *P);
 void ssx_set_error(const char *fmt, ...);
 
 /* ---- kernel launchers (ssx_kernels.hip); all return 0 or SSX_EHIP ---- */
 /* bc[e]: bits 0-5 Dirichlet faces (W,E,S,N,B,T), bits 8-13 faces with a
- * neighbour; done (nullable): skip when *done; part (nullable): per block
- * <p,Ap> over nodes of multiplicity 1 */
+ * neighbour; done (nullable): skip when *done; part (nullable): part[e] =
+ * sum of u (A_e u) over element e's nodes, one partial per element
+ * whatever the launch shape v */
 int ssxk_ax(void *stream, int n, int v, int nel, int op, const double *u,
 		double *au, const double *gf, const double *mass,
 		const double *dmat, double lambda, const int *bc, int apply_mask,
 		const int *done, double *part);
-/* workgroups of one ssxk_ax launch (= partial sums it writes) */
-int ssxk_ax_blocks(int n, int nel, int v);
 /* ssxk_ax of the new CG direction: p = (z - scal[ic]) + scal[ib] p
  * (ic < 0: no null-space constant), masked where apply_mask, written back
  * to p, and ap = A p -- the k_cg_pupdate of the previous iteration fused
@@ -231,6 +238,18 @@ int ssxk_cg_update_red_lp(void *stream, size_t n, const double *scal, int ia,
 		const void *p, const void *ap, void *u, void *r, void *z,
 		const void *dinv, const void *scale, int pressure, double *part,
 		const int *done, const ssx_lastred *lr);
+/* ssxk_cg_update with the DSS of ap as a gather over per-copy partner lists
+ * (tab, soff from ssx_build_gather): ap is read unassembled and left so;
+ * weights 1/M from the lists (no scale vector).  Bit-identical to
+ * ssxk_dss_all followed by ssxk_cg_update. */
+int ssxk_cg_update_g(void *stream, size_t n, int n3, int P, const int *soff,
+		const int *tab, const double *scal, int ia, const double *p,
+		const double *ap, double *u, double *r, double *z, const double *dinv,
+		int pressure, double *part, const int *done);
+int ssxk_cg_update_g_lp(void *stream, size_t n, int n3, int P,
+		const int *soff, const int *tab, const double *scal, int ia,
+		const void *p, const void *ap, void *u, void *r, void *z,
+		const void *dinv, int pressure, double *part, const int *done);
 /* ssxk_cg_update on a grid of nblocks <= SSX_NPART workgroups (the mode-1
  * reduce then takes cnt = nblocks) */
 int ssxk_cg_update_nb(void *stream, size_t n, const double *scal, int ia,

@@ -44,6 +44,10 @@ static inline dim3 vgrid(size_t n)
 	return dim3((unsigned) b);
 }
 
+#ifndef SSX_GATHER_U
+#define SSX_GATHER_U 2
+#endif
+
 #define GRID_STRIDE(q, n) \
 	for (size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x; q < (n); \
 			q += (size_t) gridDim.x * blockDim.x)
@@ -66,12 +70,6 @@ extern "C" int ssxk_ax_dispatch_bf16(void *stream, int n, int v, int nel,
 		int op, const void *u, void *au, const void *gf, const void *mass,
 		const void *dmat, double lambda, const int *bc, int apply_mask,
 		const int *done, double *part, const void *pu);
-
-extern "C" int ssxk_ax_blocks(int n, int nel, int v)
-{
-	const int teams = ssx_ax_teams(n, v);
-	return (nel + teams - 1) / teams;
-}
 
 extern "C" int ssxk_ax(void *stream, int n, int v, int nel, int op,
 		const double *u, double *au, const double *gf, const double *mass,
@@ -379,6 +377,130 @@ k_cg_update(size_t n, const double *__restrict__ scal, int ia,
 			cg_scalars1(v[0], v[1], v[2], v[3], lr.scal, lr.o, lr.flags,
 					lr.eps, lr.maxit, lr.nunique, lr.pressure);
 	}
+}
+
+// k_cg_update with the DSS of Ap done as a gather: every copy adds its
+// partners' unassembled values to its own in its own add order (tab, the
+// order the group kernel evaluates, so the assembled value is bit-identical)
+// and takes its weight 1/M from the length of that list (M = 1, 2, 4, 8:
+// exact).  Ap stays unassembled, so no copy is written before it is read.
+// Same grid and per-thread node order as k_cg_update, so the partials are
+// identical.  Each lane takes U of its grid-stride nodes at a time and
+// issues the table and partner loads of all U before using any: the
+// partner loads depend on the table loads, which depend on soff, and one
+// node at a time leaves those three round trips unoverlapped.
+template <typename T, int U>
+__global__ void __launch_bounds__(256)
+k_cg_update_g(unsigned n, unsigned n3, int P, const int *__restrict__ soff,
+		const int *__restrict__ tab, const double *__restrict__ scal, int ia,
+		const T *__restrict__ p, const T *__restrict__ ap,
+		T *__restrict__ u, T *__restrict__ r, T *__restrict__ z,
+		const T *__restrict__ dinv, int pressure,
+		double *__restrict__ part, const int *__restrict__ done)
+{
+	if (done && *done)
+		return;
+	__shared__ double sh[4][256];
+	const T a = (T) scal[ia];
+	const unsigned stride = gridDim.x * 256u;
+	double acc[4] = { 0.0, 0.0, 0.0, 0.0 };
+	for (unsigned q0 = blockIdx.x * 256u + threadIdx.x; q0 < n;
+			q0 += U * stride) {
+		int code[U], ix[U][7];
+		T own[U], pv[U][7];
+#pragma unroll
+		for (int v = 0; v < U; ++v) {
+			const unsigned q = q0 + v * stride;
+			code[v] = 0;
+			own[v] = T(0);
+			if (q < n) {
+				const unsigned e = q / n3;
+				const int c = soff[q - e * n3];
+				code[v] = (c & 7) ? c : 0;
+				own[v] = ap[q];
+				const int *pt = tab + (size_t) e * P + (c >> 3);
+#pragma unroll
+				for (int k = 0; k < 7; ++k)
+					ix[v][k] = k < (c & 7) ? pt[k] : -1;
+			} else {
+#pragma unroll
+				for (int k = 0; k < 7; ++k)
+					ix[v][k] = -1;
+			}
+		}
+#pragma unroll
+		for (int v = 0; v < U; ++v)
+#pragma unroll
+			for (int k = 0; k < 7; ++k)
+				pv[v][k] = ix[v][k] >= 0 ? ap[ix[v][k]] : T(0);
+#pragma unroll
+		for (int v = 0; v < U; ++v) {
+			const unsigned q = q0 + v * stride;
+			if (q >= n)
+				break;
+			// own + partners left to right; unused entries are -1 and
+			// come last
+			T s = own[v];
+			int m = 1;
+#pragma unroll
+			for (int k = 0; k < 7; ++k)
+				if (ix[v][k] >= 0) {
+					s = s + pv[v][k];
+					++m;
+				}
+			const T w = T(1) / T(m);
+			u[q] = a * p[q] + u[q];
+			const T rv = -a * s + r[q];
+			r[q] = rv;
+			const T zv = rv * dinv[q];
+			z[q] = zv;
+			acc[0] += (double) (rv * w) * (double) zv;
+			acc[1] += (double) (rv * w) * (double) rv;
+			if (pressure) {
+				acc[2] += (double) (zv * w);
+				acc[3] += (double) (rv * w);
+			}
+		}
+	}
+	block_sum4(acc, sh);
+	if (threadIdx.x < 4)
+		part[(size_t) threadIdx.x * SSX_NPART + blockIdx.x] = sh[threadIdx.x][0];
+}
+
+template <typename T>
+static int launch_cg_update_g(hipStream_t st, size_t n, int n3, int P,
+		const int *soff, const int *tab, const double *scal, int ia,
+		const void *p, const void *ap, void *u, void *r, void *z,
+		const void *dinv, int pressure, double *part, const int *done)
+{
+	if (n >= (1ull << 31) || n % (size_t) n3) {
+		ssx_set_error("k_cg_update_g: %zu nodes, %d per element", n, n3);
+		return SSX_EINVAL;
+	}
+	hipLaunchKernelGGL((k_cg_update_g<T, SSX_GATHER_U>), dim3(SSX_NPART),
+			dim3(256), 0, st, (unsigned) n, (unsigned) n3, P, soff, tab, scal, ia, (const T *) p,
+			(const T *) ap, (T *) u, (T *) r, (T *) z, (const T *) dinv,
+			pressure, part, done);
+	LAUNCH_CHECK("k_cg_update_g");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_cg_update_g(void *stream, size_t n, int n3, int P,
+		const int *soff, const int *tab, const double *scal, int ia,
+		const double *p, const double *ap, double *u, double *r, double *z,
+		const double *dinv, int pressure, double *part, const int *done)
+{
+	return launch_cg_update_g<double>((hipStream_t) stream, n, n3, P, soff,
+			tab, scal, ia, p, ap, u, r, z, dinv, pressure, part, done);
+}
+
+extern "C" int ssxk_cg_update_g_lp(void *stream, size_t n, int n3, int P,
+		const int *soff, const int *tab, const double *scal, int ia,
+		const void *p, const void *ap, void *u, void *r, void *z,
+		const void *dinv, int pressure, double *part, const int *done)
+{
+	return launch_cg_update_g<float>((hipStream_t) stream, n, n3, P, soff,
+			tab, scal, ia, p, ap, u, r, z, dinv, pressure, part, done);
 }
 
 // p = (z - c) + beta p, masked (c: pressure null-space constant)

@@ -454,3 +454,72 @@ void ssx_host_dss(int *grp[3], const long long ng[3], double *u)
 		}
 	}
 }
+
+/* Per-copy partner lists for the DSS fused into the CG update.
+ *
+ * Each shared copy (e,q) keeps its partners in its own add order, i.e. the
+ * list copy_list builds (own first, then partners by slot; global.c:
+ * 241-257), so own + partners left to right is the DSS value of that copy.
+ * A node on one face has at most 1 partner, on an edge at most 3, at a
+ * corner at most 7: element e's block of *P entries holds, for every
+ * surface node in local order, that many entries, unused ones -1.
+ * soff[q] = (offset of node q in the block) << 3 | capacity. */
+int ssx_build_gather(const ssx_mesh *m, int n, int **tab, int **soff,
+		int *P)
+{
+	const int n3 = n * n * n;
+	*tab = NULL;
+	*soff = malloc(sizeof(int) * (size_t) n3);
+	if (!*soff)
+		return SSX_ENOMEM;
+	int off = 0;
+	for (int q = 0; q < n3; ++q) {
+		const int i = q % n, j = (q / n) % n, k = q / (n * n);
+		const int on = (i == 0 || i == n - 1) + (j == 0 || j == n - 1)
+			+ (k == 0 || k == n - 1);
+		const int cap = on == 0 ? 0 : on == 1 ? 1 : on == 2 ? 3 : 7;
+		(*soff)[q] = off << 3 | cap;
+		off += cap;
+	}
+	*P = off;
+	int *part = wire(m);
+	if (!part) {
+		free(*soff);
+		*soff = NULL;
+		return SSX_EMESH;
+	}
+	int *t = malloc(sizeof(int) * (size_t) m->nel * (off ? off : 1));
+	if (!t) {
+		free(part);
+		free(*soff);
+		*soff = NULL;
+		return SSX_ENOMEM;
+	}
+	long long lst[8];
+	int rc = SSX_OK;
+	for (int e = 0; e < m->nel && rc == SSX_OK; ++e)
+		for (int q = 0; q < n3; ++q) {
+			const int cap = (*soff)[q] & 7;
+			if (!cap)
+				continue;
+			int *dst = t + (size_t) e * off + ((*soff)[q] >> 3);
+			const int len = copy_list(part, n, e, q, lst);
+			if (len < 1 || len - 1 > cap) {
+				ssx_set_error("copy (%d,%d): %d partners for %d slots", e,
+						q, len - 1, cap);
+				rc = SSX_EMESH;
+				break;
+			}
+			for (int k = 0; k < cap; ++k)
+				dst[k] = k + 1 < len ? (int) lst[k + 1] : -1;
+		}
+	free(part);
+	if (rc != SSX_OK) {
+		free(t);
+		free(*soff);
+		*soff = NULL;
+		return rc;
+	}
+	*tab = t;
+	return SSX_OK;
+}

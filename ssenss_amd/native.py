@@ -13,6 +13,7 @@ OPT_PERSISTENT_CG = 3
 OPT_LP_COARSE_RTOL = 4
 OPT_FUSED_REDUCE = 5
 OPT_COARSE_ASSEMBLED = 6
+OPT_GATHER_DSS = 7
 AX_NVARIANTS = 8
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 (DATA_GFAC, DATA_MASS, DATA_DIAG_P, DATA_DIAG_H, DATA_SCALE, DATA_MASK,
@@ -24,6 +25,7 @@ ERRORS = {1: "EINVAL", 2: "ENOMEM", 3: "EHIP", 4: "ENOTCONV", 5: "EMESH",
 # every function declared in include/ssx.h
 EXPORTED_SYMBOLS = [
     "ssx_last_error", "ssx_version", "ssx_mesh_box", "ssx_dss_host",
+    "ssx_dss_gather_host",
     "ssx_geometry_host", "ssx_create",
     "ssx_destroy", "ssx_set_stream", "ssx_own_stream", "ssx_get_stream", "ssx_synchronize",
     "ssx_nlevels", "ssx_level_n", "ssx_nel", "ssx_level_ntot", "ssx_lambda",
@@ -102,6 +104,7 @@ def load():
         "ssx_mesh_box": (i, [i, i, i, i, dp, dp, ip, i, d, dp, dp, dp, ip, ip,
                              ip, ip, ip]),
         "ssx_dss_host": (i, [C.POINTER(MeshStruct), i, dp]),
+        "ssx_dss_gather_host": (i, [C.POINTER(MeshStruct), i, dp]),
         "ssx_geometry_host": (i, [C.POINTER(MeshStruct), dp, dp]),
         "ssx_create": (i, [C.POINTER(vp), C.POINTER(MeshStruct)]),
         "ssx_destroy": (None, [vp]),

@@ -70,6 +70,41 @@ def test_host_dss_bit_exact_against_reference(case):
         assert np.array_equal(1.0 / host_dss(mesh, int(nl), one), ref)
 
 
+def gather_dss(mesh, n, u):
+    u = np.array(u, np.float64, copy=True)
+    ms = mesh.struct()
+    S.native.check(S.load().ssx_dss_gather_host(
+        C.byref(ms), n, u.ctypes.data_as(C.POINTER(C.c_double))))
+    return u
+
+
+def test_gather_dss_bit_exact_against_reference(case):
+    """The gather form (per-copy partner lists, used by the fused CG update)
+    against the reference's own DSS output, bit for bit."""
+    name, Z, mesh = case
+    assert np.array_equal(gather_dss(mesh, mesh.n, Z["in_u"]), Z["ref_dss_u"])
+
+
+@pytest.mark.parametrize("n,dims,bc", [
+    (8, (5, 4, 3), (0, 0, 0, 0, 0, 0)),
+    (8, (3, 4, 5), (-1, -1, -1, -1, -1, -1)),
+    (4, (6, 5, 4), (0, 0, -1, -1, 0, 0)),
+    (2, (6, 5, 4), (-1, -1, 0, 0, -2, -2)),
+    (5, (3, 3, 3), (0, 0, 0, 0, 0, 0)),
+    (7, (1, 2, 3), (-1, -1, -1, -1, -1, -1)),
+])
+def test_gather_dss_equals_group_dss(n, dims, bc):
+    """Every level order and boundary mix: gather form == group form, bit
+    for bit, on values whose sums round differently in every add order."""
+    mesh = S.box_mesh(n, dims, bc=bc, deform=1, amp=0.1)
+    rng = np.random.default_rng(n * 100 + dims[0])
+    u = rng.standard_normal(mesh.ntot) * np.exp(rng.uniform(-20, 20,
+                                                            mesh.ntot))
+    g = gather_dss(mesh, n, u)
+    assert np.array_equal(g, host_dss(mesh, n, u))
+    assert not np.array_equal(g, u)
+
+
 def test_host_geometry_against_reference(case):
     name, Z, mesh = case
     n, nt = mesh.n, mesh.ntot
