@@ -199,8 +199,10 @@ int ssx_set_diag_choice(ssx_ctx *ctx, int choice);
 /* SSX_OPT_COARSE_ASSEMBLED: 1 (default) runs the p-multigrid coarsest-level
  * Jacobi-PCG over unique nodes with the level's operator assembled once
  * (CSR; at n = 2 a 27-point stencil) instead of on E-vectors with the
- * element operator and DSS; 0 keeps the E-vector CG.  Same algorithm and
- * stopping test; sums in another order (equal to rounding). */
+ * element operator and DSS, two launches per iteration; 2 the same with
+ * four launches per iteration (separate reductions; bit-identical to 1);
+ * 0 keeps the E-vector CG.  Same algorithm and stopping test; the
+ * E-vector form sums in another order (equal to rounding). */
 #define SSX_OPT_COARSE_ASSEMBLED 6
 /* SSX_OPT_GATHER_DSS: 1 (default) assembles A p inside the fused CG update:
  * every copy adds its partners' values in its own add order, read from

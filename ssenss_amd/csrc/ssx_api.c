@@ -619,7 +619,11 @@ int ssx_set_option(ssx_ctx *c, int option, int value)
 		c->fused = value != 0;
 		return SSX_OK;
 	case SSX_OPT_COARSE_ASSEMBLED:
-		c->coarse_csr = value != 0;
+		if (value < 0 || value > 2) {
+			ssx_set_error("coarse option %d out of [0,2]", value);
+			return SSX_EINVAL;
+		}
+		c->coarse_csr = value;
 		return SSX_OK;
 	case SSX_OPT_GATHER_DSS:
 		c->gather_upd = value != 0;
@@ -967,6 +971,9 @@ static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
 		max_iter, pres, flags, eps_dev, (double) L->nunique };
 	int launched = 0;
 	int chunk = c->cg_hint[l] > 0 ? c->cg_hint[l] + 1 : 8;
+	/* the solution step u += alpha p of each iteration is taken by the
+	 * next operator launch, from its load of p; the last one by a flush */
+	const int defer = !c->fused_red;
 	for (;;) {
 		if (chunk > max_iter - launched)
 			chunk = max_iter - launched;
@@ -982,7 +989,7 @@ static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
 						c->scal,
 						o + CG_BETA, pres ? o + CG_NULLC : -1, ap, L->gf,
 						L->mass, L->dmat, c->lambda, L->bc, !pres, flags,
-						c->part));
+						c->part, defer ? u : NULL, o + CG_ALPHA));
 			if (c->fused_red && nbd > 0) {
 				TRY(ssxk_dss_all_red(st, L->grp, L->ngrp, ap, NULL, NULL,
 						flags, &lr0));
@@ -996,15 +1003,16 @@ static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
 				TRY(ssxk_cg_reduce(st, 0, c->part, nba, c->scal, o, flags,
 						eps_dev, max_iter, L->nunique, pres));
 				TRY(ssxk_cg_update_g(st, nt, L->n3, L->gP, L->gsoff,
-						L->gtab, c->scal, o + CG_ALPHA, p, ap, u, r, z,
-						dinv, pres, c->part, flags));
+						L->gtab, c->scal, o + CG_ALPHA, p, ap,
+						defer ? NULL : u, r, z, dinv, pres, c->part, flags));
 			} else {
 				TRY(ssxk_dss_all(st, L->grp, L->ngrp, ap, NULL, NULL, 0,
 						flags));
 				TRY(ssxk_cg_reduce(st, 0, c->part, nba, c->scal, o, flags,
 						eps_dev, max_iter, L->nunique, pres));
-				TRY(ssxk_cg_update(st, nt, c->scal, o + CG_ALPHA, p, ap, u,
-						r, z, dinv, L->scale, pres, c->part, flags));
+				TRY(ssxk_cg_update(st, nt, c->scal, o + CG_ALPHA, p, ap,
+						defer ? NULL : u, r, z, dinv, L->scale, pres, c->part,
+						flags));
 			}
 			TRY(ssxk_cg_reduce(st, 1, c->part, 0, c->scal, o, flags,
 					eps_dev, max_iter, L->nunique, pres));
@@ -1020,6 +1028,8 @@ static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
 			break;
 		chunk = 4;
 	}
+	if (defer && launched > 0)
+		TRY(ssxk_cg_uflush(st, nt, c->scal, o + CG_ALPHA, p, u));
 	}
 	const int it = c->h_flags[1];
 	const double err = sqrt(c->h_scal[o + CG_RR]);
@@ -1209,6 +1219,10 @@ static int cg_level_lp(ssx_ctx *c, int l, int op, int var, float *f,
 	TRY(ssxk_reduce(st, c->part, 0, c->scal, o + CG_RDZ));
 	TRY(ssxk_set_ints(st, flags, 0, 2));
 	const long long nba = c->nel;   /* operator partials: one per element */
+	/* The solution step stays in the update here: taking it in the next
+	 * operator launch (as cg_level_fused does) made the FP32 and BF16
+	 * smoothers slower (32^3 n = 8 solve 0.256 -> 0.266 s) */
+	const int defer = 0;
 	for (int k = 0; k < max_iter; ++k) {
 		if (k == 0)
 			TRY(ssxk_ax_lp(st, L->lp_prec, L->n, L->axv[L->lp_prec], c->nel,
@@ -1220,7 +1234,7 @@ static int cg_level_lp(ssx_ctx *c, int l, int op, int var, float *f,
 					c->nel, op, z, p,
 					c->scal, o + CG_BETA, pres ? o + CG_NULLC : -1, ap,
 					L->gf_lp, L->mass_lp, L->dmat_lp, c->lambda, L->bc, !pres,
-					flags, c->part));
+					flags, c->part, defer ? u : NULL, o + CG_ALPHA));
 		if (c->fused_red && ssxk_dss_blocks(L->ngrp) > 0) {
 			const ssx_lastred lr0 = { c->redctr, c->part, nba, c->scal, o,
 				max_iter, pres, flags, eps, (double) L->nunique };
@@ -1236,19 +1250,22 @@ static int cg_level_lp(ssx_ctx *c, int l, int op, int var, float *f,
 			TRY(ssxk_cg_reduce(st, 0, c->part, nba, c->scal, o, flags, eps,
 					max_iter, L->nunique, pres));
 			TRY(ssxk_cg_update_g_lp(st, nt, L->n3, L->gP, L->gsoff, L->gtab,
-					c->scal, o + CG_ALPHA, p, ap, u, r, z, dinv, pres,
-					c->part, flags));
+					c->scal, o + CG_ALPHA, p, ap, defer ? NULL : u, r, z,
+					dinv, pres, c->part, flags));
 		} else {
 			TRY(ssxk_dss_all_lp(st, L->grp, L->ngrp, ap, NULL, NULL, 0,
 					flags));
 			TRY(ssxk_cg_reduce(st, 0, c->part, nba, c->scal, o, flags, eps,
 					max_iter, L->nunique, pres));
-			TRY(ssxk_cg_update_lp(st, nt, c->scal, o + CG_ALPHA, p, ap, u, r,
-					z, dinv, L->scale_lp, pres, c->part, flags));
+			TRY(ssxk_cg_update_lp(st, nt, c->scal, o + CG_ALPHA, p, ap,
+					defer ? NULL : u, r, z, dinv, L->scale_lp, pres, c->part,
+					flags));
 		}
 		TRY(ssxk_cg_reduce(st, 1, c->part, 0, c->scal, o, flags, eps,
 				max_iter, L->nunique, pres));
 	}
+	if (defer && max_iter > 0)
+		TRY(ssxk_cg_uflush_lp(st, nt, c->scal, o + CG_ALPHA, p, u));
 	if (iters) {
 		hipStream_t hs = (hipStream_t) st;
 		HIPC(hipMemcpyAsync(c->h_flags, flags, 2 * sizeof(int),

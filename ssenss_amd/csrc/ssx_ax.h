@@ -50,21 +50,26 @@ struct PUpd {
 	void *p;
 	const double *scal;
 	int ib, ic;
+	void *u;                     // nullable: u += scal[ia] p_old first
+	int ia;
 };
 
-// PUPD: the update is done while the element is loaded, p is written back
-// and the operator is applied to the new p (the k_cg_pupdate arithmetic);
-// u is not read.
+// PUPD (PU >= 1): the update is done while the element is loaded, p is
+// written back and the operator is applied to the new p (the k_cg_pupdate
+// arithmetic); u is not read.  PU == 2: the previous iteration's solution
+// step uio += scal[ia] p_old is taken from the same load of p_old first
+// (the CG update then leaves u alone).
 // One workgroup's elements [eblk*TEAMS, eblk*TEAMS + TEAMS): Au (masked)
 // written, and each lane's part of sum p (A_e p) over its nodes returned.
-template <int N, int OP, typename T, typename G, bool PUPD, int V>
+template <int N, int OP, typename T, typename G, int PU, int V>
 __device__ __forceinline__ double ax_team(int eblk, int nel,
 		const T *__restrict__ u, T *__restrict__ au,
 		const G *__restrict__ gf, const T *__restrict__ mass,
 		const T *__restrict__ dmat, T lambda,
 		const int *__restrict__ bc, int apply_mask,
 		const T *__restrict__ zin, T *__restrict__ pio,
-		const double *__restrict__ scal, int ib, int ic)
+		const double *__restrict__ scal, int ib, int ic,
+		T *__restrict__ uio, int ia)
 {
 	using S = AxShape<N, V>;
 	constexpr int N2 = S::N2, N3 = S::N3, P = S::P, TEAMS = S::TEAMS;
@@ -95,16 +100,20 @@ __device__ __forceinline__ double ax_team(int eblk, int nel,
 	const bool mi = ((bits & 1) && i == 0) || ((bits & 2) && i == N - 1)
 		|| ((bits & 4) && j == 0) || ((bits & 8) && j == N - 1);
 	T uc[N], acc[N];
-	if (PUPD) {
+	if (PU >= 1) {
 		const T bb = (T) scal[ib];
 		const T cc = ic >= 0 ? (T) -scal[ic] : T(0);
+		const T aa = PU == 2 ? (T) scal[ia] : T(0);
 #pragma unroll
 		for (int k = 0; k < N; ++k) {
 			T v = T(0);
 			if (act) {
 				const size_t q = eoff + k * N2 + tid;
+				const T po = pio[q];
+				if (PU == 2)
+					uio[q] = aa * po + uio[q];
 				const T zv = ic >= 0 ? cc * T(1) + zin[q] : zin[q];
-				v = bb * pio[q] + zv;
+				v = bb * po + zv;
 				const bool m = mi || ((bits & 16) && k == 0)
 					|| ((bits & 32) && k == N - 1);
 				if (apply_mask)
@@ -244,7 +253,7 @@ __device__ __forceinline__ void ax_elem_partial(double pd, double *part,
 	}
 }
 
-template <int N, int OP, typename T, typename G, bool PUPD, int V>
+template <int N, int OP, typename T, typename G, int PU, int V>
 __global__ void __launch_bounds__((AxShape<N, V>::BLOCK))
 k_ax(int nel, const T *__restrict__ u, T *__restrict__ au,
 		const G *__restrict__ gf, const T *__restrict__ mass,
@@ -252,17 +261,19 @@ k_ax(int nel, const T *__restrict__ u, T *__restrict__ au,
 		const int *__restrict__ bc, int apply_mask,
 		const int *__restrict__ done, double *__restrict__ part,
 		const T *__restrict__ zin, T *__restrict__ pio,
-		const double *__restrict__ scal, int ib, int ic)
+		const double *__restrict__ scal, int ib, int ic,
+		T *__restrict__ uio, int ia)
 {
 	if (done && *done)          // solver already converged: no-op
 		return;
-	const double pd = ax_team<N, OP, T, G, PUPD, V>(blockIdx.x, nel, u, au,
-			gf, mass, dmat, lambda, bc, apply_mask, zin, pio, scal, ib, ic);
+	const double pd = ax_team<N, OP, T, G, PU, V>(blockIdx.x, nel, u, au,
+			gf, mass, dmat, lambda, bc, apply_mask, zin, pio, scal, ib, ic,
+			uio, ia);
 	if (part)
 		ax_elem_partial<N, V>(pd, part, blockIdx.x, nel);
 }
 
-template <int N, typename T, typename G, bool PU, int V>
+template <int N, typename T, typename G, int PU, int V>
 static void launch_ax1(hipStream_t st, int nel, int op, const void *u,
 		void *au, const void *gf, const void *mass, const void *dmat,
 		double lambda, const int *bc, int apply_mask, const int *done,
@@ -275,13 +286,13 @@ static void launch_ax1(hipStream_t st, int nel, int op, const void *u,
 				dim3(S::BLOCK), 0, st, nel, (const T *) u, (T *) au,
 				(const G *) gf, (const T *) mass, (const T *) dmat, (T) lambda,
 				bc, apply_mask, done, part, (const T *) pu.z, (T *) pu.p,
-				pu.scal, pu.ib, pu.ic);
+				pu.scal, pu.ib, pu.ic, (T *) pu.u, pu.ia);
 	else
 		hipLaunchKernelGGL((k_ax<N, 0, T, G, PU, V>), dim3(grid),
 				dim3(S::BLOCK), 0, st, nel, (const T *) u, (T *) au,
 				(const G *) gf, (const T *) mass, (const T *) dmat, (T) lambda,
 				bc, apply_mask, done, part, (const T *) pu.z, (T *) pu.p,
-				pu.scal, pu.ib, pu.ic);
+				pu.scal, pu.ib, pu.ic, (T *) pu.u, pu.ia);
 }
 
 template <int N, typename T, typename G, int V>
@@ -290,13 +301,18 @@ static void launch_ax_v(hipStream_t st, int nel, int op, const void *u,
 		double lambda, const int *bc, int apply_mask, const int *done,
 		double *part, const PUpd *pu)
 {
-	if (pu)
-		launch_ax1<N, T, G, true, V>(st, nel, op, u, au, gf, mass, dmat,
+	// the deferred solution step (PU = 2) is instantiated for FP64 fields
+	// only, where the fused CG takes it
+	if (pu && pu->u && sizeof(T) == 8)
+		launch_ax1<N, T, G, (sizeof(T) == 8 ? 2 : 1), V>(st, nel, op, u, au,
+				gf, mass, dmat, lambda, bc, apply_mask, done, part, *pu);
+	else if (pu)
+		launch_ax1<N, T, G, 1, V>(st, nel, op, u, au, gf, mass, dmat,
 				lambda, bc, apply_mask, done, part, *pu);
 	else
-		launch_ax1<N, T, G, false, V>(st, nel, op, u, au, gf, mass, dmat,
+		launch_ax1<N, T, G, 0, V>(st, nel, op, u, au, gf, mass, dmat,
 				lambda, bc, apply_mask, done, part, PUpd{ nullptr, nullptr,
-				nullptr, 0, -1 });
+				nullptr, 0, -1, nullptr, 0 });
 }
 
 // dispatcher body shared by the per-precision files: n in [2,16], variant

@@ -39,9 +39,10 @@ k_bax(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 	// PUPD reads z and p and writes p; otherwise p is the operand
 	const double *u = PUPD ? nullptr : d->p;
 	double *pio = PUPD ? d->p : nullptr;
-	const double pd = ax_team<N, OP, double, double, PUPD, V>(w.blk, d->nel,
+	const double pd = ax_team<N, OP, double, double, PUPD ? 1 : 0, V>(
+			w.blk, d->nel,
 			u, d->ap, d->gf, d->mass, d->dmat, d->lambda, d->bc,
-			d->apply_mask, d->z, pio, d->scal, CG_BETA, ic);
+			d->apply_mask, d->z, pio, d->scal, CG_BETA, ic, nullptr, 0);
 	ax_elem_partial<N, V>(pd, d->part, w.blk, d->nel);
 }
 

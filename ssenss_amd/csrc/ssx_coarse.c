@@ -376,10 +376,33 @@ int ssx_coarse_cg(ssx_ctx *c, int l, int op, int var, const double *f,
 	double *pa = cs->p0, *pb = cs->p1;      /* p before / after update */
 	int launched = 0;
 	int chunk = c->cg_hint[l] > 0 ? c->cg_hint[l] + 1 : 8;
+	/* two launches per iteration (each kernel replays the reduction it
+	 * needs); SSX_OPT_COARSE_ASSEMBLED = 2 keeps the four-launch form */
+	const int two = c->coarse_csr == 1
+		&& 4 * (size_t) SSX_NPART + (size_t) nbs <= c->npart;
+	double *ps = c->part + 4 * (size_t) SSX_NPART;   /* SpMV partials */
+	double *pu = c->part;                            /* update partials */
 	for (;;) {
 		if (chunk > max_iter - launched)
 			chunk = max_iter - launched;
-		for (int k = 0; k < chunk; ++k) {
+		for (int k = 0; k < chunk && two; ++k) {
+			const int j = launched + k;
+			TRYQ(ssxk_csr_cg_spmv(st, nu, cs->rowptr, cs->col, val, pa, pb,
+					cs->z, mask, c->scal, o, j, eps_dev, max_iter,
+					(double) nu, pres, cs->ap, ps, pu, nbu, flags, 0));
+			TRYQ(ssxk_csr_cg_update(st, nu, c->scal, o, j, pb, cs->ap,
+					cs->x, cs->r, cs->z, dinv, cs->ones, pres, ps, (int) nbs,
+					pu, nbu, flags));
+			double *t = pa;
+			pa = pb;
+			pb = t;
+		}
+		if (two)      /* the stopping test of the last iteration */
+			TRYQ(ssxk_csr_cg_spmv(st, nu, cs->rowptr, cs->col, val, pa, pb,
+					cs->z, mask, c->scal, o, launched + chunk, eps_dev,
+					max_iter, (double) nu, pres, cs->ap, ps, pu, nbu, flags,
+					1));
+		for (int k = 0; k < chunk && !two; ++k) {
 			const int first = launched + k == 0;
 			TRYQ(ssxk_csr_pupd(st, nu, cs->rowptr, cs->col, val, pa, pb,
 					cs->z, mask, c->scal, o + CG_BETA,

@@ -35,7 +35,8 @@ enum {
 
 /* per-level scalar block of the fused CG (solver.c:33-81) */
 enum { CG_RDZ = 0, CG_PAP, CG_ALPHA, CG_BETA, CG_RR, CG_NULLC, CG_TOL,
-	CG_NSLOT = 8 };
+	CG_RDZ1, CG_NSLOT = 8 };     /* CG_RDZ1: <r,z> of odd iterations in
+	                                the two-launch coarse CG          */
 #define SC_CG(l) (64 + CG_NSLOT * (l))
 
 /* host-side basis of one level */
@@ -158,6 +159,28 @@ int ssxk_csr_pupd(void *stream, int nu, const int *rowptr, const int *col,
 		const double *z, const double *mask, const double *scal, int ib,
 		int ic, int first, double *ap, double *part, const int *done);
 int ssxk_csr_blocks(int nu);
+/* Two-launch coarse CG iteration j (0-based) over unique nodes.
+ * ssxk_csr_cg_spmv: every workgroup first replays the stopping test and
+ * beta of iteration j-1 from the update partials pu (nbu per quantity,
+ * stride SSX_NPART) exactly as ssxk_cg_reduce mode 1 does, workgroup 0
+ * storing the scalars (<r,z>_j in slot CG_RDZ or CG_RDZ1 by parity of j,
+ * count j, done flag); then, unless stopped, pnew = mask((z - c) + beta
+ * pold) (j = 0: pnew = pold) and ap = mask(A pnew), partials of <pnew,ap>
+ * to ps.  check != 0: the prologue only, one workgroup (the host's
+ * stopping test after a chunk; idempotent).
+ * ssxk_csr_cg_update: every workgroup forms alpha = <r,z>_j / sum(ps) as
+ * ssxk_cg_reduce mode 0 does (workgroup 0 stores PAP, ALPHA), then the
+ * ssxk_cg_update_nb step with its partials to pu.  ps and pu must not
+ * overlap. */
+int ssxk_csr_cg_spmv(void *stream, int nu, const int *rowptr, const int *col,
+		const double *val, const double *pold, double *pnew,
+		const double *z, const double *mask, double *scal, int o, int j,
+		double eps, int maxit, double nunique, int pressure, double *ap,
+		double *ps, const double *pu, int nbu, int *flags, int check);
+int ssxk_csr_cg_update(void *stream, int nu, double *scal, int o, int j,
+		const double *p, const double *ap, double *u, double *r, double *z,
+		const double *dinv, const double *scale, int pressure,
+		const double *ps, int nbs, double *pu, int nbu, const int *flags);
 
 /* ---- host setup (ssx_setup.c) ---- */
 void ssx_basis_init(ssx_basis *b, int n, int svvn, double svva);
@@ -187,22 +210,32 @@ int ssxk_ax(void *stream, int n, int v, int nel, int op, const double *u,
 /* ssxk_ax of the new CG direction: p = (z - scal[ic]) + scal[ib] p
  * (ic < 0: no null-space constant), masked where apply_mask, written back
  * to p, and ap = A p -- the k_cg_pupdate of the previous iteration fused
- * into the operator */
+ * into the operator; u (nullable): first u += scal[ia] p (old p), the
+ * previous iteration's solution step */
 int ssxk_ax_pupd(void *stream, int n, int v, int nel, int op, const double *z,
 		double *p, const double *scal, int ib, int ic, double *ap,
 		const double *gf, const double *mass, const double *dmat,
 		double lambda, const int *bc, int apply_mask, const int *done,
-		double *part);
+		double *part, double *u, int ia);
 int ssxk_ax_pupd_lp(void *stream, int prec, int n, int v, int nel, int op,
 		const void *z, void *p, const double *scal, int ib, int ic, void *ap,
 		const void *gf, const void *mass, const void *dmat, double lambda,
-		const int *bc, int apply_mask, const int *done, double *part);
+		const int *bc, int apply_mask, const int *done, double *part,
+		void *u, int ia);
+/* u += scal[ia] p whatever the done flags (the last solution step of a CG
+ * whose steps the operator launches take: ssxk_ax_pupd with u, updates
+ * with u = NULL) */
+int ssxk_cg_uflush(void *stream, size_t n, const double *scal, int ia,
+		const double *p, double *u);
+int ssxk_cg_uflush_lp(void *stream, size_t n, const double *scal, int ia,
+		const void *p, void *u);
 long long ssxk_dss_blocks(const long long ng[3]);
 /* DSS of all groups in one launch; average: scale by 1/M; p, part
  * (nullable): per block shared-node part of <p, DSS(u)>_w */
 int ssxk_dss_all(void *stream, int *const grp[3], const long long ng[3],
 		double *u, const double *p, double *part, int average,
 		const int *done);
+/* u += a p (skipped with u = NULL); r -= a ap; z = dinv r; partials */
 int ssxk_cg_update(void *stream, size_t n, const double *scal, int ia,
 		const double *p, const double *ap, double *u, double *r, double *z,
 		const double *dinv, const double *scale, int pressure, double *part,

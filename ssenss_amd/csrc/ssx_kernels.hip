@@ -84,9 +84,9 @@ extern "C" int ssxk_ax_pupd(void *stream, int n, int v, int nel, int op,
 		const double *z, double *p, const double *scal, int ib, int ic,
 		double *ap, const double *gf, const double *mass, const double *dmat,
 		double lambda, const int *bc, int apply_mask, const int *done,
-		double *part)
+		double *part, double *u, int ia)
 {
-	const PUpd pu = { z, p, scal, ib, ic };
+	const PUpd pu = { z, p, scal, ib, ic, u, ia };
 	return ssxk_ax_dispatch_f64(stream, n, v, nel, op, nullptr, ap, gf, mass,
 			dmat, lambda, bc, apply_mask, done, part, &pu);
 }
@@ -107,9 +107,14 @@ extern "C" int ssxk_ax_pupd_lp(void *stream, int prec, int n, int v, int nel,
 		int op, const void *z, void *p, const double *scal, int ib, int ic,
 		void *ap, const void *gf, const void *mass, const void *dmat,
 		double lambda, const int *bc, int apply_mask, const int *done,
-		double *part)
+		double *part, void *u, int ia)
 {
-	const PUpd pu = { z, p, scal, ib, ic };
+	if (u) {
+		ssx_set_error("ssxk_ax_pupd_lp: the deferred solution step is FP64 "
+				"only");
+		return SSX_EINVAL;
+	}
+	const PUpd pu = { z, p, scal, ib, ic, u, ia };
 	if (prec == SSX_PREC_BF16)
 		return ssxk_ax_dispatch_bf16(stream, n, v, nel, op, nullptr, ap, gf,
 				mass, dmat, lambda, bc, apply_mask, done, part, &pu);
@@ -332,8 +337,11 @@ __device__ __forceinline__ void block_sum4(double *v, double (*sh)[256])
 }
 
 // u += a p; r -= a ap; z = dinv r; partial <r,z>_w, <r,r>_w and, for the
-// pressure null space, <z,1>_w and <r,1>_w (FP64 accumulation)
-template <typename T, bool RED>
+// pressure null space, <z,1>_w and <r,1>_w (FP64 accumulation).  UPD
+// false (the launchers pick it for u null): the solution step is taken
+// elsewhere (by the next operator launch and a final k_cg_uflush) and
+// neither u nor p is touched.
+template <typename T, bool RED, bool UPD>
 __global__ void __launch_bounds__(256)
 k_cg_update(size_t n, const double *__restrict__ scal, int ia,
 		const T *__restrict__ p, const T *__restrict__ ap,
@@ -348,7 +356,8 @@ k_cg_update(size_t n, const double *__restrict__ scal, int ia,
 	const T a = (T) scal[ia];
 	double acc[4] = { 0.0, 0.0, 0.0, 0.0 };
 	GRID_STRIDE(q, n) {
-		u[q] = a * p[q] + u[q];
+		if (UPD)
+			u[q] = a * p[q] + u[q];
 		const T rv = -a * ap[q] + r[q];
 		r[q] = rv;
 		const T zv = rv * dinv[q];
@@ -389,7 +398,7 @@ k_cg_update(size_t n, const double *__restrict__ scal, int ia,
 // issues the table and partner loads of all U before using any: the
 // partner loads depend on the table loads, which depend on soff, and one
 // node at a time leaves those three round trips unoverlapped.
-template <typename T, int U>
+template <typename T, int U, bool UPD>
 __global__ void __launch_bounds__(256)
 k_cg_update_g(unsigned n, unsigned n3, int P, const int *__restrict__ soff,
 		const int *__restrict__ tab, const double *__restrict__ scal, int ia,
@@ -449,7 +458,8 @@ k_cg_update_g(unsigned n, unsigned n3, int P, const int *__restrict__ soff,
 					++m;
 				}
 			const T w = T(1) / T(m);
-			u[q] = a * p[q] + u[q];
+			if (UPD)
+				u[q] = a * p[q] + u[q];
 			const T rv = -a * s + r[q];
 			r[q] = rv;
 			const T zv = rv * dinv[q];
@@ -477,10 +487,18 @@ static int launch_cg_update_g(hipStream_t st, size_t n, int n3, int P,
 		ssx_set_error("k_cg_update_g: %zu nodes, %d per element", n, n3);
 		return SSX_EINVAL;
 	}
-	hipLaunchKernelGGL((k_cg_update_g<T, SSX_GATHER_U>), dim3(SSX_NPART),
-			dim3(256), 0, st, (unsigned) n, (unsigned) n3, P, soff, tab, scal, ia, (const T *) p,
-			(const T *) ap, (T *) u, (T *) r, (T *) z, (const T *) dinv,
-			pressure, part, done);
+	if (u)
+		hipLaunchKernelGGL((k_cg_update_g<T, SSX_GATHER_U, true>),
+				dim3(SSX_NPART), dim3(256), 0, st, (unsigned) n,
+				(unsigned) n3, P, soff, tab, scal, ia, (const T *) p,
+				(const T *) ap, (T *) u, (T *) r, (T *) z, (const T *) dinv,
+				pressure, part, done);
+	else
+		hipLaunchKernelGGL((k_cg_update_g<T, SSX_GATHER_U, false>),
+				dim3(SSX_NPART), dim3(256), 0, st, (unsigned) n,
+				(unsigned) n3, P, soff, tab, scal, ia, (const T *) p,
+				(const T *) ap, (T *) u, (T *) r, (T *) z, (const T *) dinv,
+				pressure, part, done);
 	LAUNCH_CHECK("k_cg_update_g");
 	return SSX_OK;
 }
@@ -519,6 +537,36 @@ k_cg_pupdate(size_t n, const T *__restrict__ z, T *__restrict__ p,
 		T v = b * p[q] + zv;
 		p[q] = mask ? v * mask[q] : v;
 	}
+}
+
+// u += scal[ia] p: the last CG iteration's solution step when the steps
+// are taken by the operator launches (runs whatever the done flag says)
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_cg_uflush(size_t n, const double *__restrict__ scal, int ia,
+		const T *__restrict__ p, T *__restrict__ u)
+{
+	const T a = (T) scal[ia];
+	GRID_STRIDE(q, n)
+		u[q] = a * p[q] + u[q];
+}
+
+extern "C" int ssxk_cg_uflush(void *stream, size_t n, const double *scal,
+		int ia, const double *p, double *u)
+{
+	hipLaunchKernelGGL(k_cg_uflush<double>, vgrid(n), dim3(256), 0,
+			(hipStream_t) stream, n, scal, ia, p, u);
+	LAUNCH_CHECK("k_cg_uflush");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_cg_uflush_lp(void *stream, size_t n, const double *scal,
+		int ia, const void *p, void *u)
+{
+	hipLaunchKernelGGL(k_cg_uflush<float>, vgrid(n), dim3(256), 0,
+			(hipStream_t) stream, n, scal, ia, (const float *) p, (float *) u);
+	LAUNCH_CHECK("k_cg_uflush");
+	return SSX_OK;
 }
 
 // single-block reductions + scalar logic of one CG iteration
@@ -586,13 +634,22 @@ static int launch_cg_update(hipStream_t st, size_t n, const double *scal,
 		const int *done, const ssx_lastred *lr = nullptr,
 		int nblocks = SSX_NPART)
 {
+	if (lr && !u) {
+		ssx_set_error("k_cg_update: the last-workgroup form takes u");
+		return SSX_EINVAL;
+	}
 	if (lr)
-		hipLaunchKernelGGL((k_cg_update<T, true>), dim3(SSX_NPART), dim3(256),
-				0, st, n, scal, ia, (const T *) p, (const T *) ap, (T *) u,
-				(T *) r, (T *) z, (const T *) dinv, (const T *) scale,
-				pressure, part, done, *lr);
+		hipLaunchKernelGGL((k_cg_update<T, true, true>), dim3(SSX_NPART),
+				dim3(256), 0, st, n, scal, ia, (const T *) p, (const T *) ap,
+				(T *) u, (T *) r, (T *) z, (const T *) dinv,
+				(const T *) scale, pressure, part, done, *lr);
+	else if (u)
+		hipLaunchKernelGGL((k_cg_update<T, false, true>), dim3(nblocks),
+				dim3(256), 0, st, n, scal, ia, (const T *) p, (const T *) ap,
+				(T *) u, (T *) r, (T *) z, (const T *) dinv,
+				(const T *) scale, pressure, part, done, ssx_lastred{});
 	else
-		hipLaunchKernelGGL((k_cg_update<T, false>), dim3(nblocks),
+		hipLaunchKernelGGL((k_cg_update<T, false, false>), dim3(nblocks),
 				dim3(256), 0, st, n, scal, ia, (const T *) p, (const T *) ap,
 				(T *) u, (T *) r, (T *) z, (const T *) dinv,
 				(const T *) scale, pressure, part, done, ssx_lastred{});
@@ -1157,6 +1214,215 @@ extern "C" int ssxk_csr_pupd(void *stream, int nu, const int *rowptr,
 			(hipStream_t) stream, nu, rowptr, col, val, pold, pnew, z, mask,
 			scal, ib, ic, first, ap, part, done);
 	LAUNCH_CHECK("k_csr_pupd");
+	return SSX_OK;
+}
+
+// ---------------------------------------------------------------------
+// Two-launch coarse CG: each kernel replays the single-workgroup
+// reduction it depends on (same partials, same order, same scalar
+// arithmetic as k_cg_reduce), so the iteration is bit-identical to the
+// four-launch one with half the launches.
+// ---------------------------------------------------------------------
+
+// the four mode-1 sums (rz, rr, zs, rs) of k_cg_reduce in a 256-lane
+// workgroup: lane t folds virtual lanes t, t+256, t+512, t+768 as the first
+// two tree levels do, then the same tree
+__device__ void sum4_as_1024(const double *__restrict__ part, long long m,
+		int nq, double out[4], double (*sh)[256])
+{
+#pragma unroll
+	for (int c = 0; c < 4; ++c) {
+		double a[4];
+#pragma unroll
+		for (int v = 0; v < 4; ++v) {
+			double acc = 0.0;
+			if (c < nq)
+				for (long long q = threadIdx.x + 256 * v; q < m; q += 1024)
+					acc += part[(size_t) c * SSX_NPART + q];
+			a[v] = acc;
+		}
+		a[0] += a[2];
+		a[1] += a[3];
+		a[0] += a[1];
+		sh[c][threadIdx.x] = a[0];
+	}
+	__syncthreads();
+	for (int st = 128; st > 0; st >>= 1) {
+		if ((int) threadIdx.x < st)
+#pragma unroll
+			for (int c = 0; c < 4; ++c)
+				sh[c][threadIdx.x] += sh[c][threadIdx.x + st];
+		__syncthreads();
+	}
+#pragma unroll
+	for (int c = 0; c < 4; ++c)
+		out[c] = sh[c][0];
+	__syncthreads();
+}
+
+// the mode-0 sum of k_cg_reduce (m partials) in a 256-lane workgroup
+__device__ double sum1_as_1024(const double *__restrict__ part, long long m,
+		double *sh)
+{
+	double a[4];
+#pragma unroll
+	for (int v = 0; v < 4; ++v) {
+		double acc = 0.0;
+		for (long long q = threadIdx.x + 256 * v; q < m; q += 1024)
+			acc += part[q];
+		a[v] = acc;
+	}
+	a[0] += a[2];
+	a[1] += a[3];
+	a[0] += a[1];
+	sh[threadIdx.x] = a[0];
+	__syncthreads();
+	for (int st = 128; st > 0; st >>= 1) {
+		if ((int) threadIdx.x < st)
+			sh[threadIdx.x] += sh[threadIdx.x + st];
+		__syncthreads();
+	}
+	const double t = sh[0];
+	__syncthreads();
+	return t;
+}
+
+__global__ void __launch_bounds__(256)
+k_csr_cg_spmv(int nu, const int *__restrict__ rowptr,
+		const int *__restrict__ col, const double *__restrict__ val,
+		const double *__restrict__ pold, double *__restrict__ pnew,
+		const double *__restrict__ z, const double *__restrict__ mask,
+		double *__restrict__ scal, int o, int j, double eps, int maxit,
+		double nunique, int pressure, double *__restrict__ ap,
+		double *__restrict__ ps, const double *__restrict__ pu, int nbu,
+		int *__restrict__ flags, int check)
+{
+	if (flags[0])
+		return;
+	__shared__ double s4[4][256];
+	double b = 0.0, cc = 0.0;
+	if (j > 0) {
+		// cg_scalars1 of iteration j-1, in every workgroup
+		double v[4];
+		sum4_as_1024(pu, nbu, pressure ? 4 : 2, v, s4);
+		double c = 0.0, rz = v[0];
+		if (pressure) {
+			c = v[2] / nunique;
+			rz = rz - c * v[3];
+		}
+		const double beta = rz / scal[o + ((j - 1) & 1 ? CG_RDZ1 : CG_RDZ)];
+		const double tol = eps >= 0.0 ? eps : scal[o + CG_TOL];
+		const bool stop = sqrt(v[1]) <= tol || j >= maxit || !(v[1] == v[1]);
+		if (blockIdx.x == 0 && threadIdx.x == 0) {
+			scal[o + CG_NULLC] = c;
+			scal[o + CG_BETA] = beta;
+			scal[o + ((j & 1) ? CG_RDZ1 : CG_RDZ)] = rz;
+			scal[o + CG_RR] = v[1];
+			flags[1] = j;
+			if (stop)
+				flags[0] = 1;
+		}
+		if (stop)
+			return;
+		b = beta;
+		cc = pressure ? -c : 0.0;
+	}
+	if (check)
+		return;
+	const int first = j == 0;
+	const int ic = pressure ? 0 : -1;      // csr_pn: null-space shift on
+	const int lane = threadIdx.x % CSR_LPR;
+	const int u = (blockIdx.x * 256 + threadIdx.x) / CSR_LPR;
+	double pd = 0.0;
+	double y = 0.0;
+	if (u < nu)
+		for (int k = rowptr[u] + lane; k < rowptr[u + 1]; k += CSR_LPR)
+			y += val[k] * csr_pn(pold, z, mask, col[k], first, b, cc, ic);
+	for (int w = 1; w < CSR_LPR; w <<= 1)
+		y += __shfl_xor(y, w, CSR_LPR);
+	if (u < nu && lane == 0) {
+		const double pv = csr_pn(pold, z, mask, u, first, b, cc, ic);
+		pnew[u] = pv;
+		if (mask && mask[u] == 0.0)
+			y = 0.0;
+		ap[u] = y;
+		pd = pv * y;
+	}
+	const double t = block_sum256(pd, s4[0]);
+	if (threadIdx.x == 0)
+		ps[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(256)
+k_csr_cg_update(size_t n, double *__restrict__ scal, int o, int j,
+		const double *__restrict__ p, const double *__restrict__ ap,
+		double *__restrict__ u, double *__restrict__ r, double *__restrict__ z,
+		const double *__restrict__ dinv, const double *__restrict__ scale,
+		int pressure, const double *__restrict__ ps, int nbs,
+		double *__restrict__ pu, const int *__restrict__ flags)
+{
+	if (flags[0])
+		return;
+	__shared__ double sh[4][256];
+	// cg_scalars0, in every workgroup
+	const double pap = sum1_as_1024(ps, nbs, sh[0]);
+	const double a = scal[o + ((j & 1) ? CG_RDZ1 : CG_RDZ)] / pap;
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		scal[o + CG_PAP] = pap;
+		scal[o + CG_ALPHA] = a;
+	}
+	double acc[4] = { 0.0, 0.0, 0.0, 0.0 };
+	GRID_STRIDE(q, n) {
+		u[q] = a * p[q] + u[q];
+		const double rv = -a * ap[q] + r[q];
+		r[q] = rv;
+		const double zv = rv * dinv[q];
+		z[q] = zv;
+		const double w = scale[q];
+		acc[0] += (double) (rv * w) * (double) zv;
+		acc[1] += (double) (rv * w) * (double) rv;
+		if (pressure) {
+			acc[2] += (double) (zv * w);
+			acc[3] += (double) (rv * w);
+		}
+	}
+	block_sum4(acc, sh);
+	if (threadIdx.x < 4)
+		pu[(size_t) threadIdx.x * SSX_NPART + blockIdx.x] = sh[threadIdx.x][0];
+}
+
+extern "C" int ssxk_csr_cg_spmv(void *stream, int nu, const int *rowptr,
+		const int *col, const double *val, const double *pold, double *pnew,
+		const double *z, const double *mask, double *scal, int o, int j,
+		double eps, int maxit, double nunique, int pressure, double *ap,
+		double *ps, const double *pu, int nbu, int *flags, int check)
+{
+	if (nbu < 1 || nbu > SSX_NPART) {
+		ssx_set_error("ssxk_csr_cg_spmv: %d update partials", nbu);
+		return SSX_EINVAL;
+	}
+	hipLaunchKernelGGL(k_csr_cg_spmv,
+			dim3(check ? 1 : ssxk_csr_blocks(nu)), dim3(256), 0,
+			(hipStream_t) stream, nu, rowptr, col, val, pold, pnew, z, mask,
+			scal, o, j, eps, maxit, nunique, pressure, ap, ps, pu, nbu, flags,
+			check);
+	LAUNCH_CHECK("k_csr_cg_spmv");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_csr_cg_update(void *stream, int nu, double *scal, int o,
+		int j, const double *p, const double *ap, double *u, double *r,
+		double *z, const double *dinv, const double *scale, int pressure,
+		const double *ps, int nbs, double *pu, int nbu, const int *flags)
+{
+	if (nbu < 1 || nbu > SSX_NPART) {
+		ssx_set_error("ssxk_csr_cg_update: %d workgroups", nbu);
+		return SSX_EINVAL;
+	}
+	hipLaunchKernelGGL(k_csr_cg_update, dim3(nbu), dim3(256), 0,
+			(hipStream_t) stream, (size_t) nu, scal, o, j, p, ap, u, r, z,
+			dinv, scale, pressure, ps, nbs, pu, flags);
+	LAUNCH_CHECK("k_csr_cg_update");
 	return SSX_OK;
 }
 

@@ -728,6 +728,47 @@ def test_gather_dss_update_bit_identical(name):
         assert np.array_equal(a[1], b[1])
 
 
+@pytest.mark.parametrize("name", ["cube2_n8", "periodic3_n6", "channel_n6"])
+def test_coarse_two_launch_bit_identical(name):
+    """the assembled coarse CG with two launches per iteration (each kernel
+    replays the reduction it needs) against four launches (separate
+    reductions): bit-identical coarse solves for both operators and
+    variable kinds, and bit-identical FCG + p-MG solves in FP64 and FP32"""
+    Z = golden(name)
+    mesh = golden_mesh(Z)
+    eps = Z["ref_scalars"][0]
+    with make_ctx(mesh) as ctx:
+        lc = ctx.nlevels - 1
+        rng = np.random.default_rng(11)
+        fc = rng.standard_normal(ctx.ntot(lc))
+        res = {}
+        for opt in (1, 2):
+            ctx.set_option(S.OPT_COARSE_ASSEMBLED, opt)
+            runs = []
+            for op, var in ((S.OP_POISSON, S.PRESSURE),
+                            (S.OP_HELMHOLTZ, S.X_VELOCITY)):
+                for mi in (500, 3):          # converged, stopped by max_iter
+                    f = dev(fc)
+                    ctx.dss_average(lc, f)
+                    u = torch.zeros_like(f)
+                    st = ctx.cg(lc, op, var, f, u, 1e-9 * float(f.norm()), mi,
+                                allow_notconv=True)
+                    runs.append((st.iterations, st.residual, host(u)))
+            for prec in (S.PREC_FP64, S.PREC_FP32):
+                ctx.set_option(S.OPT_SMOOTHER_PRECISION, prec)
+                f = dev(Z["in_f"])
+                v = torch.zeros_like(f)
+                st = ctx.fcg(S.OP_POISSON, S.PRESSURE, S.PC_MG, f, v, eps, 5000)
+                runs.append((st.iterations, st.coarse_iters, host(v)))
+            ctx.set_option(S.OPT_SMOOTHER_PRECISION, S.PREC_FP64)
+            res[opt] = runs
+        ctx.set_option(S.OPT_COARSE_ASSEMBLED, 1)
+    print(f"[gpu] coarse two-launch {name}: {[r[:2] for r in res[1]]}")
+    for a, b in zip(res[1], res[2]):
+        assert a[0] == b[0] and a[1] == b[1]
+        assert np.array_equal(a[2], b[2])
+
+
 @pytest.mark.parametrize("name", ["cube2_n8", "periodic3_n6", "channel_n6",
                                   "cube2_n12"])
 def test_assembled_coarse_level(name):
