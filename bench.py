@@ -110,7 +110,8 @@ def solve_case(S, torch, ctx, mesh, max_iter, precs=("fp64", "fp32", "bf16"),
 
 
 def particle_case(S, torch, stream, npart=1_000_000, seed=7):
-    """Inertial point particles (integrate_particle_equations) on the
+    """Inertial point particles (integrate_particle_equations), and RK2
+    tracers (ssx_tracers_step) from the same positions, on the
     channel of north-star configuration 5: 24x16x24 hexahedra at n = 10 on
     [0,4pi] x [-1,1] x [0,4pi/3], tanh wall stretching 2.0, periodic in x
     and z, walls in y.  Velocity: Poiseuille u = 1 - y^2 plus a seeded
@@ -160,12 +161,29 @@ def particle_case(S, torch, stream, npart=1_000_000, seed=7):
         times.append(time.perf_counter() - t0)
     moved = int((pr["el"].cpu().numpy() != base["el"]).sum())
     reinit = int(pr["reinit"].sum().item())
+    # RK2 tracers (configuration 5's tracer mode) on the same field and
+    # positions, one step of dt
+    ctx.tracers_step(fresh(), T, dt)                # warm-up
+    tt = []
+    for _ in range(3):
+        tr = fresh()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tpasses = ctx.tracers_step(tr, T, dt)      # synchronous
+        tt.append(time.perf_counter() - t0)
+    tmoved = int((tr["el"].cpu().numpy() != base["el"]).sum())
+    treinit = int(tr["reinit"].sum().item())
     ctx.close()
     t = min(times)
     return {"particles": npart, "local_dofs": mesh.ntot, "seconds": t,
             "particles_per_s": npart / t, "max_passes": passes,
             "changed_element": moved, "reseeded": reinit,
-            "mesh": "channel 24x16x24, n=10, tanh 2.0", "dt": dt}
+            "mesh": "channel 24x16x24, n=10, tanh 2.0", "dt": dt,
+            "tracers_rk2": {"tracers": npart, "seconds": min(tt),
+                            "tracers_per_s": npart / min(tt),
+                            "max_passes": tpasses,
+                            "changed_element": tmoved,
+                            "reseeded": treinit}}
 
 
 def batch_case(S, torch, nprob=16, seed=11):
@@ -359,7 +377,8 @@ def main():
     if particles:
         log(f"[bench] {particles['particles']} particles: "
             f"{particles['seconds']*1e3:.2f} ms per step, "
-            f"{particles['max_passes']} passes")
+            f"{particles['max_passes']} passes; RK2 tracers "
+            f"{particles['tracers_rk2']['seconds']*1e3:.2f} ms per step")
     batchres = None if args.no_batch else batch_case(S, torch)
     if batchres:
         log(f"[bench] batch of {batchres['problems']}: sequential "

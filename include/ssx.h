@@ -270,6 +270,24 @@ int ssx_particles_set_velocity_host(ssx_ctx *ctx, const double *u,
 		const double *v, const double *w);
 int ssx_particles_step_host(ssx_ctx *ctx, const ssx_particles *p,
 		double t_now, double dt, int *max_passes);
+/* RK2 tracers (dx/dt = u_f(x, t), no inertia), the tracer mode of the same
+ * walk (no counterpart in the reference; its particles are inertial, RK4):
+ * every tracer goes from its t to t_now by Heun's method in the reference
+ * coordinates of its element, with the velocity history of
+ * ssx_particles_set_velocity.  A stage that leaves [-1,1]^3 is replaced by
+ * a straight move to the boundary along the slope that left (k1 if the
+ * predictor left, else (k1 + k2)/2), a hop across the closest face with
+ * the crossed coordinate set to the opposite face, and a new step over the
+ * remaining time; at a wall the tracer stays on the wall for the rest of
+ * the step; after more than 10 passes it restarts at the element centre
+ * with reinit set.  u, v, w receive the fluid velocity (reference space) at
+ * the end point; tau is not read (may be NULL).  Synchronous; a tracer
+ * whose el is not an element index is left untouched and the call returns
+ * SSX_EINVAL. */
+int ssx_tracers_step(ssx_ctx *ctx, const ssx_particles *p, double t_now,
+		double dt, int *max_passes);
+int ssx_tracers_step_host(ssx_ctx *ctx, const ssx_particles *p,
+		double t_now, double dt, int *max_passes);
 
 /* ---- batches of independent problems ----
  * Jacobi-PCG (conjugate_gradient, solver.c:33-81) on level 0 of `count`

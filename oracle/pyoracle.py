@@ -272,6 +272,9 @@ def _particle_sigs(L):
                                      C.c_double, C.c_double]
     L.orc_particle_position.restype = None
     L.orc_particle_position.argtypes = [vp, C.POINTER(Particle), dp]
+    L.orc_tracers_step.restype = C.c_int
+    L.orc_tracers_step.argtypes = [vp, C.POINTER(Particle), C.c_long,
+                                   C.c_double, C.c_double]
 
 
 def interp_point(n, xi, f, x, y, z):
@@ -316,3 +319,49 @@ def _oracle_particles(self, vel, scales, rec, t_now, dt):
 
 
 Oracle.particles = _oracle_particles
+
+
+def _oracle_tracers(self, vel, scales, rec, t_now, dt, steps=1):
+    """Three set_velocity calls (oldest first), then `steps` RK2 tracer
+    steps of dt ending at t_now (orc_tracers_step).  rec as particles();
+    returns (n, 8) records, reinit flags, the largest pass count and the
+    physical positions (n, 3)."""
+    L = self.L
+    _particle_sigs(L)
+    nt = self.ntot()
+    for s in scales:
+        u = np.ascontiguousarray(s * vel[:nt])
+        v = np.ascontiguousarray(s * vel[nt:2 * nt])
+        w = np.ascontiguousarray(s * vel[2 * nt:])
+        L.orc_particles_set_velocity(self.h, _dp(u), _dp(v), _dp(w))
+    arr = particles_from_records(rec)
+    passes = 0
+    for k in range(steps):
+        tk = t_now - (steps - 1 - k) * dt
+        passes = max(passes, L.orc_tracers_step(self.h, arr, len(rec), tk,
+                                                dt))
+    out, re = particles_to_records(arr)
+    xyz = np.zeros((len(rec), 3))
+    for i in range(len(rec)):
+        L.orc_particle_position(self.h, C.byref(arr[i]),
+                                xyz[i].ctypes.data_as(C.POINTER(C.c_double)))
+    return out, re, passes, xyz
+
+
+Oracle.tracers = _oracle_tracers
+
+
+def _oracle_positions(self, out):
+    """physical coordinates (n, 3) of records (n, >=4): el, x, y, z"""
+    L = self.L
+    _particle_sigs(L)
+    xyz = np.zeros((len(out), 3))
+    p = Particle()
+    for i, r in enumerate(out):
+        p.el, p.x, p.y, p.z = int(r[0]), float(r[1]), float(r[2]), float(r[3])
+        L.orc_particle_position(self.h, C.byref(p),
+                                xyz[i].ctypes.data_as(C.POINTER(C.c_double)))
+    return xyz
+
+
+Oracle.positions = _oracle_positions

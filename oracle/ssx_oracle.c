@@ -1583,6 +1583,96 @@ int orc_particles_step(orc_ctx *c, orc_particle *ps, long n, double t_now,
 	return maxpass;
 }
 
+/* RK2 tracers (header).  Every product and sum is written out in the
+ * order the device kernel evaluates it. */
+static double tstep_to_boundary(const double *y, const double *f)
+{
+	double d[3];                    /* an axis without motion: never */
+	for (int i = 0; i < 3; ++i)
+		d[i] = f[i] != 0.0 ? (copysign(1.0, f[i]) - y[i]) / f[i] : INFINITY;
+	return dmin2(dmin2(d[0], d[1]), d[2]);
+}
+
+static void tracer_velocity(orc_ctx *c, int el, const double *y, double t,
+		double *f)
+{
+	for (int i = 0; i < 3; ++i)
+		f[i] = fluid_velocity(c, el, y, t, i);
+}
+
+int orc_tracers_step(orc_ctx *c, orc_particle *ps, long n, double t_now,
+		double dt)
+{
+	c->pt = t_now;
+	c->pdt = dt;
+	int maxpass = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(max:maxpass)
+	for (long q = 0; q < n; ++q) {
+		orc_particle *p = ps + q;
+		double x[3] = { p->x, p->y, p->z };
+		int j = 0;
+		for (;;) {
+			if (j > 10) {                   /* as particles.c:195-198 */
+				x[0] = x[1] = x[2] = 0.0;
+				p->reinit = 1;
+				p->t = t_now;
+				break;
+			}
+			++j;
+			const double h = t_now - p->t;
+			if (!(h > 0.0))
+				break;
+			double k1[3], k2[3], xp[3], xn[3], s[3];
+			tracer_velocity(c, p->el, x, p->t, k1);
+			for (int i = 0; i < 3; ++i)
+				xp[i] = x[i] + h * k1[i];
+			int out = crossed(xp);
+			if (!out) {
+				tracer_velocity(c, p->el, xp, p->t + h, k2);
+				for (int i = 0; i < 3; ++i) {
+					s[i] = 0.5 * (k1[i] + k2[i]);
+					xn[i] = x[i] + h * s[i];
+				}
+				if (!crossed(xn)) {
+					x[0] = xn[0]; x[1] = xn[1]; x[2] = xn[2];
+					p->t = t_now;
+					break;
+				}
+			} else {
+				s[0] = k1[0]; s[1] = k1[1]; s[2] = k1[2];
+			}
+			double dtau = tstep_to_boundary(x, s);
+			if (!(dtau < h))
+				dtau = h;
+			if (dtau < 0.0)
+				dtau = 0.0;
+			for (int i = 0; i < 3; ++i)
+				x[i] = x[i] + dtau * s[i];
+			p->t = p->t + dtau;
+			orc_particle tmp = *p;
+			tmp.x = x[0]; tmp.y = x[1]; tmp.z = x[2];
+			const int k = closest(&tmp);
+			const int nb = c->neigh_copy[p->el * 6 + k];
+			x[k / 2] = (k % 2) ? 1.0 : -1.0;
+			if (nb >= 0) {
+				p->old_el = p->el;
+				p->el = nb;
+				x[k / 2] = (k % 2) ? -1.0 : 1.0;
+			} else {                        /* wall: stay on it */
+				p->t = t_now;
+				break;
+			}
+		}
+		p->x = x[0]; p->y = x[1]; p->z = x[2];
+		double uf[3];
+		tracer_velocity(c, p->el, x, t_now, uf);
+		p->u = uf[0]; p->v = uf[1]; p->w = uf[2];
+		if (j > maxpass)
+			maxpass = j;
+	}
+	return maxpass;
+}
+
 void orc_particle_position(orc_ctx *c, const orc_particle *p, double *xyz)
 {
 	const level_t *L = &c->lev[0];

@@ -145,6 +145,20 @@ void orc_particles_set_velocity(orc_ctx *c, const double *u, const double *v,
  * time step dt (params->t, params->dt); returns the number of passes. */
 int orc_particles_step(orc_ctx *c, orc_particle *p, long n, double t_now,
 		double dt);
+/* RK2 tracers (dx/dt = u_f(x, t), no inertia): the tracer mode of the
+ * particle walk, an extension with no reference counterpart (pinned by
+ * closed forms: uniform flow, linear fields).  Each tracer goes from its t
+ * to t_now by Heun's method in the reference coordinates of its element.
+ * A stage that leaves [-1,1]^3 is replaced by a straight move to the
+ * boundary along the slope that left (k1 if the predictor left, else
+ * (k1 + k2)/2), a hop to the neighbour across the closest face with the
+ * crossed coordinate set to the opposite face, and a new step over the
+ * remaining time.  At a wall the tracer stays on the wall for the rest of
+ * the step.  More than 10 passes: restart at the element centre, reinit
+ * flag set.  u, v, w receive the fluid velocity (reference space) at the
+ * end point; tau is not used.  Returns the largest pass count. */
+int orc_tracers_step(orc_ctx *c, orc_particle *p, long n, double t_now,
+		double dt);
 /* physical coordinates of a particle (particles.c:539-544) */
 void orc_particle_position(orc_ctx *c, const orc_particle *p, double *xyz);
 

@@ -1733,13 +1733,15 @@ int ssx_particles_set_velocity(ssx_ctx *c, const double *u, const double *v,
 			c->p_ur[0][1], c->p_ur[0][2]);
 }
 
-int ssx_particles_step(ssx_ctx *c, const ssx_particles *p, double t_now,
-		double dt, int *max_passes)
+/* one step of the particle walk on device arrays: inertial particles
+ * (particles.c) or, with tracers, RK2 tracers (tau not used) */
+static int walk_step(ssx_ctx *c, const ssx_particles *p, double t_now,
+		double dt, int *max_passes, int tracers)
 {
 	TRY(chk(c, 0));
 	if (!p || p->n < 0 || (p->n > 0 && (!p->el || !p->old_el || !p->x
 			|| !p->y || !p->z || !p->u || !p->v || !p->w || !p->t
-			|| !p->tau))) {
+			|| (!tracers && !p->tau)))) {
 		ssx_set_error("bad particle list");
 		return SSX_EINVAL;
 	}
@@ -1752,11 +1754,16 @@ int ssx_particles_step(ssx_ctx *c, const ssx_particles *p, double t_now,
 	for (int a = 0; a < 9; ++a)
 		ur[a] = c->p_ur[a / 3][a % 3];
 	TRY(ssxk_set_ints(c->stream, c->p_maxpass, 0, 2));
-	TRY(ssxk_particles_step(c->stream, c->lev[0].n, c->nel, c->p_xi, ur,
-			(const double *const *) c->p_jac,
-			(const double *const *) c->p_jinv, c->d_neigh, t_now, dt, p->n,
-			p->el, p->old_el, p->x, p->y, p->z, p->u, p->v, p->w, p->t,
-			p->tau, p->reinit, c->p_maxpass));
+	if (tracers)
+		TRY(ssxk_tracers_step(c->stream, c->lev[0].n, c->nel, c->p_xi, ur,
+				c->d_neigh, t_now, dt, p->n, p->el, p->old_el, p->x, p->y,
+				p->z, p->u, p->v, p->w, p->t, p->reinit, c->p_maxpass));
+	else
+		TRY(ssxk_particles_step(c->stream, c->lev[0].n, c->nel, c->p_xi, ur,
+				(const double *const *) c->p_jac,
+				(const double *const *) c->p_jinv, c->d_neigh, t_now, dt,
+				p->n, p->el, p->old_el, p->x, p->y, p->z, p->u, p->v, p->w,
+				p->t, p->tau, p->reinit, c->p_maxpass));
 	int fl[2] = { 0, 0 };
 	HIPC(hipMemcpyAsync(fl, c->p_maxpass, sizeof(fl), hipMemcpyDeviceToHost,
 			(hipStream_t) c->stream));
@@ -1769,6 +1776,18 @@ int ssx_particles_step(ssx_ctx *c, const ssx_particles *p, double t_now,
 		return SSX_EINVAL;
 	}
 	return SSX_OK;
+}
+
+int ssx_particles_step(ssx_ctx *c, const ssx_particles *p, double t_now,
+		double dt, int *max_passes)
+{
+	return walk_step(c, p, t_now, dt, max_passes, 0);
+}
+
+int ssx_tracers_step(ssx_ctx *c, const ssx_particles *p, double t_now,
+		double dt, int *max_passes)
+{
+	return walk_step(c, p, t_now, dt, max_passes, 1);
 }
 
 /* host-array variants: stage through device memory held by the context */
@@ -1811,13 +1830,13 @@ int ssx_particles_set_velocity_host(ssx_ctx *c, const double *u,
 	return SSX_OK;
 }
 
-int ssx_particles_step_host(ssx_ctx *c, const ssx_particles *h, double t_now,
-		double dt, int *max_passes)
+static int walk_step_host(ssx_ctx *c, const ssx_particles *h, double t_now,
+		double dt, int *max_passes, int tracers)
 {
 	TRY(chk(c, 0));
 	if (!h || h->n < 0 || (h->n > 0 && (!h->el || !h->old_el || !h->x
 			|| !h->y || !h->z || !h->u || !h->v || !h->w || !h->t
-			|| !h->tau))) {
+			|| (!tracers && !h->tau)))) {
 		ssx_set_error("bad particle list");
 		return SSX_EINVAL;
 	}
@@ -1834,15 +1853,16 @@ int ssx_particles_step_host(ssx_ctx *c, const ssx_particles *h, double t_now,
 	double *hcol[8] = { h->x, h->y, h->z, h->u, h->v, h->w, h->t, h->tau };
 	hipStream_t st = (hipStream_t) c->stream;
 	for (int k = 0; k < 8; ++k)
-		HIPC(hipMemcpyAsync(dd + k * n, hcol[k], n * sizeof(double),
-				hipMemcpyHostToDevice, st));
+		if (hcol[k])
+			HIPC(hipMemcpyAsync(dd + k * n, hcol[k], n * sizeof(double),
+					hipMemcpyHostToDevice, st));
 	HIPC(hipMemcpyAsync(di, h->el, n * sizeof(int), hipMemcpyHostToDevice,
 			st));
 	HIPC(hipMemcpyAsync(di + n, h->old_el, n * sizeof(int),
 			hipMemcpyHostToDevice, st));
 	ssx_particles d = { h->n, di, di + n, dd, dd + n, dd + 2 * n, dd + 3 * n,
 		dd + 4 * n, dd + 5 * n, dd + 6 * n, dd + 7 * n, di + 2 * n };
-	const int rc = ssx_particles_step(c, &d, t_now, dt, max_passes);
+	const int rc = walk_step(c, &d, t_now, dt, max_passes, tracers);
 	if (rc != SSX_OK && rc != SSX_EINVAL)
 		return rc;
 	/* results come back also when some particles named no element */
@@ -1858,6 +1878,18 @@ int ssx_particles_step_host(ssx_ctx *c, const ssx_particles *h, double t_now,
 				hipMemcpyDeviceToHost, st));
 	HIPC(hipStreamSynchronize(st));
 	return rc;
+}
+
+int ssx_particles_step_host(ssx_ctx *c, const ssx_particles *h, double t_now,
+		double dt, int *max_passes)
+{
+	return walk_step_host(c, h, t_now, dt, max_passes, 0);
+}
+
+int ssx_tracers_step_host(ssx_ctx *c, const ssx_particles *h, double t_now,
+		double dt, int *max_passes)
+{
+	return walk_step_host(c, h, t_now, dt, max_passes, 1);
 }
 
 /* ------------------------------------------------------------------ */

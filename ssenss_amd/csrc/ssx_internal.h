@@ -344,6 +344,10 @@ int ssxk_copy_lp(void *stream, void *dst, const void *src, size_t n);
 int ssxk_particles_ur(void *stream, size_t nt, const double *const jinv[9],
 		const double *u, const double *v, const double *w, double *u0,
 		double *u1, double *u2);
+int ssxk_tracers_step(void *stream, int n_pts, int nel, const double *xi,
+		const double *const ur[9], const int *neigh, double T, double dt,
+		long np, int *el, int *old_el, double *x, double *y, double *z,
+		double *u, double *v, double *w, double *t, int *reinit, int *flags);
 int ssxk_particles_step(void *stream, int n_pts, int nel, const double *xi,
 		const double *const ur[9], const double *const jac[9],
 		const double *const jinv[9], const int *neigh, double T, double dt,

@@ -254,18 +254,23 @@ __device__ void to_boundary(const PField &F, PState &p, double dt)
 }
 
 // particles.c:380-393
-__device__ int closest(const PState &p)
+__device__ int closest3(const double *s)
 {
 	double hmin = 2.0;
 	int k = 0;
 	for (int i = 0; i < 6; ++i) {
-		const double h = fabs(p.s[i / 2] - ((i % 2) ? 1.0 : -1.0));
+		const double h = fabs(s[i / 2] - ((i % 2) ? 1.0 : -1.0));
 		if (h < hmin) {
 			hmin = h;
 			k = i;
 		}
 	}
 	return k;
+}
+
+__device__ int closest(const PState &p)
+{
+	return closest3(p.s);
 }
 
 // physical / parametric particle velocity, particles.c:517-539; the
@@ -380,6 +385,112 @@ k_particles_step(PField F, long n, int *__restrict__ el,
 	atomicMax(flags, j);                    // a maximum: order-independent
 }
 
+// ---------------------------------------------------------------------
+// RK2 tracers (dx/dt = u_f(x, t), no inertia): the tracer mode of the
+// walk, an extension with no reference counterpart.  Heun's method in the
+// reference coordinates of the owning element; a stage that leaves
+// [-1,1]^3 is replaced by a straight move to the boundary along the slope
+// that left (k1 if the predictor left, else (k1 + k2)/2), a hop across the
+// closest face with the crossed coordinate set to the opposite face, and
+// a new step over the remaining time.  At a wall the tracer stays on the
+// wall for the rest of the step; after more than 10 passes it restarts at
+// the element centre (flagged), as the inertial walk does.
+// ---------------------------------------------------------------------
+
+__device__ double tstep_to_boundary(const double *y, const double *f)
+{
+	double d[3];                    // an axis without motion: never
+	for (int i = 0; i < 3; ++i)
+		d[i] = f[i] != 0.0 ? (copysign(1.0, f[i]) - y[i]) / f[i] : INFINITY;
+	return dmin2(dmin2(d[0], d[1]), d[2]);
+}
+
+__global__ void __launch_bounds__(64)
+k_tracers_step(PField F, long n, int *__restrict__ el,
+		int *__restrict__ old_el, double *__restrict__ x,
+		double *__restrict__ y, double *__restrict__ z,
+		double *__restrict__ u, double *__restrict__ v,
+		double *__restrict__ w, double *__restrict__ t,
+		int *__restrict__ reinit_out, int *__restrict__ flags)
+{
+	__shared__ double sxi[PMAXN];
+	if (threadIdx.x < F.n)
+		sxi[threadIdx.x] = F.xi[threadIdx.x];
+	__syncthreads();
+	F.xi = sxi;
+	const long q = (long) blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= n)
+		return;
+	int e = el[q];
+	if (e < 0 || e >= F.nel) {              // not an element: report, skip
+		atomicAdd(flags + 1, 1);
+		return;
+	}
+	int oe = old_el[q], re = 0;
+	double xs[3] = { x[q], y[q], z[q] };
+	double tt = t[q];
+	int j = 0;
+	for (;;) {
+		if (j > 10) {
+			xs[0] = xs[1] = xs[2] = 0.0;
+			re = 1;
+			tt = F.T;
+			break;
+		}
+		++j;
+		const double h = F.T - tt;
+		if (!(h > 0.0))
+			break;
+		double k1[3], k2[3], xp[3], xn[3], sl[3];
+		fluid(F, e, xs, tt, k1);
+		for (int i = 0; i < 3; ++i)
+			xp[i] = xs[i] + h * k1[i];
+		if (!crossed(xp)) {
+			fluid(F, e, xp, tt + h, k2);
+			for (int i = 0; i < 3; ++i) {
+				sl[i] = 0.5 * (k1[i] + k2[i]);
+				xn[i] = xs[i] + h * sl[i];
+			}
+			if (!crossed(xn)) {
+				xs[0] = xn[0]; xs[1] = xn[1]; xs[2] = xn[2];
+				tt = F.T;
+				break;
+			}
+		} else {
+			sl[0] = k1[0]; sl[1] = k1[1]; sl[2] = k1[2];
+		}
+		double dtau = tstep_to_boundary(xs, sl);
+		if (!(dtau < h))
+			dtau = h;
+		if (dtau < 0.0)
+			dtau = 0.0;
+		for (int i = 0; i < 3; ++i)
+			xs[i] = xs[i] + dtau * sl[i];
+		tt = tt + dtau;
+		const int k = closest3(xs);
+		const int nb = F.neigh[e * 6 + k];
+		xs[k / 2] = (k % 2) ? 1.0 : -1.0;
+		if (nb >= 0) {
+			oe = e;
+			e = nb;
+			xs[k / 2] = (k % 2) ? -1.0 : 1.0;
+		} else {                        // wall: stay on it
+			tt = F.T;
+			break;
+		}
+	}
+	double uf[3];
+	fluid(F, e, xs, F.T, uf);
+	el[q] = e;
+	old_el[q] = oe;
+	x[q] = xs[0]; y[q] = xs[1]; z[q] = xs[2];
+	u[q] = uf[0]; v[q] = uf[1]; w[q] = uf[2];
+	t[q] = tt;
+	if (reinit_out)
+		reinit_out[q] = re;
+	atomicMax(flags, j);
+}
+
 struct P9 {
 	const double *p[9];
 };
@@ -419,6 +530,39 @@ extern "C" int ssxk_particles_ur(void *stream, size_t nt,
 	hipError_t e = hipGetLastError();
 	if (e != hipSuccess) {
 		ssx_set_error("k_particles_ur: %s", hipGetErrorString(e));
+		return SSX_EHIP;
+	}
+	return SSX_OK;
+}
+
+extern "C" int ssxk_tracers_step(void *stream, int n_pts, int nel,
+		const double *xi, const double *const ur[9], const int *neigh,
+		double T, double dt, long np, int *el, int *old_el, double *x,
+		double *y, double *z, double *u, double *v, double *w, double *t,
+		int *reinit, int *flags)
+{
+	PField F;
+	F.n = n_pts;
+	F.nel = nel;
+	F.xi = xi;
+	for (int l = 0; l < 3; ++l)
+		for (int c = 0; c < 3; ++c)
+			F.ur[l][c] = ur[l * 3 + c];
+	for (int a = 0; a < 9; ++a) {
+		F.jac[a] = nullptr;
+		F.jinv[a] = nullptr;
+	}
+	F.neigh = neigh;
+	F.T = T;
+	F.dt = dt;
+	if (np <= 0)
+		return SSX_OK;
+	hipLaunchKernelGGL(k_tracers_step, dim3((unsigned) ((np + 63) / 64)),
+			dim3(64), 0, (hipStream_t) stream, F, np, el, old_el, x, y, z, u,
+			v, w, t, reinit, flags);
+	hipError_t e = hipGetLastError();
+	if (e != hipSuccess) {
+		ssx_set_error("k_tracers_step: %s", hipGetErrorString(e));
 		return SSX_EHIP;
 	}
 	return SSX_OK;

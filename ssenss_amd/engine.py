@@ -197,6 +197,19 @@ class Context:
                                            C.byref(mp)))
         return mp.value
 
+    def tracers_step(self, p, t_now, dt):
+        """RK2 tracers (ssx_tracers_step): p as particles_step, tau
+        optional; u, v, w receive the fluid velocity at the end point"""
+        ps = ParticlesStruct(int(p["x"].numel()),
+                             *(p[k].data_ptr() if p.get(k) is not None
+                               else None for k in
+                               ("el", "old_el", "x", "y", "z", "u", "v", "w",
+                                "t", "tau", "reinit")))
+        mp = C.c_int()
+        check(self._lib.ssx_tracers_step(self.h, C.byref(ps), t_now, dt,
+                                         C.byref(mp)))
+        return mp.value
+
     # --- particles on host arrays (numpy) ---
     def particles_set_velocity_host(self, u, v, w):
         arrs = [np.ascontiguousarray(a, np.float64) for a in (u, v, w)]
@@ -216,6 +229,22 @@ class Context:
         mp = C.c_int()
         check(self._lib.ssx_particles_step_host(self.h, C.byref(ps), t_now,
                                                 dt, C.byref(mp)))
+        return mp.value
+
+    def tracers_step_host(self, p, t_now, dt):
+        """RK2 tracers on host arrays (ssx_tracers_step_host); p as
+        particles_step_host, tau optional"""
+        for k in ("el", "old_el", "reinit"):
+            assert p[k].dtype == np.int32 and p[k].flags.c_contiguous
+        for k in ("x", "y", "z", "u", "v", "w", "t"):
+            assert p[k].dtype == np.float64 and p[k].flags.c_contiguous
+        ps = ParticlesStruct(int(p["x"].size), *(
+            p[k].ctypes.data if p.get(k) is not None else None for k in
+            ("el", "old_el", "x", "y", "z", "u", "v", "w", "t", "tau",
+             "reinit")))
+        mp = C.c_int()
+        check(self._lib.ssx_tracers_step_host(self.h, C.byref(ps), t_now,
+                                              dt, C.byref(mp)))
         return mp.value
 
 

@@ -36,7 +36,8 @@ EXPORTED_SYMBOLS = [
     "ssx_fcg", "ssx_mg_precon", "ssx_solve_poisson_cg",
     "ssx_solve_helmholtz_cg", "ssx_particles_set_velocity",
     "ssx_particles_step", "ssx_particles_set_velocity_host",
-    "ssx_particles_step_host", "ssx_autotune", "ssx_ax_variant",
+    "ssx_particles_step_host", "ssx_tracers_step", "ssx_tracers_step_host",
+    "ssx_autotune", "ssx_ax_variant",
     "ssx_set_ax_variant", "ssx_batch_create", "ssx_batch_destroy",
     "ssx_batch_set_stream", "ssx_batch_cg",
 ]
@@ -95,6 +96,14 @@ def load():
         raise RuntimeError(
             f"{path} is missing: build it with `make -C ssenss_amd/csrc` or "
             "__graft_entry__.build(); there is no fallback path")
+    # PyTorch-ROCm ships its own HIP runtime libraries.  Loaded after
+    # libssx.so pulled in the system ROCm ones, torch then finds no GPU
+    # ("No HIP GPUs are available"), so torch, when installed, is imported
+    # first; the device tensors it allocates are what the caller passes.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(path)
     vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
     i, d, sz = C.c_int, C.c_double, C.c_size_t
@@ -150,6 +159,10 @@ def load():
                                         C.POINTER(C.c_int)]),
         "ssx_particles_step": (i, [vp, C.POINTER(ParticlesStruct), d, d,
                                    C.POINTER(C.c_int)]),
+        "ssx_tracers_step": (i, [vp, C.POINTER(ParticlesStruct), d, d,
+                                 C.POINTER(C.c_int)]),
+        "ssx_tracers_step_host": (i, [vp, C.POINTER(ParticlesStruct), d, d,
+                                      C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
