@@ -152,6 +152,9 @@ static int build(ssx_batch *b)
 		d->mask = d->pressure ? NULL : L->mask;
 		d->bc = L->bc;
 		d->g2 = L->grp[0]; d->g4 = L->grp[1]; d->g8 = L->grp[2];
+		d->gP = c->gather_upd ? L->gP : 0;
+		d->gsoff = L->gsoff;
+		d->gtab = L->gtab;
 		d->G2 = L->ngrp[0]; d->G4 = L->ngrp[1]; d->G8 = L->ngrp[2];
 		const int v = L->axv[SSX_PREC_FP64];
 		const int teams = ssxk_bax_teams(L->n, v);
@@ -269,7 +272,9 @@ static int fill_tables(ssx_batch *b, const unsigned char *active)
 		for (int i = 0; i < b->np; ++i) {
 			if (!active[i])
 				continue;
-			const long long nb = pass == 0 ? b->h_prob[i].nbd
+			/* problems with the gather form have no DSS launch */
+			const long long nb = pass == 0
+				? (b->h_prob[i].gP > 0 ? 0 : b->h_prob[i].nbd)
 				: b->h_prob[i].nbu;
 			for (long long q = 0; q < nb; ++q) {
 				t[off + m].prob = i;
@@ -363,7 +368,8 @@ int ssx_batch_set_stream(ssx_batch *b, void *s)
 	return SSX_OK;
 }
 
-/* one iteration: operator per order, DSS, alpha, update, beta + stop */
+/* one iteration: operator per order, DSS (problems without the gather
+ * form only), alpha, update, beta + stop */
 static int iteration(ssx_batch *b, int pupd)
 {
 	void *st = b->stream;
@@ -477,6 +483,10 @@ int ssx_batch_cg(ssx_batch *b, double *const *f, double *const *u,
 	if (rc0 == SSX_OK)
 		rc0 = iteration(b, 0);
 	int launched = 1;
+	/* replays between host checks: doubled (up to 16) while no problem
+	 * finishes, back to 1 when one does; a finished problem's kernels are
+	 * no-ops until the next check drops it from the tables */
+	int reps = 1;
 	for (; rc0 == SSX_OK;) {
 		int all = 1, changed = 0;
 		if ((rc0 = hipMemcpyAsync(b->h_flags, b->flags, sizeof(int) * 2 * np,
@@ -497,18 +507,21 @@ int ssx_batch_cg(ssx_batch *b, double *const *f, double *const *u,
 		/* drop converged problems from the launches */
 		if (changed && (rc0 = fill_tables(b, act)) != SSX_OK)
 			break;
+		reps = changed ? 1 : (reps < 16 ? 2 * reps : 16);
+		if ((max_iter - launched + GRAPH_ITERS - 1) / GRAPH_ITERS < reps)
+			reps = (max_iter - launched + GRAPH_ITERS - 1) / GRAPH_ITERS;
 		if (use_graph && !b->have_graph && (rc0 = capture(b)) != SSX_OK)
 			break;
-		if (use_graph) {
-			if (hipGraphLaunch(b->gexec, st) != hipSuccess) {
-				ssx_set_error("hipGraphLaunch failed (batch)");
-				rc0 = SSX_EHIP;
-				break;
+		for (int r = 0; r < reps && rc0 == SSX_OK; ++r) {
+			if (use_graph) {
+				if (hipGraphLaunch(b->gexec, st) != hipSuccess) {
+					ssx_set_error("hipGraphLaunch failed (batch)");
+					rc0 = SSX_EHIP;
+				}
+			} else {
+				for (int k = 0; k < GRAPH_ITERS && rc0 == SSX_OK; ++k)
+					rc0 = iteration(b, 1);
 			}
-			launched += GRAPH_ITERS;
-		} else {
-			for (int k = 0; k < GRAPH_ITERS && rc0 == SSX_OK; ++k)
-				rc0 = iteration(b, 1);
 			launched += GRAPH_ITERS;
 		}
 	}

@@ -15,6 +15,11 @@ typedef struct ssx_bprob {
 	const int *bc;
 	const int *g2, *g4, *g8;
 	long long G2, G4, G8;
+	/* gather form of the DSS (gP > 0): per-copy partner lists in add
+	 * order, [nel][gP], and per-node offset << 3 | capacity [n^3]; the
+	 * problem then has no DSS launch and its update assembles Ap */
+	int gP;
+	const int *gsoff, *gtab;
 	long long nba, nbd, nbu;   /* operator, DSS, update workgroups */
 	double *u, *r, *z, *p, *ap;
 	double *part;              /* [nel][nbd][4][nbu] partial sums */

@@ -73,7 +73,12 @@ k_bdss(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 }
 
 // u += alpha p; r -= alpha Ap; z = D^-1 r; partials of <r,z>_w, <r,r>_w,
-// <z,1>_w, <r,1>_w over this workgroup's chunk
+// <z,1>_w, <r,1>_w over this workgroup's chunk.  A problem with the gather
+// form (gP > 0) had no DSS launch: each copy adds its partners' values to
+// its own in its own add order (the group kernel's value bit for bit) and
+// its weight is 1/M from the partner count; the lane takes two of its
+// nodes at a time and issues their list and partner loads before using
+// any.  Otherwise Ap is assembled and the weight is scale.
 __global__ void __launch_bounds__(256)
 k_bupdate(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 {
@@ -83,23 +88,85 @@ k_bupdate(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 		return;
 	__shared__ double sh[256];
 	const double a = d->scal[CG_ALPHA];
+	const int gP = d->gP;
+	const long long n3 = (long long) d->n * d->n * d->n;
+	// the problem's fields in registers: the loop's stores cannot then be
+	// taken to alias them
+	const long long ntot = d->ntot;
+	const int pressure = d->pressure;
+	const int *__restrict__ gsoff = d->gsoff;
+	const int *__restrict__ gtab = d->gtab;
+	const double *__restrict__ ap = d->ap;
+	const double *__restrict__ pp = d->p;
+	const double *__restrict__ dinv = d->dinv;
+	const double *__restrict__ scale = d->scale;
+	double *__restrict__ u = d->u;
+	double *__restrict__ r = d->r;
+	double *__restrict__ z = d->z;
 	double acc[4] = { 0.0, 0.0, 0.0, 0.0 };
 	const long long q0 = (long long) w.blk * SSX_BATCH_CHUNK;
-	for (int t = 0; t < SSX_BATCH_CHUNK / 256; ++t) {
-		const long long q = q0 + t * 256 + threadIdx.x;
-		if (q >= d->ntot)
-			break;
-		d->u[q] = a * d->p[q] + d->u[q];
-		const double rv = -a * d->ap[q] + d->r[q];
-		d->r[q] = rv;
-		const double zv = rv * d->dinv[q];
-		d->z[q] = zv;
-		const double s = d->scale[q];
-		acc[0] += (rv * s) * zv;
-		acc[1] += (rv * s) * rv;
-		if (d->pressure) {
-			acc[2] += zv * s;
-			acc[3] += rv * s;
+	for (int t = 0; t < SSX_BATCH_CHUNK / 256; t += 2) {
+		long long qq[2];
+		double s[2], wt[2], pv[2][7];
+		int ix[2][7];
+#pragma unroll
+		for (int v = 0; v < 2; ++v) {
+			const long long q = q0 + (t + v) * 256 + threadIdx.x;
+			qq[v] = q;
+			s[v] = 0.0;
+			wt[v] = 1.0;
+#pragma unroll
+			for (int k = 0; k < 7; ++k)
+				ix[v][k] = -1;
+			if (q >= ntot)
+				continue;
+			s[v] = ap[q];
+			if (gP > 0) {
+				const long long e = q / n3;
+				const int code = gsoff[q - e * n3];
+				const int *pt = gtab + e * gP + (code >> 3);
+#pragma unroll
+				for (int k = 0; k < 7; ++k)
+					ix[v][k] = k < (code & 7) ? pt[k] : -1;
+			} else {
+				wt[v] = scale[q];
+			}
+		}
+#pragma unroll
+		for (int v = 0; v < 2; ++v)
+#pragma unroll
+			for (int k = 0; k < 7; ++k)
+				pv[v][k] = ix[v][k] >= 0 ? ap[ix[v][k]] : 0.0;
+#pragma unroll
+		for (int v = 0; v < 2; ++v) {
+			const long long q = qq[v];
+			if (q >= ntot)
+				break;
+			double sv = s[v];
+			if (gP > 0) {
+				int m = 1;
+				// own + partners left to right; unused entries are -1 and
+				// come last
+#pragma unroll
+				for (int k = 0; k < 7; ++k)
+					if (ix[v][k] >= 0) {
+						sv = sv + pv[v][k];
+						++m;
+					}
+				wt[v] = 1.0 / m;
+			}
+			u[q] = a * pp[q] + u[q];
+			const double rv = -a * sv + r[q];
+			r[q] = rv;
+			const double zv = rv * dinv[q];
+			z[q] = zv;
+			const double ws = wt[v];
+			acc[0] += (rv * ws) * zv;
+			acc[1] += (rv * ws) * rv;
+			if (pressure) {
+				acc[2] += zv * ws;
+				acc[3] += rv * ws;
+			}
 		}
 	}
 	double *base = d->part + d->nel + d->nbd;
