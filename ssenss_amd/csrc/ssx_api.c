@@ -768,6 +768,19 @@ static int dot_to(ssx_ctx *c, int l, const double *x, const double *y,
 	return ssxk_reduce(c->stream, c->part, 0, c->scal, dst);
 }
 
+/* global_opeval's element loop and mask (solver.c:191-198) without the
+ * DSS, into the level's Ap buffer: the gather-form kernels assemble it as
+ * they read it */
+static int opeval_raw(ssx_ctx *c, int l, int op, int var, const double *p)
+{
+	ssx_level *L = &c->lev[l];
+	if (l == 0)
+		c->cnt[0]++;
+	return ssxk_ax(c->stream, L->n, L->axv[0], c->nel, op, p, L->ap, L->gf,
+			L->mass, L->dmat, c->lambda, L->bc, var != SSX_PRESSURE, NULL,
+			NULL);
+}
+
 static int read_scal(ssx_ctx *c, int slot, double *v)
 {
 	HIPC(hipMemcpyAsync(c->h_scal + slot, c->scal + slot, sizeof(double),
@@ -917,17 +930,31 @@ static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
 	const int o = SC_CG(l);
 	int *flags = c->cg_flags + 2 * l;
 	TRY(ssx_mask(c, l, var, f));
-	if (x_zero) {
-		TRY(ssxk_copy(st, r, f, nt));
+	if (!x_zero && !pres && c->gather_upd && L->gP > 0) {
+		/* r = f - A u in one pass over the unassembled A u (bit-identical
+		 * to the DSS launch and axpby below).  z, p and <r,z> stay
+		 * separate here: the fully fused start (used by the FP32
+		 * smoother, where it is bit-identical) moved the FP64 32^3
+		 * solution by ~1e-16 after a few outer iterations */
+		TRY(opeval_raw(c, l, op, var, u));
+		TRY(ssxk_resid_g(st, 0, nt, L->n3, L->gP, L->gsoff, L->gtab, ap, f,
+				NULL, r, NULL, NULL, NULL, NULL));
+		TRY(ssxk_mul(st, nt, r, dinv, z));
+		TRY(ssxk_copy(st, p, z, nt));
+		TRY(dot_to(c, l, r, z, o + CG_RDZ));
 	} else {
-		TRY(ssx_opeval(c, l, op, var, u, r));
-		TRY(ssxk_axpby(st, nt, 1.0, f, NULL, -1.0, r));
+		if (x_zero) {
+			TRY(ssxk_copy(st, r, f, nt));
+		} else {
+			TRY(ssx_opeval(c, l, op, var, u, r));
+			TRY(ssxk_axpby(st, nt, 1.0, f, NULL, -1.0, r));
+		}
+		TRY(ssx_nullspace(c, l, var, r));
+		TRY(ssxk_mul(st, nt, r, dinv, z));
+		TRY(ssx_nullspace(c, l, var, z));
+		TRY(ssxk_copy(st, p, z, nt));
+		TRY(dot_to(c, l, r, z, o + CG_RDZ));
 	}
-	TRY(ssx_nullspace(c, l, var, r));
-	TRY(ssxk_mul(st, nt, r, dinv, z));
-	TRY(ssx_nullspace(c, l, var, z));
-	TRY(ssxk_copy(st, p, z, nt));
-	TRY(dot_to(c, l, r, z, o + CG_RDZ));
 	TRY(ssxk_set_ints(st, flags, 0, 2));
 	double eps_dev = eps;
 	if (rtol > 0.0) {       /* stop at max(eps, rtol * |r0|_w) */
@@ -1071,7 +1098,11 @@ static int mg_recurse(ssx_ctx *c, int l, int op, int var, double *f,
 	ssx_level *L = &c->lev[l], *C = &c->lev[l + 1];
 	void *st = c->stream;
 	double *r = L->mg_r, *fn = C->mg_fn, *e = C->mg_e;
-	if (l == 0) {                    /* precon.c:302-309 */
+	if (l == 0 && c->gather_upd && L->gP > 0) {   /* precon.c:302-309 */
+		TRY(opeval_raw(c, l, op, var, x));
+		TRY(ssxk_resid_g(st, 0, L->ntot, L->n3, L->gP, L->gsoff, L->gtab,
+				L->ap, f, NULL, r, NULL, NULL, NULL, NULL));
+	} else if (l == 0) {
 		TRY(ssx_opeval(c, l, op, var, x, r));
 		TRY(ssxk_axpby(st, L->ntot, 1.0, f, NULL, -1.0, r));
 	} else {
@@ -1182,6 +1213,17 @@ static int opeval_lp(ssx_ctx *c, int l, int op, int var, const float *p,
 			NULL);
 }
 
+/* opeval_lp without the DSS, into the level's FP32 Ap buffer */
+static int opeval_raw_lp(ssx_ctx *c, int l, int op, int var, const float *p)
+{
+	ssx_level *L = &c->lev[l];
+	if (l == 0)
+		c->cnt[0]++;
+	return ssxk_ax_lp(c->stream, L->lp_prec, L->n, L->axv[L->lp_prec],
+			c->nel, op, p, L->ap_lp, L->gf_lp, L->mass_lp, L->dmat_lp,
+			c->lambda, L->bc, var != SSX_PRESSURE, NULL, NULL);
+}
+
 static int nullspace_lp(ssx_ctx *c, int l, int var, float *u)
 {
 	if (var != SSX_PRESSURE)
@@ -1207,15 +1249,24 @@ static int cg_level_lp(ssx_ctx *c, int l, int op, int var, float *f,
 	const float *dinv = jdiag_lp(c, l, var);
 	const int o = SC_CG(l);
 	int *flags = c->cg_flags + 2 * l;
-	if (mask)
-		TRY(ssxk_mul_lp(st, nt, f, mask, f));
-	TRY(opeval_lp(c, l, op, var, u, r));
-	TRY(ssxk_axpby_lp(st, nt, 1.0, f, NULL, -1.0, r));
-	TRY(nullspace_lp(c, l, var, r));
-	TRY(ssxk_mul_lp(st, nt, r, dinv, z));
-	TRY(nullspace_lp(c, l, var, z));
-	TRY(ssxk_copy_lp(st, p, z, nt));
-	TRY(ssxk_dot_lp(st, nt, r, L->scale_lp, z, c->part, 0));
+	if (!pres && c->gather_upd && L->gP > 0) {
+		/* f = f mask, r = f - A u, z = D^-1 r, p = z and <r,z>_w in one
+		 * pass over the unassembled A u (bit-identical to the sequence
+		 * below) */
+		TRY(opeval_raw_lp(c, l, op, var, u));
+		TRY(ssxk_resid_g_lp(st, 1, nt, L->n3, L->gP, L->gsoff, L->gtab, ap,
+				f, mask, r, z, p, dinv, c->part));
+	} else {
+		if (mask)
+			TRY(ssxk_mul_lp(st, nt, f, mask, f));
+		TRY(opeval_lp(c, l, op, var, u, r));
+		TRY(ssxk_axpby_lp(st, nt, 1.0, f, NULL, -1.0, r));
+		TRY(nullspace_lp(c, l, var, r));
+		TRY(ssxk_mul_lp(st, nt, r, dinv, z));
+		TRY(nullspace_lp(c, l, var, z));
+		TRY(ssxk_copy_lp(st, p, z, nt));
+		TRY(ssxk_dot_lp(st, nt, r, L->scale_lp, z, c->part, 0));
+	}
 	TRY(ssxk_reduce(st, c->part, 0, c->scal, o + CG_RDZ));
 	TRY(ssxk_set_ints(st, flags, 0, 2));
 	const long long nba = c->nel;   /* operator partials: one per element */
@@ -1304,7 +1355,11 @@ static int mg_recurse_lp(ssx_ctx *c, int l, int op, int var, double eps)
 	}
 	ssx_level *C = &c->lev[l + 1];
 	float *r = L->mgr_lp;
-	if (l == 0) {
+	if (l == 0 && c->gather_upd && L->gP > 0) {
+		TRY(opeval_raw_lp(c, l, op, var, L->e_lp));
+		TRY(ssxk_resid_g_lp(st, 0, L->ntot, L->n3, L->gP, L->gsoff, L->gtab,
+				L->ap_lp, L->fn_lp, NULL, r, NULL, NULL, NULL, NULL));
+	} else if (l == 0) {
 		TRY(opeval_lp(c, l, op, var, L->e_lp, r));
 		TRY(ssxk_axpby_lp(st, L->ntot, 1.0, L->fn_lp, NULL, -1.0, r));
 	} else {

@@ -283,6 +283,18 @@ int ssxk_cg_update_g_lp(void *stream, size_t n, int n3, int P,
 		const int *soff, const int *tab, const double *scal, int ia,
 		const void *p, const void *ap, void *u, void *r, void *z,
 		const void *dinv, int pressure, double *part, const int *done);
+/* r = f - DSS(a) with the DSS of the unassembled a as a gather (lists as
+ * ssxk_cg_update_g); init != 0: the CG start -- f = f * mask first (mask
+ * nullable, f written back), then r, z = dinv r, p = z and the k_dot
+ * partials of <r,z>_w in part[0..SSX_NPART) (ssxk_reduce slot 0) */
+int ssxk_resid_g(void *stream, int init, size_t n, int n3, int P,
+		const int *soff, const int *tab, const double *a, double *f,
+		const double *mask, double *r, double *z, double *p,
+		const double *dinv, double *part);
+int ssxk_resid_g_lp(void *stream, int init, size_t n, int n3, int P,
+		const int *soff, const int *tab, const void *a, void *f,
+		const void *mask, void *r, void *z, void *p, const void *dinv,
+		double *part);
 /* ssxk_cg_update on a grid of nblocks <= SSX_NPART workgroups (the mode-1
  * reduce then takes cnt = nblocks) */
 int ssxk_cg_update_nb(void *stream, size_t n, const double *scal, int ia,

@@ -521,6 +521,136 @@ extern "C" int ssxk_cg_update_g_lp(void *stream, size_t n, int n3, int P,
 			tab, scal, ia, p, ap, u, r, z, dinv, pressure, part, done);
 }
 
+__device__ __forceinline__ double block_sum256(double v, double *sh);
+
+// The residual r = f - DSS(a) with the DSS of the operator's unassembled
+// output a done as a gather (k_cg_update_g's partner lists and order, so
+// the assembled value is the group kernel's bit for bit).
+//   INIT false: r only (the V-cycle's compute_residual, precon.c:302-309,
+//               after the unassembled global_opeval).
+//   INIT true:  the start of the CG (solver.c:46-57, var != PRESSURE):
+//               f = f * mask first (mask nullable, f written back), then
+//               r, z = dinv r, p = z, and the partials of <r,z>_w in the
+//               grid, per-thread order and form of k_dot (part[blockIdx],
+//               weights 1/M), so ssxk_reduce gives k_dot's value.
+// f - s is the exact k_axpby(1, f, -1, r) result: both products are exact.
+template <typename T, bool INIT>
+__global__ void __launch_bounds__(256)
+k_resid_g(unsigned n, unsigned n3, int P, const int *__restrict__ soff,
+		const int *__restrict__ tab, const T *__restrict__ a,
+		T *__restrict__ f, const T *__restrict__ mask,
+		T *__restrict__ r, T *__restrict__ z, T *__restrict__ p,
+		const T *__restrict__ dinv, double *__restrict__ part)
+{
+	constexpr int U = 2;
+	__shared__ double sh[256];
+	const unsigned stride = gridDim.x * 256u;
+	double acc = 0.0;
+	for (unsigned q0 = blockIdx.x * 256u + threadIdx.x; q0 < n;
+			q0 += U * stride) {
+		int ix[U][7];
+		T own[U], pv[U][7];
+#pragma unroll
+		for (int v = 0; v < U; ++v) {
+			const unsigned q = q0 + v * stride;
+			own[v] = T(0);
+			if (q < n) {
+				const unsigned e = q / n3;
+				const int c = soff[q - e * n3];
+				own[v] = a[q];
+				const int *pt = tab + (size_t) e * P + (c >> 3);
+#pragma unroll
+				for (int k = 0; k < 7; ++k)
+					ix[v][k] = k < (c & 7) ? pt[k] : -1;
+			} else {
+#pragma unroll
+				for (int k = 0; k < 7; ++k)
+					ix[v][k] = -1;
+			}
+		}
+#pragma unroll
+		for (int v = 0; v < U; ++v)
+#pragma unroll
+			for (int k = 0; k < 7; ++k)
+				pv[v][k] = ix[v][k] >= 0 ? a[ix[v][k]] : T(0);
+#pragma unroll
+		for (int v = 0; v < U; ++v) {
+			const unsigned q = q0 + v * stride;
+			if (q >= n)
+				break;
+			T s = own[v];
+			int m = 1;
+#pragma unroll
+			for (int k = 0; k < 7; ++k)
+				if (ix[v][k] >= 0) {
+					s = s + pv[v][k];
+					++m;
+				}
+			T fv = f[q];
+			if (INIT && mask) {
+				fv = fv * mask[q];
+				f[q] = fv;
+			}
+			const T rv = fv - s;
+			r[q] = rv;
+			if (INIT) {
+				const T zv = rv * dinv[q];
+				z[q] = zv;
+				p[q] = zv;
+				const T wr = rv * (T(1) / T(m));
+				acc += (double) wr * (double) zv;
+			}
+		}
+	}
+	if (INIT) {
+		const double t = block_sum256(acc, sh);
+		if (threadIdx.x == 0)
+			part[blockIdx.x] = t;
+	}
+}
+
+template <typename T>
+static int launch_resid_g(hipStream_t st, int init, size_t n, int n3, int P,
+		const int *soff, const int *tab, const void *a, void *f,
+		const void *mask, void *r, void *z, void *p, const void *dinv,
+		double *part)
+{
+	if (n >= (1ull << 31) || n % (size_t) n3) {
+		ssx_set_error("k_resid_g: %zu nodes, %d per element", n, n3);
+		return SSX_EINVAL;
+	}
+	if (init)
+		hipLaunchKernelGGL((k_resid_g<T, true>), dim3(SSX_NPART), dim3(256),
+				0, st, (unsigned) n, (unsigned) n3, P, soff, tab,
+				(const T *) a, (T *) f, (const T *) mask, (T *) r, (T *) z,
+				(T *) p, (const T *) dinv, part);
+	else
+		hipLaunchKernelGGL((k_resid_g<T, false>), dim3(SSX_NPART), dim3(256),
+				0, st, (unsigned) n, (unsigned) n3, P, soff, tab,
+				(const T *) a, (T *) f, (const T *) mask, (T *) r, (T *) z,
+				(T *) p, (const T *) dinv, part);
+	LAUNCH_CHECK("k_resid_g");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_resid_g(void *stream, int init, size_t n, int n3, int P,
+		const int *soff, const int *tab, const double *a, double *f,
+		const double *mask, double *r, double *z, double *p,
+		const double *dinv, double *part)
+{
+	return launch_resid_g<double>((hipStream_t) stream, init, n, n3, P, soff,
+			tab, a, f, mask, r, z, p, dinv, part);
+}
+
+extern "C" int ssxk_resid_g_lp(void *stream, int init, size_t n, int n3,
+		int P, const int *soff, const int *tab, const void *a, void *f,
+		const void *mask, void *r, void *z, void *p, const void *dinv,
+		double *part)
+{
+	return launch_resid_g<float>((hipStream_t) stream, init, n, n3, P, soff,
+			tab, a, f, mask, r, z, p, dinv, part);
+}
+
 // p = (z - c) + beta p, masked (c: pressure null-space constant)
 template <typename T>
 __global__ void __launch_bounds__(256)
