@@ -1398,13 +1398,18 @@ static int mg_precon_lp(ssx_ctx *c, int op, int var, double eps,
 	ssx_level *L = &c->lev[0];
 	void *st = c->stream;
 	c->cnt[3]++;
-	TRY(ssxk_cvt(st, SSX_PREC_FP64, SSX_PREC_FP32, x, L->fn_lp, L->ntot));
-	TRY(ssxk_mul_lp(st, L->ntot, L->fn_lp, jdiag_lp(c, 0, var), L->e_lp));
+	/* FP32 right-hand side and the Jacobi guess in one pass */
+	TRY(ssxk_d2f_jacobi(st, L->ntot, x, L->fn_lp, jdiag_lp(c, 0, var),
+			L->e_lp));
 	TRY(mg_recurse_lp(c, 0, op, var, eps));
-	TRY(ssxk_cvt(st, SSX_PREC_FP32, SSX_PREC_FP64, L->e_lp, px, L->ntot));
 	/* the FP32 smoother leaves the copies of a shared node unequal at FP32
 	 * rounding; averaging them in FP64 returns an assembled vector, as the
-	 * FP64 cycle does, so the FCG iterate stays continuous */
+	 * FP64 cycle does, so the FCG iterate stays continuous.  With the
+	 * partner lists the conversion and the average are one gather pass. */
+	if (c->gather_upd && L->gP > 0)
+		return ssxk_f2d_average_g(st, L->ntot, L->n3, L->gP, L->gsoff,
+				L->gtab, L->e_lp, px);
+	TRY(ssxk_cvt(st, SSX_PREC_FP32, SSX_PREC_FP64, L->e_lp, px, L->ntot));
 	return ssxk_dss_all(st, L->grp, L->ngrp, px, NULL, NULL, 1, NULL);
 }
 

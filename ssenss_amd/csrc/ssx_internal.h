@@ -295,6 +295,13 @@ int ssxk_resid_g_lp(void *stream, int init, size_t n, int n3, int P,
 		const int *soff, const int *tab, const void *a, void *f,
 		const void *mask, void *r, void *z, void *p, const void *dinv,
 		double *part);
+/* the mixed V-cycle's ends: y = (float) x and e = y * dinv (FP32 dinv);
+ * y = the FP64 DSS-average of the FP32 E-vector e as a gather (lists as
+ * ssxk_cg_update_g): bit-identical to ssxk_cvt + averaging ssxk_dss_all */
+int ssxk_d2f_jacobi(void *stream, size_t n, const double *x, void *y,
+		const void *dinv, void *e);
+int ssxk_f2d_average_g(void *stream, size_t n, int n3, int P,
+		const int *soff, const int *tab, const void *e, double *y);
 /* ssxk_cg_update on a grid of nblocks <= SSX_NPART workgroups (the mode-1
  * reduce then takes cnt = nblocks) */
 int ssxk_cg_update_nb(void *stream, size_t n, const double *scal, int ia,

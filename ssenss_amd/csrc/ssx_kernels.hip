@@ -1784,6 +1784,99 @@ __global__ void k_d2bf(size_t n, const double *__restrict__ x,
 	}
 }
 
+// The mixed-precision V-cycle's ends in one pass each.
+//   k_d2f_jac:  y = (float) x; e = y * dinv  (k_d2f, then k_mul)
+//   k_f2d_avg_g: y = DSS-average of (double) e over the copies of each
+//               shared node, as a gather over the per-copy partner lists:
+//               own + partners in the copy's add order, times 1/M (k_f2d,
+//               then the averaging group kernel, bit for bit)
+__global__ void k_d2f_jac(size_t n, const double *__restrict__ x,
+		float *__restrict__ y, const float *__restrict__ dinv,
+		float *__restrict__ e)
+{
+	GRID_STRIDE(q, n) {
+		const float v = (float) x[q];
+		y[q] = v;
+		e[q] = v * dinv[q];
+	}
+}
+
+__global__ void __launch_bounds__(256)
+k_f2d_avg_g(unsigned n, unsigned n3, int P, const int *__restrict__ soff,
+		const int *__restrict__ tab, const float *__restrict__ e,
+		double *__restrict__ y)
+{
+	constexpr int U = 2;
+	const unsigned stride = gridDim.x * 256u;
+	for (unsigned q0 = blockIdx.x * 256u + threadIdx.x; q0 < n;
+			q0 += U * stride) {
+		int ix[U][7];
+		float own[U], pv[U][7];
+#pragma unroll
+		for (int v = 0; v < U; ++v) {
+			const unsigned q = q0 + v * stride;
+			own[v] = 0.0f;
+			if (q < n) {
+				const unsigned el = q / n3;
+				const int c = soff[q - el * n3];
+				own[v] = e[q];
+				const int *pt = tab + (size_t) el * P + (c >> 3);
+#pragma unroll
+				for (int k = 0; k < 7; ++k)
+					ix[v][k] = k < (c & 7) ? pt[k] : -1;
+			} else {
+#pragma unroll
+				for (int k = 0; k < 7; ++k)
+					ix[v][k] = -1;
+			}
+		}
+#pragma unroll
+		for (int v = 0; v < U; ++v)
+#pragma unroll
+			for (int k = 0; k < 7; ++k)
+				pv[v][k] = ix[v][k] >= 0 ? e[ix[v][k]] : 0.0f;
+#pragma unroll
+		for (int v = 0; v < U; ++v) {
+			const unsigned q = q0 + v * stride;
+			if (q >= n)
+				break;
+			double s = (double) own[v];
+			int m = 1;
+#pragma unroll
+			for (int k = 0; k < 7; ++k)
+				if (ix[v][k] >= 0) {
+					s = s + (double) pv[v][k];
+					++m;
+				}
+			y[q] = m > 1 ? s * (1.0 / (double) m) : s;
+		}
+	}
+}
+
+extern "C" int ssxk_d2f_jacobi(void *stream, size_t n, const double *x,
+		void *y, const void *dinv, void *e)
+{
+	hipLaunchKernelGGL(k_d2f_jac, vgrid(n), dim3(256), 0,
+			(hipStream_t) stream, n, x, (float *) y, (const float *) dinv,
+			(float *) e);
+	LAUNCH_CHECK("k_d2f_jac");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_f2d_average_g(void *stream, size_t n, int n3, int P,
+		const int *soff, const int *tab, const void *e, double *y)
+{
+	if (n >= (1ull << 31) || n % (size_t) n3) {
+		ssx_set_error("k_f2d_avg_g: %zu nodes, %d per element", n, n3);
+		return SSX_EINVAL;
+	}
+	hipLaunchKernelGGL(k_f2d_avg_g, dim3(SSX_NPART), dim3(256), 0,
+			(hipStream_t) stream, (unsigned) n, (unsigned) n3, P, soff, tab,
+			(const float *) e, y);
+	LAUNCH_CHECK("k_f2d_avg_g");
+	return SSX_OK;
+}
+
 extern "C" int ssxk_cvt(void *stream, int from, int to, const void *x,
 		void *y, size_t n)
 {
