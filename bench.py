@@ -290,7 +290,8 @@ def main():
                     help="skip the N = 5 / 7 / 11 mixed-precision sweep")
     ap.add_argument("--profile", action="store_true",
                     help="kernels only, for rocprofv3 counter runs: the "
-                    "Poisson Ax on the workload, then its DSS, then a 1 GiB "
+                    "Poisson Ax on the workload, then its DSS, two outer "
+                    "FCG iterations with FP64 and FP32 smoothers, then a 1 GiB "
                     "device copy (known bytes, to calibrate FETCH_SIZE/"
                     "WRITE_SIZE)")
     args = ap.parse_args()
@@ -335,6 +336,11 @@ def main():
             ctx.ax(0, S.OP_POISSON, u, au)
         for _ in range(args.steps):
             ctx.dss(0, au)
+        # two outer FCG iterations per smoother precision: the fused CG
+        # update with the gather DSS, the residual kernels and the
+        # coarse-level CG
+        solve_case(S, torch, ctx, mesh, 2, precs=("fp64", "fp32"),
+                   log_prefix="profile ")
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()

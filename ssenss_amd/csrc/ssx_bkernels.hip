@@ -82,7 +82,7 @@ k_bdss(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 __global__ void __launch_bounds__(256)
 k_bupdate(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 {
-	const ssx_bwork w = tab[blockIdx.x];
+	const ssx_bwork w = tab[xcd_block()];
 	const ssx_bprob *d = pr + w.prob;
 	if (d->flags[0])
 		return;

@@ -21,6 +21,22 @@ __device__ __forceinline__ float ldv(bf16_t x)
 	return __uint_as_float(((unsigned) x) << 16);
 }
 
+// Logical workgroup of a gather kernel.  Workgroups go to the 8 XCDs
+// round-robin (workgroup b on XCD b % 8), and each XCD has its own L2.
+// Renumbering so that XCD x takes a contiguous range of logical workgroups
+// (the i-th workgroup it receives is the i-th of its range) gives each XCD
+// a contiguous run of elements: the partner copies a node gathers from the
+// neighbouring elements are then mostly in the L2 that loaded them.  The
+// map is a bijection of [0, nb), and logical workgroup lb covers exactly
+// the nodes workgroup lb covered before, so partials stored at lb keep
+// every reduction's order.
+__device__ __forceinline__ unsigned xcd_block(void)
+{
+	const unsigned nb = gridDim.x, b = blockIdx.x;
+	const unsigned x = b & 7u, q = nb >> 3, rem = nb & 7u;
+	return x * q + (x < rem ? x : rem) + (b >> 3);
+}
+
 // partner order of copy t: t ^ ord(k), k = 1..M-1 (faces x,y,z, then edges
 // along z, y, x, then the corner; global.c:252-256)
 __device__ __forceinline__ int dss_ord(int M, int k)

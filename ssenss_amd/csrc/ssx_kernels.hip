@@ -412,8 +412,9 @@ k_cg_update_g(unsigned n, unsigned n3, int P, const int *__restrict__ soff,
 	__shared__ double sh[4][256];
 	const T a = (T) scal[ia];
 	const unsigned stride = gridDim.x * 256u;
+	const unsigned lb = xcd_block();
 	double acc[4] = { 0.0, 0.0, 0.0, 0.0 };
-	for (unsigned q0 = blockIdx.x * 256u + threadIdx.x; q0 < n;
+	for (unsigned q0 = lb * 256u + threadIdx.x; q0 < n;
 			q0 += U * stride) {
 		int code[U], ix[U][7];
 		T own[U], pv[U][7];
@@ -474,7 +475,7 @@ k_cg_update_g(unsigned n, unsigned n3, int P, const int *__restrict__ soff,
 	}
 	block_sum4(acc, sh);
 	if (threadIdx.x < 4)
-		part[(size_t) threadIdx.x * SSX_NPART + blockIdx.x] = sh[threadIdx.x][0];
+		part[(size_t) threadIdx.x * SSX_NPART + lb] = sh[threadIdx.x][0];
 }
 
 template <typename T>
@@ -531,7 +532,8 @@ __device__ __forceinline__ double block_sum256(double v, double *sh);
 //   INIT true:  the start of the CG (solver.c:46-57, var != PRESSURE):
 //               f = f * mask first (mask nullable, f written back), then
 //               r, z = dinv r, p = z, and the partials of <r,z>_w in the
-//               grid, per-thread order and form of k_dot (part[blockIdx],
+//               grid, per-thread order and form of k_dot (one partial per
+//               logical workgroup,
 //               weights 1/M), so ssxk_reduce gives k_dot's value.
 // f - s is the exact k_axpby(1, f, -1, r) result: both products are exact.
 template <typename T, bool INIT>
@@ -545,8 +547,9 @@ k_resid_g(unsigned n, unsigned n3, int P, const int *__restrict__ soff,
 	constexpr int U = 2;
 	__shared__ double sh[256];
 	const unsigned stride = gridDim.x * 256u;
+	const unsigned lb = xcd_block();
 	double acc = 0.0;
-	for (unsigned q0 = blockIdx.x * 256u + threadIdx.x; q0 < n;
+	for (unsigned q0 = lb * 256u + threadIdx.x; q0 < n;
 			q0 += U * stride) {
 		int ix[U][7];
 		T own[U], pv[U][7];
@@ -605,7 +608,7 @@ k_resid_g(unsigned n, unsigned n3, int P, const int *__restrict__ soff,
 	if (INIT) {
 		const double t = block_sum256(acc, sh);
 		if (threadIdx.x == 0)
-			part[blockIdx.x] = t;
+			part[lb] = t;
 	}
 }
 
@@ -1273,6 +1276,19 @@ __device__ __forceinline__ double csr_pn(const double *pold,
 	return pj;
 }
 
+// csr_pn on values already loaded (pold, z and mask at the column)
+__device__ __forceinline__ double csr_pv(double pj, double zj, double mj,
+		bool has_mask, int first, double b, double cc, int ic)
+{
+	if (!first) {
+		const double zv = ic >= 0 ? cc * 1.0 + zj : zj;
+		pj = b * pj + zv;
+		if (has_mask)
+			pj = pj * mj;
+	}
+	return pj;
+}
+
 __global__ void __launch_bounds__(256)
 k_csr_pupd(int nu, const int *__restrict__ rowptr,
 		const int *__restrict__ col, const double *__restrict__ val,
@@ -1417,6 +1433,10 @@ __device__ double sum1_as_1024(const double *__restrict__ part, long long m,
 	return t;
 }
 
+// the row entries a lane holds in registers across the reduction replay
+// (rows of up to CSR_LPR * CSR_PRE entries; longer rows load the rest after)
+#define CSR_PRE 4
+
 __global__ void __launch_bounds__(256)
 k_csr_cg_spmv(int nu, const int *__restrict__ rowptr,
 		const int *__restrict__ col, const double *__restrict__ val,
@@ -1430,6 +1450,45 @@ k_csr_cg_spmv(int nu, const int *__restrict__ rowptr,
 	if (flags[0])
 		return;
 	__shared__ double s4[4][256];
+	const int first = j == 0;
+	const int lane = threadIdx.x % CSR_LPR;
+	const int u = (blockIdx.x * 256 + threadIdx.x) / CSR_LPR;
+	// Every load that does not depend on beta goes out before the
+	// reduction replay, so the two latency chains overlap: the lane's
+	// first CSR_PRE entries with their column's pold, z and mask, and the
+	// row's own.
+	int kb = 0, ke = 0;
+	double va[CSR_PRE], pa[CSR_PRE], za[CSR_PRE], ma[CSR_PRE];
+	double po = 0.0, zo = 0.0, mo = 1.0;
+	if (!check && u < nu) {
+		kb = rowptr[u] + lane;
+		ke = rowptr[u + 1];
+#pragma unroll
+		for (int i = 0; i < CSR_PRE; ++i) {
+			const int k = kb + i * CSR_LPR;
+			va[i] = 0.0;
+			pa[i] = 0.0;
+			za[i] = 0.0;
+			ma[i] = 1.0;
+			if (k < ke) {
+				const int c = col[k];
+				va[i] = val[k];
+				pa[i] = pold[c];
+				if (!first) {
+					za[i] = z[c];
+					if (mask)
+						ma[i] = mask[c];
+				}
+			}
+		}
+		if (lane == 0) {
+			po = pold[u];
+			if (!first)
+				zo = z[u];
+			if (mask)
+				mo = mask[u];
+		}
+	}
 	double b = 0.0, cc = 0.0;
 	if (j > 0) {
 		// cg_scalars1 of iteration j-1, in every workgroup
@@ -1459,21 +1518,25 @@ k_csr_cg_spmv(int nu, const int *__restrict__ rowptr,
 	}
 	if (check)
 		return;
-	const int first = j == 0;
 	const int ic = pressure ? 0 : -1;      // csr_pn: null-space shift on
-	const int lane = threadIdx.x % CSR_LPR;
-	const int u = (blockIdx.x * 256 + threadIdx.x) / CSR_LPR;
 	double pd = 0.0;
 	double y = 0.0;
-	if (u < nu)
-		for (int k = rowptr[u] + lane; k < rowptr[u + 1]; k += CSR_LPR)
+	if (u < nu) {
+#pragma unroll
+		for (int i = 0; i < CSR_PRE; ++i)
+			if (kb + i * CSR_LPR < ke)
+				y += va[i] * csr_pv(pa[i], za[i], ma[i], mask != nullptr,
+						first, b, cc, ic);
+		for (int k = kb + CSR_PRE * CSR_LPR; k < ke; k += CSR_LPR)
 			y += val[k] * csr_pn(pold, z, mask, col[k], first, b, cc, ic);
+	}
 	for (int w = 1; w < CSR_LPR; w <<= 1)
 		y += __shfl_xor(y, w, CSR_LPR);
 	if (u < nu && lane == 0) {
-		const double pv = csr_pn(pold, z, mask, u, first, b, cc, ic);
+		const double pv = csr_pv(po, zo, mo, mask != nullptr, first, b, cc,
+				ic);
 		pnew[u] = pv;
-		if (mask && mask[u] == 0.0)
+		if (mask && mo == 0.0)
 			y = 0.0;
 		ap[u] = y;
 		pd = pv * y;
@@ -1494,6 +1557,19 @@ k_csr_cg_update(size_t n, double *__restrict__ scal, int o, int j,
 	if (flags[0])
 		return;
 	__shared__ double sh[4][256];
+	// the lane's first node is loaded before the reduction replay (the
+	// loads do not depend on alpha), the rest of its nodes after
+	const size_t q0 = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	const size_t gs = (size_t) gridDim.x * blockDim.x;
+	double p0 = 0.0, ap0 = 0.0, u0 = 0.0, r0 = 0.0, d0 = 0.0, w0 = 0.0;
+	if (q0 < n) {
+		p0 = p[q0];
+		ap0 = ap[q0];
+		u0 = u[q0];
+		r0 = r[q0];
+		d0 = dinv[q0];
+		w0 = scale[q0];
+	}
 	// cg_scalars0, in every workgroup
 	const double pap = sum1_as_1024(ps, nbs, sh[0]);
 	const double a = scal[o + ((j & 1) ? CG_RDZ1 : CG_RDZ)] / pap;
@@ -1502,13 +1578,19 @@ k_csr_cg_update(size_t n, double *__restrict__ scal, int o, int j,
 		scal[o + CG_ALPHA] = a;
 	}
 	double acc[4] = { 0.0, 0.0, 0.0, 0.0 };
-	GRID_STRIDE(q, n) {
-		u[q] = a * p[q] + u[q];
-		const double rv = -a * ap[q] + r[q];
+	for (size_t q = q0; q < n; q += gs) {
+		const bool pre = q == q0;
+		const double pq = pre ? p0 : p[q];
+		const double apq = pre ? ap0 : ap[q];
+		const double uq = pre ? u0 : u[q];
+		const double rq = pre ? r0 : r[q];
+		const double dq = pre ? d0 : dinv[q];
+		const double w = pre ? w0 : scale[q];
+		u[q] = a * pq + uq;
+		const double rv = -a * apq + rq;
 		r[q] = rv;
-		const double zv = rv * dinv[q];
+		const double zv = rv * dq;
 		z[q] = zv;
-		const double w = scale[q];
 		acc[0] += (double) (rv * w) * (double) zv;
 		acc[1] += (double) (rv * w) * (double) rv;
 		if (pressure) {
@@ -1808,7 +1890,7 @@ k_f2d_avg_g(unsigned n, unsigned n3, int P, const int *__restrict__ soff,
 {
 	constexpr int U = 2;
 	const unsigned stride = gridDim.x * 256u;
-	for (unsigned q0 = blockIdx.x * 256u + threadIdx.x; q0 < n;
+	for (unsigned q0 = xcd_block() * 256u + threadIdx.x; q0 < n;
 			q0 += U * stride) {
 		int ix[U][7];
 		float own[U], pv[U][7];
