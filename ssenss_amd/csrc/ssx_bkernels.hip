@@ -198,11 +198,36 @@ k_breduce(int mode, const ssx_bprob *__restrict__ pr)
 	const int nq = mode == 0 ? 1 : (d->pressure ? 4 : 2);
 	const double *base = d->part + (mode == 0 ? 0 : d->nel + d->nbd);
 	const long long m = mode == 0 ? d->nel : d->nbu;
+	// the lane's partials are added in index order but loaded 8 at a time
+	// (the first of every channel together), so their latencies overlap
+	const long long t = threadIdx.x;
+	double first[4];
+#pragma unroll
+	for (int c = 0; c < 4; ++c) {
+		// branch-free: an unused slot loads base[0] and discards it
+		const bool use = c < nq && t < m;
+		const double v = base[use ? (long long) c * m + t : 0];
+		first[c] = use ? v : 0.0;
+	}
 	for (int c = 0; c < 4; ++c) {
 		double acc = 0.0;
-		if (c < nq)
-			for (long long q = threadIdx.x; q < m; q += 1024)
-				acc += base[(long long) c * m + q];
+		if (c < nq && t < m) {
+			const double *pp = base + (long long) c * m;
+			acc += first[c];
+			// 8 loads in flight; past m a slot loads pp[0] unused
+			for (long long q = t + 1024; q < m; q += 8 * 1024) {
+				double v[8];
+#pragma unroll
+				for (int i = 0; i < 8; ++i) {
+					const long long qi = q + i * 1024;
+					v[i] = pp[qi < m ? qi : 0];
+				}
+#pragma unroll
+				for (int i = 0; i < 8; ++i)
+					if (q + i * 1024 < m)
+						acc += v[i];
+			}
+		}
 		sh[c][threadIdx.x] = acc;
 	}
 	__syncthreads();

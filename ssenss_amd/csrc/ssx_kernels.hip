@@ -702,12 +702,16 @@ extern "C" int ssxk_cg_uflush_lp(void *stream, size_t n, const double *scal,
 	return SSX_OK;
 }
 
+// partials a reduction lane loads ahead of its adds
+#define RED_PRE 8
+
 // single-block reductions + scalar logic of one CG iteration
 //   mode 0: s[o+PAP] = sum part[0..cnt); s[o+ALPHA] = s[o+RDZ] / pap
 //   mode 1: rz, rr (, zs, rs) from the 4 update partials; null-space
 //           constant c = zs/nunique; rdz = rz - c rs; beta = rdz / rdz_old;
 //           iteration count ++, done when sqrt(rr) <= eps or count >= maxit
 //           (eps < 0: the tolerance is s[o+TOL], formed on the device)
+
 __global__ void __launch_bounds__(1024)
 k_cg_reduce(int mode, const double *__restrict__ part, long long cnt,
 		double *__restrict__ scal, int o, int *__restrict__ flags,
@@ -717,14 +721,37 @@ k_cg_reduce(int mode, const double *__restrict__ part, long long cnt,
 		return;
 	__shared__ double sh[4][1024];
 	const int nq = mode == 0 ? 1 : (pressure ? 4 : 2);
+	const long long m = (mode == 0 || cnt > 0) ? cnt : SSX_NPART;
+	const long long t = threadIdx.x;
+	// The lane's partials are added in index order, as one loop would, but
+	// loaded RED_PRE at a time (and every channel's first together) so
+	// that the loads' latencies overlap.
+	double first[4];
+#pragma unroll
+	for (int c = 0; c < 4; ++c) {
+		// branch-free: an unused slot loads part[0] and discards it
+		const bool use = c < nq && t < m;
+		const double v = part[use ? (size_t) c * SSX_NPART + t : 0];
+		first[c] = use ? v : 0.0;
+	}
 	for (int c = 0; c < 4; ++c) {
 		double acc = 0.0;
-		if (c < nq) {
-			const double *pp = mode == 0 ? part
-				: part + (size_t) c * SSX_NPART;
-			const long long m = (mode == 0 || cnt > 0) ? cnt : SSX_NPART;
-			for (long long q = threadIdx.x; q < m; q += 1024)
-				acc += pp[q];
+		if (c < nq && t < m) {
+			const double *pp = part + (size_t) c * SSX_NPART;
+			acc += first[c];
+			// RED_PRE loads in flight; past m a slot loads pp[0] unused
+			for (long long q = t + 1024; q < m; q += RED_PRE * 1024) {
+				double v[RED_PRE];
+#pragma unroll
+				for (int i = 0; i < RED_PRE; ++i) {
+					const long long qi = q + i * 1024;
+					v[i] = pp[qi < m ? qi : 0];
+				}
+#pragma unroll
+				for (int i = 0; i < RED_PRE; ++i)
+					if (q + i * 1024 < m)
+						acc += v[i];
+			}
 		}
 		sh[c][threadIdx.x] = acc;
 	}
@@ -1376,15 +1403,29 @@ extern "C" int ssxk_csr_pupd(void *stream, int nu, const int *rowptr,
 __device__ void sum4_as_1024(const double *__restrict__ part, long long m,
 		int nq, double out[4], double (*sh)[256])
 {
+	// each virtual lane's first partial of every channel is loaded before
+	// any add, so the loads' latencies overlap; the adds keep index order
+	double x[4][4];
+#pragma unroll
+	for (int c = 0; c < 4; ++c)
+#pragma unroll
+		for (int v = 0; v < 4; ++v) {
+			const long long q = threadIdx.x + 256 * v;
+			x[c][v] = (c < nq && q < m) ? part[(size_t) c * SSX_NPART + q]
+				: 0.0;
+		}
 #pragma unroll
 	for (int c = 0; c < 4; ++c) {
 		double a[4];
 #pragma unroll
 		for (int v = 0; v < 4; ++v) {
 			double acc = 0.0;
-			if (c < nq)
-				for (long long q = threadIdx.x + 256 * v; q < m; q += 1024)
+			const long long q0 = threadIdx.x + 256 * v;
+			if (c < nq && q0 < m) {
+				acc += x[c][v];
+				for (long long q = q0 + 1024; q < m; q += 1024)
 					acc += part[(size_t) c * SSX_NPART + q];
+			}
 			a[v] = acc;
 		}
 		a[0] += a[2];
@@ -1410,11 +1451,26 @@ __device__ void sum4_as_1024(const double *__restrict__ part, long long m,
 __device__ double sum1_as_1024(const double *__restrict__ part, long long m,
 		double *sh)
 {
+	// the first two partials of each virtual lane are loaded before any
+	// add (latencies overlap); the adds keep index order
+	double x[4][2];
+#pragma unroll
+	for (int v = 0; v < 4; ++v)
+#pragma unroll
+		for (int k = 0; k < 2; ++k) {
+			const long long q = threadIdx.x + 256 * v + 1024 * k;
+			x[v][k] = q < m ? part[q] : 0.0;
+		}
 	double a[4];
 #pragma unroll
 	for (int v = 0; v < 4; ++v) {
 		double acc = 0.0;
-		for (long long q = threadIdx.x + 256 * v; q < m; q += 1024)
+		const long long q0 = threadIdx.x + 256 * v;
+#pragma unroll
+		for (int k = 0; k < 2; ++k)
+			if (q0 + 1024 * k < m)
+				acc += x[v][k];
+		for (long long q = q0 + 2048; q < m; q += 1024)
 			acc += part[q];
 		a[v] = acc;
 	}
