@@ -186,19 +186,22 @@ def particle_case(S, torch, stream, npart=1_000_000, seed=7):
                             "reseeded": treinit}}
 
 
-def batch_case(S, torch, nprob=16, seed=11):
+def batch_case(S, torch, nprob=16, seed=11, mrange=(8, 16),
+               points=(5, 6, 8, 10, 12)):
     """Uneven batch (north-star configuration 4, reduced to nprob problems
     of at most 16^3 elements so the bench stays within minutes):
     independent Helmholtz problems on randomly deformed boxes, element
-    counts m^3 with m in [8, 16], orders N in {4,5,7,9,11}, lambda ~
-    U[1, 100], Jacobi-PCG to 1e-10 relative.  The batch (one launch per CG
-    stage, graph-replayed) against the same problems solved one by one."""
+    counts m^3 with m in mrange, orders N in {4,5,7,9,11} (points per
+    direction n = N + 1 from `points`), lambda ~ U[1, 100], Jacobi-PCG to
+    1e-10 relative.  The batch (one launch per CG stage, graph-replayed)
+    against the same problems solved one by one; both are timed on their
+    second run."""
     from ssenss_amd.mesh import box_mesh
     rng = np.random.default_rng(seed)
     ctxs, rhs = [], []
     for _ in range(nprob):
-        m = int(rng.integers(8, 17))
-        n = int(rng.choice([5, 6, 8, 10, 12]))
+        m = int(rng.integers(mrange[0], mrange[1] + 1))
+        n = int(rng.choice(list(points)))
         lam = rng.uniform(1, 100)
         mesh = box_mesh(n, (m, m, m), deform=1, amp=rng.uniform(0, 0.15),
                         dt=(11.0 / 6.0) / lam, visc=1.0)
@@ -213,16 +216,17 @@ def batch_case(S, torch, nprob=16, seed=11):
         rhs.append(b)
     eps = [1e-10 * float(np.sqrt(c.dot(0, b, b))) for c, b in zip(ctxs, rhs)]
     dofs = sum(c.ntot(0) for c in ctxs)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    single_its = []
-    for c, b, e in zip(ctxs, rhs, eps):
-        f = b.clone()
-        u = torch.zeros_like(f)
-        st = c.cg(0, S.OP_HELMHOLTZ, S.X_VELOCITY, f, u, e, 5000)
-        single_its.append(st.iterations)
-    torch.cuda.synchronize()
-    t_seq = time.perf_counter() - t0
+    for rep in range(2):                 # the first run is a warm-up
+        fs = [b.clone() for b in rhs]
+        us = [torch.zeros_like(f) for f in fs]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        single_its = []
+        for c, f, u, e in zip(ctxs, fs, us, eps):
+            st = c.cg(0, S.OP_HELMHOLTZ, S.X_VELOCITY, f, u, e, 5000)
+            single_its.append(st.iterations)
+        torch.cuda.synchronize()
+        t_seq = time.perf_counter() - t0
     batch = S.Batch([(c, S.OP_HELMHOLTZ, S.X_VELOCITY) for c in ctxs])
     res = {}
     for graph in (False, True):
@@ -238,12 +242,15 @@ def batch_case(S, torch, nprob=16, seed=11):
     for c in ctxs:
         c.close()
     return {"problems": nprob, "local_dofs": dofs,
+            "elements_per_dir": list(mrange),
             "orders": sorted({c.mesh.n - 1 for c in ctxs}),
             "sequential_s": t_seq, "batched_s": res[False][0],
             "batched_graph_s": res[True][0],
             "speedup_batched_graph": t_seq / res[True][0],
             "iterations_single": single_its,
-            "iterations_batched": res[True][1]}
+            "iterations_batched": res[True][1],
+            "max_iteration_difference": max(
+                abs(a - b) for a, b in zip(single_its, res[True][1]))}
 
 
 def cpu_baseline(mesh, seconds):
@@ -385,12 +392,18 @@ def main():
             f"{particles['seconds']*1e3:.2f} ms per step, "
             f"{particles['max_passes']} passes; RK2 tracers "
             f"{particles['tracers_rk2']['seconds']*1e3:.2f} ms per step")
-    batchres = None if args.no_batch else batch_case(S, torch)
-    if batchres:
-        log(f"[bench] batch of {batchres['problems']}: sequential "
-            f"{batchres['sequential_s']:.3f} s, batched "
-            f"{batchres['batched_s']:.3f} s, with graphs "
-            f"{batchres['batched_graph_s']:.3f} s")
+    batchres = batch_small = None
+    if not args.no_batch:
+        batchres = batch_case(S, torch)
+        # many small problems, where one problem leaves most of the GPU idle
+        batch_small = batch_case(S, torch, nprob=64, seed=12, mrange=(3, 6),
+                                 points=(5, 6, 8))
+    for br in (batchres, batch_small):
+        if br:
+            log(f"[bench] batch of {br['problems']} ({br['local_dofs']} "
+                f"nodes): sequential {br['sequential_s']:.3f} s, batched "
+                f"{br['batched_s']:.3f} s, with graphs "
+                f"{br['batched_graph_s']:.3f} s")
 
     sweep = {}
     if not args.no_sweep:
@@ -435,6 +448,7 @@ def main():
         "ax_launch_variants": variants,
         "particles": particles,
         "batch_uneven": batchres,
+        "batch_small": batch_small,
         "mixed_precision_sweep": sweep or None,
     }
     print(json.dumps(out), flush=True)

@@ -528,6 +528,12 @@ int ssx_batch_cg(ssx_batch *b, double *const *f, double *const *u,
 	free(act);
 	if (rc0 != SSX_OK)
 		return rc0;
+	/* the last iteration's solution step, which the update leaves to the
+	 * next operator launch (u += alpha p, the arithmetic of the update) */
+	for (int i = 0; i < np; ++i)
+		TRYB(ssxk_cg_uflush(st, (size_t) b->h_prob[i].ntot,
+				b->scal + CG_NSLOT * i, CG_ALPHA, b->h_prob[i].p,
+				b->h_prob[i].u));
 	HIPB(hipMemcpyAsync(b->h_scal, b->scal, sizeof(double) * CG_NSLOT * np,
 			hipMemcpyDeviceToHost, st));
 	HIPB(hipStreamSynchronize(st));

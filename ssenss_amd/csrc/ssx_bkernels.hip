@@ -25,8 +25,10 @@
 		}                                                          \
 	} while (0)
 
-// operator (PUPD: with the direction update of the previous iteration);
-// one table per (order, operator) group
+// operator (PUPD: with the direction update of the previous iteration,
+// and before it the previous iteration's solution step u += alpha p_old,
+// which the update kernel leaves to it); one table per (order, operator)
+// group
 template <int N, int OP, bool PUPD, int V>
 __global__ void __launch_bounds__((AxShape<N, V>::BLOCK))
 k_bax(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
@@ -39,10 +41,11 @@ k_bax(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 	// PUPD reads z and p and writes p; otherwise p is the operand
 	const double *u = PUPD ? nullptr : d->p;
 	double *pio = PUPD ? d->p : nullptr;
-	const double pd = ax_team<N, OP, double, double, PUPD ? 1 : 0, V>(
+	const double pd = ax_team<N, OP, double, double, PUPD ? 2 : 0, V>(
 			w.blk, d->nel,
 			u, d->ap, d->gf, d->mass, d->dmat, d->lambda, d->bc,
-			d->apply_mask, d->z, pio, d->scal, CG_BETA, ic, nullptr, 0);
+			d->apply_mask, d->z, pio, d->scal, CG_BETA, ic,
+			PUPD ? d->u : nullptr, CG_ALPHA);
 	ax_elem_partial<N, V>(pd, d->part, w.blk, d->nel);
 }
 
@@ -72,8 +75,9 @@ k_bdss(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 	}
 }
 
-// u += alpha p; r -= alpha Ap; z = D^-1 r; partials of <r,z>_w, <r,r>_w,
-// <z,1>_w, <r,1>_w over this workgroup's chunk.  A problem with the gather
+// r -= alpha Ap; z = D^-1 r; partials of <r,z>_w, <r,r>_w, <z,1>_w,
+// <r,1>_w over this workgroup's chunk.  The solution step u += alpha p is
+// left to the next operator launch (or the batch's final flush).  A problem with the gather
 // form (gP > 0) had no DSS launch: each copy adds its partners' values to
 // its own in its own add order (the group kernel's value bit for bit) and
 // its weight is 1/M from the partner count; the lane takes two of its
@@ -97,10 +101,8 @@ k_bupdate(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 	const int *__restrict__ gsoff = d->gsoff;
 	const int *__restrict__ gtab = d->gtab;
 	const double *__restrict__ ap = d->ap;
-	const double *__restrict__ pp = d->p;
 	const double *__restrict__ dinv = d->dinv;
 	const double *__restrict__ scale = d->scale;
-	double *__restrict__ u = d->u;
 	double *__restrict__ r = d->r;
 	double *__restrict__ z = d->z;
 	double acc[4] = { 0.0, 0.0, 0.0, 0.0 };
@@ -155,7 +157,6 @@ k_bupdate(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 					}
 				wt[v] = 1.0 / m;
 			}
-			u[q] = a * pp[q] + u[q];
 			const double rv = -a * sv + r[q];
 			r[q] = rv;
 			const double zv = rv * dinv[q];
