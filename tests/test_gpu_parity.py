@@ -615,6 +615,51 @@ def test_batch_cg_matches_single_solves_and_reference():
             c.close()
 
 
+def test_batch_many_small_problems_match_single_solves():
+    """40 small Helmholtz problems of orders 2..7 on 2^3..4^3-element boxes
+    (random deformation and lambda), more than one flag block of the batch:
+    each problem's iteration count within one of the single-problem
+    solver's, its solution within the solve tolerance, and the final
+    solution step (taken after the last iteration) applied exactly once"""
+    rng = np.random.default_rng(2026)
+    ctxs, rhs = [], []
+    try:
+        for _ in range(40):
+            n = int(rng.choice([3, 4, 5, 6, 8]))
+            m = int(rng.integers(2, 5))
+            lam = rng.uniform(1, 100)
+            mesh = box_mesh(n, (m, m, m), deform=1, amp=rng.uniform(0, 0.15),
+                            dt=(11.0 / 6.0) / lam, visc=1.0)
+            c = make_ctx(mesh)
+            ctxs.append(c)
+            b = dev(rng.standard_normal(mesh.ntot)
+                    * c.level_data(0, S.DATA_MASS))
+            c.dss(0, b)
+            rhs.append(host(b))
+        eps = [1e-10 * np.linalg.norm(r) for r in rhs]
+        batch = S.Batch([(c, S.OP_HELMHOLTZ, S.X_VELOCITY) for c in ctxs])
+        try:
+            fs = [dev(r) for r in rhs]
+            us = [torch.zeros_like(f) for f in fs]
+            stats = batch.cg(fs, us, eps, 5000)
+        finally:
+            batch.close()
+        worst = 0
+        for k, c in enumerate(ctxs):
+            f = dev(rhs[k])
+            u = torch.zeros_like(f)
+            st1 = c.cg(0, S.OP_HELMHOLTZ, S.X_VELOCITY, f, u, eps[k], 5000)
+            assert stats[k].converged
+            assert abs(stats[k].iterations - st1.iterations) <= 1
+            assert rel(host(us[k]), host(u)) <= 1e-8
+            worst = max(worst, abs(stats[k].iterations - st1.iterations))
+        print(f"[gpu] batch of 40 small problems: largest iteration "
+              f"difference {worst}")
+    finally:
+        for c in ctxs:
+            c.close()
+
+
 def test_batch_graph_replays_match_eager_launches():
     """the captured-graph iterations give bit for bit what eager launches
     give, replay after replay, with the program's own eager work and
