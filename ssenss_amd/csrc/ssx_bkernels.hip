@@ -98,13 +98,17 @@ k_bupdate(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 	// taken to alias them
 	const long long ntot = d->ntot;
 	const int pressure = d->pressure;
-	const int *__restrict__ gsoff = d->gsoff;
-	const int *__restrict__ gtab = d->gtab;
-	const double *__restrict__ ap = d->ap;
-	const double *__restrict__ dinv = d->dinv;
-	const double *__restrict__ scale = d->scale;
-	double *__restrict__ r = d->r;
-	double *__restrict__ z = d->z;
+	const SSX_GLOBAL int *__restrict__ gsoff =
+		(const SSX_GLOBAL int *) d->gsoff;
+	const SSX_GLOBAL int *__restrict__ gtab = (const SSX_GLOBAL int *) d->gtab;
+	const SSX_GLOBAL double *__restrict__ ap =
+		(const SSX_GLOBAL double *) d->ap;
+	const SSX_GLOBAL double *__restrict__ dinv =
+		(const SSX_GLOBAL double *) d->dinv;
+	const SSX_GLOBAL double *__restrict__ scale =
+		(const SSX_GLOBAL double *) d->scale;
+	SSX_GLOBAL double *__restrict__ r = (SSX_GLOBAL double *) d->r;
+	SSX_GLOBAL double *__restrict__ z = (SSX_GLOBAL double *) d->z;
 	double acc[4] = { 0.0, 0.0, 0.0, 0.0 };
 	const long long q0 = (long long) w.blk * SSX_BATCH_CHUNK;
 	for (int t = 0; t < SSX_BATCH_CHUNK / 256; t += 2) {
@@ -126,7 +130,7 @@ k_bupdate(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 			if (gP > 0) {
 				const long long e = q / n3;
 				const int code = gsoff[q - e * n3];
-				const int *pt = gtab + e * gP + (code >> 3);
+				const SSX_GLOBAL int *pt = gtab + e * gP + (code >> 3);
 #pragma unroll
 				for (int k = 0; k < 7; ++k)
 					ix[v][k] = k < (code & 7) ? pt[k] : -1;

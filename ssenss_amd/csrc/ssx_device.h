@@ -21,6 +21,13 @@ __device__ __forceinline__ float ldv(bf16_t x)
 	return __uint_as_float(((unsigned) x) << 16);
 }
 
+// A pointer read from device memory (a batch problem's description) is a
+// generic pointer: loads and stores through it compile to flat memory
+// instructions, which also count against the LDS counter and make the
+// compiler wait more often.  The buffers are all in global memory, and a
+// pointer of the global address space gets global loads and stores.
+#define SSX_GLOBAL __attribute__((address_space(1)))
+
 // Logical workgroup of a gather kernel.  Workgroups go to the 8 XCDs
 // round-robin (workgroup b on XCD b % 8), and each XCD has its own L2.
 // Renumbering so that XCD x takes a contiguous range of logical workgroups

@@ -1968,11 +1968,16 @@ k_f2d_avg_g(unsigned n, unsigned n3, int P, const int *__restrict__ soff,
 					ix[v][k] = -1;
 			}
 		}
+		// branch-free: an unused slot loads e[0] and discards it (a
+		// conditional load here was compiled into one wait per load)
 #pragma unroll
 		for (int v = 0; v < U; ++v)
 #pragma unroll
-			for (int k = 0; k < 7; ++k)
-				pv[v][k] = ix[v][k] >= 0 ? e[ix[v][k]] : 0.0f;
+			for (int k = 0; k < 7; ++k) {
+				const int j = ix[v][k];
+				const float t = e[j >= 0 ? j : 0];
+				pv[v][k] = j >= 0 ? t : 0.0f;
+			}
 #pragma unroll
 		for (int v = 0; v < U; ++v) {
 			const unsigned q = q0 + v * stride;
