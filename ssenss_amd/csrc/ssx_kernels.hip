@@ -410,24 +410,34 @@ k_cg_update_g(unsigned n, unsigned n3, int P, const int *__restrict__ soff,
 	if (done && *done)
 		return;
 	__shared__ double sh[4][256];
+	constexpr bool HOIST = sizeof(T) == 4;
 	const T a = (T) scal[ia];
 	const unsigned stride = gridDim.x * 256u;
 	const unsigned lb = xcd_block();
 	double acc[4] = { 0.0, 0.0, 0.0, 0.0 };
 	for (unsigned q0 = lb * 256u + threadIdx.x; q0 < n;
 			q0 += U * stride) {
-		int code[U], ix[U][7];
-		T own[U], pv[U][7];
+		int ix[U][7];
+		T own[U], pv[U][7], rq[U], dq[U], pq[U], uq[U];
 #pragma unroll
 		for (int v = 0; v < U; ++v) {
 			const unsigned q = q0 + v * stride;
-			code[v] = 0;
 			own[v] = T(0);
+			rq[v] = dq[v] = pq[v] = uq[v] = T(0);
 			if (q < n) {
 				const unsigned e = q / n3;
 				const int c = soff[q - e * n3];
-				code[v] = (c & 7) ? c : 0;
 				own[v] = ap[q];
+				// FP32: the node's own operands go out with the lists
+				// (measured faster; in FP64 the extra registers cost more)
+				if (HOIST) {
+					rq[v] = r[q];
+					dq[v] = dinv[q];
+					if (UPD) {
+						pq[v] = p[q];
+						uq[v] = u[q];
+					}
+				}
 				const int *pt = tab + (size_t) e * P + (c >> 3);
 #pragma unroll
 				for (int k = 0; k < 7; ++k)
@@ -460,10 +470,10 @@ k_cg_update_g(unsigned n, unsigned n3, int P, const int *__restrict__ soff,
 				}
 			const T w = T(1) / T(m);
 			if (UPD)
-				u[q] = a * p[q] + u[q];
-			const T rv = -a * s + r[q];
+				u[q] = a * (HOIST ? pq[v] : p[q]) + (HOIST ? uq[v] : u[q]);
+			const T rv = -a * s + (HOIST ? rq[v] : r[q]);
 			r[q] = rv;
-			const T zv = rv * dinv[q];
+			const T zv = rv * (HOIST ? dq[v] : dinv[q]);
 			z[q] = zv;
 			acc[0] += (double) (rv * w) * (double) zv;
 			acc[1] += (double) (rv * w) * (double) rv;
