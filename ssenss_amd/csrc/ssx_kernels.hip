@@ -555,6 +555,9 @@ k_resid_g(unsigned n, unsigned n3, int P, const int *__restrict__ soff,
 		const T *__restrict__ dinv, double *__restrict__ part)
 {
 	constexpr int U = 2;
+	// FP32: the node's own operands go out with the lists (as in
+	// k_cg_update_g)
+	constexpr bool HOIST = sizeof(T) == 4;
 	__shared__ double sh[256];
 	const unsigned stride = gridDim.x * 256u;
 	const unsigned lb = xcd_block();
@@ -562,15 +565,23 @@ k_resid_g(unsigned n, unsigned n3, int P, const int *__restrict__ soff,
 	for (unsigned q0 = lb * 256u + threadIdx.x; q0 < n;
 			q0 += U * stride) {
 		int ix[U][7];
-		T own[U], pv[U][7];
+		T own[U], pv[U][7], fq[U], mq[U], dq[U];
 #pragma unroll
 		for (int v = 0; v < U; ++v) {
 			const unsigned q = q0 + v * stride;
 			own[v] = T(0);
+			fq[v] = mq[v] = dq[v] = T(0);
 			if (q < n) {
 				const unsigned e = q / n3;
 				const int c = soff[q - e * n3];
 				own[v] = a[q];
+				if (HOIST) {
+					fq[v] = f[q];
+					if (INIT && mask)
+						mq[v] = mask[q];
+					if (INIT)
+						dq[v] = dinv[q];
+				}
 				const int *pt = tab + (size_t) e * P + (c >> 3);
 #pragma unroll
 				for (int k = 0; k < 7; ++k)
@@ -599,15 +610,15 @@ k_resid_g(unsigned n, unsigned n3, int P, const int *__restrict__ soff,
 					s = s + pv[v][k];
 					++m;
 				}
-			T fv = f[q];
+			T fv = HOIST ? fq[v] : f[q];
 			if (INIT && mask) {
-				fv = fv * mask[q];
+				fv = fv * (HOIST ? mq[v] : mask[q]);
 				f[q] = fv;
 			}
 			const T rv = fv - s;
 			r[q] = rv;
 			if (INIT) {
-				const T zv = rv * dinv[q];
+				const T zv = rv * (HOIST ? dq[v] : dinv[q]);
 				z[q] = zv;
 				p[q] = zv;
 				const T wr = rv * (T(1) / T(m));
