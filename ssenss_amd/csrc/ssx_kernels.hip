@@ -1431,9 +1431,11 @@ __device__ void sum4_as_1024(const double *__restrict__ part, long long m,
 	for (int c = 0; c < 4; ++c)
 #pragma unroll
 		for (int v = 0; v < 4; ++v) {
+			// branch-free: an unused slot loads part[0] and discards it
 			const long long q = threadIdx.x + 256 * v;
-			x[c][v] = (c < nq && q < m) ? part[(size_t) c * SSX_NPART + q]
-				: 0.0;
+			const bool use = c < nq && q < m;
+			const double t = part[use ? (size_t) c * SSX_NPART + q : 0];
+			x[c][v] = use ? t : 0.0;
 		}
 #pragma unroll
 	for (int c = 0; c < 4; ++c) {
@@ -1479,8 +1481,10 @@ __device__ double sum1_as_1024(const double *__restrict__ part, long long m,
 	for (int v = 0; v < 4; ++v)
 #pragma unroll
 		for (int k = 0; k < 2; ++k) {
+			// branch-free: an unused slot loads part[0] and discards it
 			const long long q = threadIdx.x + 256 * v + 1024 * k;
-			x[v][k] = q < m ? part[q] : 0.0;
+			const double t = part[q < m ? q : 0];
+			x[v][k] = q < m ? t : 0.0;
 		}
 	double a[4];
 #pragma unroll
