@@ -112,7 +112,7 @@ def solve_case(S, torch, ctx, mesh, max_iter, precs=("fp64", "fp32", "bf16"),
 def particle_case(S, torch, stream, npart=1_000_000, seed=7):
     """Inertial point particles (integrate_particle_equations), and RK2
     tracers (ssx_tracers_step) from the same positions, on the
-    channel of north-star configuration 5: 24x16x24 hexahedra at n = 10 on
+    channel workload: 24x16x24 hexahedra at n = 10 on
     [0,4pi] x [-1,1] x [0,4pi/3], tanh wall stretching 2.0, periodic in x
     and z, walls in y.  Velocity: Poiseuille u = 1 - y^2 plus a seeded
     divergence-free perturbation of 10% amplitude, set three times (the
@@ -161,7 +161,7 @@ def particle_case(S, torch, stream, npart=1_000_000, seed=7):
         times.append(time.perf_counter() - t0)
     moved = int((pr["el"].cpu().numpy() != base["el"]).sum())
     reinit = int(pr["reinit"].sum().item())
-    # RK2 tracers (configuration 5's tracer mode) on the same field and
+    # RK2 tracers (the channel workload's tracer mode) on the same field and
     # positions, one step of dt
     ctx.tracers_step(fresh(), T, dt)                # warm-up
     tt = []
@@ -188,7 +188,7 @@ def particle_case(S, torch, stream, npart=1_000_000, seed=7):
 
 def batch_case(S, torch, nprob=16, seed=11, mrange=(8, 16),
                points=(5, 6, 8, 10, 12)):
-    """Uneven batch (north-star configuration 4, reduced to nprob problems
+    """Uneven batch (the ensemble workload, reduced to nprob problems
     of at most 16^3 elements so the bench stays within minutes):
     independent Helmholtz problems on randomly deformed boxes, element
     counts m^3 with m in mrange, orders N in {4,5,7,9,11} (points per

@@ -12,7 +12,7 @@ CASES = ["box2_n5", "cube2_n8", "periodic3_n6", "channel_n6", "cube2_n12"]
 
 def pytest_configure(config):
     config.addinivalue_line(
-        "markers", "gpu: needs an MI355X (run with -m gpu on the GPU box)")
+        "markers", "gpu: needs an MI355X GPU (select with -m gpu)")
 
 
 def golden(name):
