@@ -25,6 +25,15 @@
 		}                                                          \
 	} while (0)
 
+// a generic pointer rebuilt from its address in the global address space:
+// loads through it (after inlining) can then be global loads rather than
+// flat ones
+template <typename T>
+__device__ __forceinline__ T *gptr(T *p)
+{
+	return (T *) (SSX_GLOBAL T *) (unsigned long long) p;
+}
+
 // operator (PUPD: with the direction update of the previous iteration,
 // and before it the previous iteration's solution step u += alpha p_old,
 // which the update kernel leaves to it); one table per (order, operator)
@@ -39,13 +48,13 @@ k_bax(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 		return;
 	const int ic = d->pressure ? CG_NULLC : -1;
 	// PUPD reads z and p and writes p; otherwise p is the operand
-	const double *u = PUPD ? nullptr : d->p;
-	double *pio = PUPD ? d->p : nullptr;
+	const double *u = PUPD ? nullptr : gptr(d->p);
+	double *pio = PUPD ? gptr(d->p) : nullptr;
 	const double pd = ax_team<N, OP, double, double, PUPD ? 2 : 0, V>(
 			w.blk, d->nel,
-			u, d->ap, d->gf, d->mass, d->dmat, d->lambda, d->bc,
-			d->apply_mask, d->z, pio, d->scal, CG_BETA, ic,
-			PUPD ? d->u : nullptr, CG_ALPHA);
+			u, gptr(d->ap), gptr(d->gf), gptr(d->mass), gptr(d->dmat),
+			d->lambda, gptr(d->bc), d->apply_mask, gptr(d->z), pio,
+			d->scal, CG_BETA, ic, PUPD ? gptr(d->u) : nullptr, CG_ALPHA);
 	ax_elem_partial<N, V>(pd, d->part, w.blk, d->nel);
 }
 
