@@ -25,15 +25,6 @@
 		}                                                          \
 	} while (0)
 
-// a generic pointer rebuilt from its address in the global address space:
-// loads through it (after inlining) can then be global loads rather than
-// flat ones
-template <typename T>
-__device__ __forceinline__ T *gptr(T *p)
-{
-	return (T *) (SSX_GLOBAL T *) (unsigned long long) p;
-}
-
 // operator (PUPD: with the direction update of the previous iteration,
 // and before it the previous iteration's solution step u += alpha p_old,
 // which the update kernel leaves to it); one table per (order, operator)
@@ -183,7 +174,7 @@ k_bupdate(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 			}
 		}
 	}
-	double *base = d->part + d->nel + d->nbd;
+	double *base = gptr(d->part) + d->nel + d->nbd;
 	for (int c = 0; c < 4; ++c) {
 		sh[threadIdx.x] = acc[c];
 		__syncthreads();
@@ -210,7 +201,7 @@ k_breduce(int mode, const ssx_bprob *__restrict__ pr)
 		return;
 	__shared__ double sh[4][1024];
 	const int nq = mode == 0 ? 1 : (d->pressure ? 4 : 2);
-	const double *base = d->part + (mode == 0 ? 0 : d->nel + d->nbd);
+	const double *base = gptr(d->part) + (mode == 0 ? 0 : d->nel + d->nbd);
 	const long long m = mode == 0 ? d->nel : d->nbu;
 	// the lane's partials are added in index order but loaded 8 at a time
 	// (the first of every channel together), so their latencies overlap

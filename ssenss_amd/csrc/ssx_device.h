@@ -28,6 +28,15 @@ __device__ __forceinline__ float ldv(bf16_t x)
 // pointer of the global address space gets global loads and stores.
 #define SSX_GLOBAL __attribute__((address_space(1)))
 
+// a generic pointer rebuilt from its address in the global address space:
+// loads through it (once inlined) can then be global loads rather than
+// flat ones
+template <typename T>
+__device__ __forceinline__ T *gptr(T *p)
+{
+	return (T *) (SSX_GLOBAL T *) (unsigned long long) p;
+}
+
 // Logical workgroup of a gather kernel.  Workgroups go to the 8 XCDs
 // round-robin (workgroup b on XCD b % 8), and each XCD has its own L2.
 // Renumbering so that XCD x takes a contiguous range of logical workgroups
