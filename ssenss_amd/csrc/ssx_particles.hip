@@ -32,8 +32,12 @@ struct PField {
 	double T, dt;           // params->t, params->dt
 };
 
+// the GLL points of the field, staged in LDS by each kernel (the field
+// argument itself stays unmodified, so its arrays keep global loads)
+static __shared__ double s_xi[PMAXN];
+
 // Lagrange polynomial, interpolation.c:31-39
-__device__ double lagr(double xi, const double *x, int j, int n)
+__device__ __forceinline__ double lagr(double xi, const double *x, int j, int n)
 {
 	double li = 1.0;
 	for (int i = 0; i < j; ++i)
@@ -43,7 +47,7 @@ __device__ double lagr(double xi, const double *x, int j, int n)
 	return li;
 }
 
-__device__ int first_pt(const double *x, double xp, int n, int m)
+__device__ __forceinline__ int first_pt(const double *x, double xp, int n, int m)
 {
 	for (int i = 0; i < n - m + 1; ++i)
 		if (x[i] > xp)
@@ -58,31 +62,31 @@ struct IW {
 	double lx[5], ly[5], lz[5];        // n <= 5 or the 5-point stencil
 };
 
-__device__ void weights(const PField &F, double x, double y, double z, IW &w)
+__device__ __forceinline__ void weights(const PField &F, double x, double y, double z, IW &w)
 {
 	const int n = F.n;
 	w.full = n <= 5;
 	if (w.full) {
 		w.i1 = w.j1 = w.k1 = 0;
 		for (int i = 0; i < n; ++i) {
-			w.lx[i] = lagr(x, F.xi, i, n);
-			w.ly[i] = lagr(y, F.xi, i, n);
-			w.lz[i] = lagr(z, F.xi, i, n);
+			w.lx[i] = lagr(x, s_xi, i, n);
+			w.ly[i] = lagr(y, s_xi, i, n);
+			w.lz[i] = lagr(z, s_xi, i, n);
 		}
 		return;
 	}
-	w.i1 = first_pt(F.xi, x, n, 5);
-	w.j1 = first_pt(F.xi, y, n, 5);
-	w.k1 = first_pt(F.xi, z, n, 5);
+	w.i1 = first_pt(s_xi, x, n, 5);
+	w.j1 = first_pt(s_xi, y, n, 5);
+	w.k1 = first_pt(s_xi, z, n, 5);
 	for (int i = 0; i < 5; ++i) {
-		w.lx[i] = lagr(x, F.xi + w.i1, i, 5);
-		w.ly[i] = lagr(y, F.xi + w.j1, i, 5);
-		w.lz[i] = lagr(z, F.xi + w.k1, i, 5);
+		w.lx[i] = lagr(x, s_xi + w.i1, i, 5);
+		w.ly[i] = lagr(y, s_xi + w.j1, i, 5);
+		w.lz[i] = lagr(z, s_xi + w.k1, i, 5);
 	}
 }
 
 // polynomial_interpolation, interpolation.c:41-96
-__device__ double interp(const PField &F, const IW &w, const double *f)
+__device__ __forceinline__ double interp(const PField &F, const IW &w, const double *f)
 {
 	const int n = F.n;
 	if (w.full) {
@@ -117,7 +121,7 @@ struct PState {
 enum { P_NORMAL, P_COLLISION, P_OUTSIDE };
 
 // fluid_point_velocity, particles.c:493-515 (all three components)
-__device__ void fluid(const PField &F, int el, const double *y, double t,
+__device__ __forceinline__ void fluid(const PField &F, int el, const double *y, double t,
 		double *uf)
 {
 	const double eps = 1.0e-6;
@@ -141,7 +145,7 @@ __device__ void fluid(const PField &F, int el, const double *y, double t,
 	}
 }
 
-__device__ void rhs(const PField &F, const PState &p, const double *y,
+__device__ __forceinline__ void rhs(const PField &F, const PState &p, const double *y,
 		double t, double *f)
 {
 	const double a = 1.0 / p.tau;           // particles.c:228-236
@@ -153,7 +157,7 @@ __device__ void rhs(const PField &F, const PState &p, const double *y,
 	}
 }
 
-__device__ int crossed(const double *y)
+__device__ __forceinline__ int crossed(const double *y)
 {
 	for (int i = 0; i < 3; ++i)
 		if (fabs(y[i]) > 1.0)
@@ -164,7 +168,7 @@ __device__ int crossed(const double *y)
 // the reference re-seeds with rand() (particles.c:170-180); here the
 // particle restarts at the element centre with the fluid velocity and is
 // flagged
-__device__ void reinit(const PField &F, PState &p)
+__device__ __forceinline__ void reinit(const PField &F, PState &p)
 {
 	double y[6] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
 	double uf[3];
@@ -177,7 +181,7 @@ __device__ void reinit(const PField &F, PState &p)
 }
 
 // particles.c:238-269
-__device__ void rk4(const PField &F, PState &p, double dt, int force)
+__device__ __forceinline__ void rk4(const PField &F, PState &p, double dt, int force)
 {
 	const double a[4] = { 0.0, 0.5, 0.5, 1.0 };
 	double y[6], f[4][6];
@@ -206,12 +210,12 @@ __device__ void rk4(const PField &F, PState &p, double dt, int force)
 	p.t = F.T;
 }
 
-__device__ double dmin2(double a, double b)
+__device__ __forceinline__ double dmin2(double a, double b)
 {
 	return a < b ? a : b;                   // utils.c:220-226
 }
 
-__device__ double step_to_boundary(const double *y, const double *f)
+__device__ __forceinline__ double step_to_boundary(const double *y, const double *f)
 {
 	double d[3];                            // particles.c:310-316
 	for (int i = 0; i < 3; ++i)
@@ -220,7 +224,7 @@ __device__ double step_to_boundary(const double *y, const double *f)
 }
 
 // particles.c:271-308
-__device__ void to_boundary(const PField &F, PState &p, double dt)
+__device__ __forceinline__ void to_boundary(const PField &F, PState &p, double dt)
 {
 	const double a[4] = { 0.0, 0.5, 0.5, 1.0 };
 	double y[6], f[5][6], y0[6];
@@ -254,7 +258,7 @@ __device__ void to_boundary(const PField &F, PState &p, double dt)
 }
 
 // particles.c:380-393
-__device__ int closest3(const double *s)
+__device__ __forceinline__ int closest3(const double *s)
 {
 	double hmin = 2.0;
 	int k = 0;
@@ -268,7 +272,7 @@ __device__ int closest3(const double *s)
 	return k;
 }
 
-__device__ int closest(const PState &p)
+__device__ __forceinline__ int closest(const PState &p)
 {
 	return closest3(p.s);
 }
@@ -276,7 +280,7 @@ __device__ int closest(const PState &p)
 // physical / parametric particle velocity, particles.c:517-539; the
 // callers overwrite u, v, w one component at a time, so later components
 // see the already converted earlier ones (kept as in the reference)
-__device__ double conv_vel(const PField &F, const PState &p,
+__device__ __forceinline__ double conv_vel(const PField &F, const PState &p,
 		const double *const *m, int comp)
 {
 	IW w;
@@ -289,7 +293,7 @@ __device__ double conv_vel(const PField &F, const PState &p,
 }
 
 // particles.c:333-349 (single rank)
-__device__ void new_element(const PField &F, PState &p)
+__device__ __forceinline__ void new_element(const PField &F, PState &p)
 {
 	const int j = closest(p);
 	const int nb = F.neigh[p.el * 6 + j];
@@ -304,7 +308,7 @@ __device__ void new_element(const PField &F, PState &p)
 }
 
 // particles.c:351-378 (neigh_rot is always 0)
-__device__ void new_location(const PField &F, PState &p)
+__device__ __forceinline__ void new_location(const PField &F, PState &p)
 {
 	double y0[3] = { p.s[0], p.s[1], p.s[2] };
 	double x2[2];
@@ -327,7 +331,7 @@ __device__ void new_location(const PField &F, PState &p)
 }
 
 __global__ void __launch_bounds__(64)
-k_particles_step(PField F, long n, int *__restrict__ el,
+k_particles_step(const PField F, long n, int *__restrict__ el,
 		int *__restrict__ old_el, double *__restrict__ x,
 		double *__restrict__ y, double *__restrict__ z,
 		double *__restrict__ u, double *__restrict__ v,
@@ -335,11 +339,9 @@ k_particles_step(PField F, long n, int *__restrict__ el,
 		const double *__restrict__ tau, int *__restrict__ reinit_out,
 		int *__restrict__ flags)
 {
-	__shared__ double sxi[PMAXN];
 	if (threadIdx.x < F.n)
-		sxi[threadIdx.x] = F.xi[threadIdx.x];
+		s_xi[threadIdx.x] = F.xi[threadIdx.x];
 	__syncthreads();
-	F.xi = sxi;
 	const long q = (long) blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= n)
 		return;
@@ -397,7 +399,7 @@ k_particles_step(PField F, long n, int *__restrict__ el,
 // the element centre (flagged), as the inertial walk does.
 // ---------------------------------------------------------------------
 
-__device__ double tstep_to_boundary(const double *y, const double *f)
+__device__ __forceinline__ double tstep_to_boundary(const double *y, const double *f)
 {
 	double d[3];                    // an axis without motion: never
 	for (int i = 0; i < 3; ++i)
@@ -406,18 +408,16 @@ __device__ double tstep_to_boundary(const double *y, const double *f)
 }
 
 __global__ void __launch_bounds__(64)
-k_tracers_step(PField F, long n, int *__restrict__ el,
+k_tracers_step(const PField F, long n, int *__restrict__ el,
 		int *__restrict__ old_el, double *__restrict__ x,
 		double *__restrict__ y, double *__restrict__ z,
 		double *__restrict__ u, double *__restrict__ v,
 		double *__restrict__ w, double *__restrict__ t,
 		int *__restrict__ reinit_out, int *__restrict__ flags)
 {
-	__shared__ double sxi[PMAXN];
 	if (threadIdx.x < F.n)
-		sxi[threadIdx.x] = F.xi[threadIdx.x];
+		s_xi[threadIdx.x] = F.xi[threadIdx.x];
 	__syncthreads();
-	F.xi = sxi;
 	const long q = (long) blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= n)
 		return;
