@@ -288,6 +288,18 @@ int ssx_tracers_step(ssx_ctx *ctx, const ssx_particles *p, double t_now,
 		double dt, int *max_passes);
 int ssx_tracers_step_host(ssx_ctx *ctx, const ssx_particles *p,
 		double t_now, double dt, int *max_passes);
+/* ssx_particles_step_host without any device allocation.  No counterpart
+ * in the reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * host lists are staged through a fixed device array the library holds on
+ * each GPU it runs on (until the process ends), for at most
+ * SSX_NOALLOC_MAX_PARTICLES particles (more: SSX_EINVAL).  The particle
+ * tables must exist already: call ssx_particles_set_velocity(_host) first
+ * (else SSX_EINVAL).  All calls on one GPU share that array: calls that
+ * overlap on one GPU race.  Results are those of ssx_particles_step_host. */
+#define SSX_NOALLOC_MAX_PARTICLES 131072
+int ssx_particles_step_host_noalloc(ssx_ctx *ctx, const ssx_particles *p,
+		double t_now, double dt, int *max_passes);
 
 /* ---- batches of independent problems ----
  * Jacobi-PCG (conjugate_gradient, solver.c:33-81) on level 0 of `count`

@@ -393,6 +393,11 @@ int ssx_create(ssx_ctx **out, const ssx_mesh *m)
 	ssx_ctx *c = calloc(1, sizeof(*c));
 	if (!c)
 		return SSX_ENOMEM;
+	if (hipGetDevice(&c->device) != hipSuccess) {
+		free(c);
+		ssx_set_error("ssx_create: hipGetDevice failed");
+		return SSX_EHIP;
+	}
 	c->nel = m->nel;
 	c->n = m->n;
 	c->diag_choice = -1;
@@ -1795,8 +1800,8 @@ int ssx_particles_set_velocity(ssx_ctx *c, const double *u, const double *v,
 
 /* one step of the particle walk on device arrays: inertial particles
  * (particles.c) or, with tracers, RK2 tracers (tau not used) */
-static int walk_step(ssx_ctx *c, const ssx_particles *p, double t_now,
-		double dt, int *max_passes, int tracers)
+int ssx_walk_dev(ssx_ctx *c, const ssx_particles *p, double t_now,
+		double dt, int *max_passes, int tracers, int *counters)
 {
 	TRY(chk(c, 0));
 	if (!p || p->n < 0 || (p->n > 0 && (!p->el || !p->old_el || !p->x
@@ -1809,23 +1814,34 @@ static int walk_step(ssx_ctx *c, const ssx_particles *p, double t_now,
 		ssx_set_error("particle step: dt must be positive");
 		return SSX_EINVAL;
 	}
-	TRY(particles_init(c));
+	if (counters) {
+		/* callers with their own counters allocate nothing: the tables
+		 * must exist already (ssx_particles_set_velocity sets them up) */
+		if (!c->p_maxpass) {
+			ssx_set_error("particle step: set the velocity "
+					"(ssx_particles_set_velocity) first");
+			return SSX_EINVAL;
+		}
+	} else {
+		TRY(particles_init(c));
+		counters = c->p_maxpass;
+	}
 	const double *ur[9];
 	for (int a = 0; a < 9; ++a)
 		ur[a] = c->p_ur[a / 3][a % 3];
-	TRY(ssxk_set_ints(c->stream, c->p_maxpass, 0, 2));
+	TRY(ssxk_set_ints(c->stream, counters, 0, 2));
 	if (tracers)
 		TRY(ssxk_tracers_step(c->stream, c->lev[0].n, c->nel, c->p_xi, ur,
 				c->d_neigh, t_now, dt, p->n, p->el, p->old_el, p->x, p->y,
-				p->z, p->u, p->v, p->w, p->t, p->reinit, c->p_maxpass));
+				p->z, p->u, p->v, p->w, p->t, p->reinit, counters));
 	else
 		TRY(ssxk_particles_step(c->stream, c->lev[0].n, c->nel, c->p_xi, ur,
 				(const double *const *) c->p_jac,
 				(const double *const *) c->p_jinv, c->d_neigh, t_now, dt,
 				p->n, p->el, p->old_el, p->x, p->y, p->z, p->u, p->v, p->w,
-				p->t, p->tau, p->reinit, c->p_maxpass));
+				p->t, p->tau, p->reinit, counters));
 	int fl[2] = { 0, 0 };
-	HIPC(hipMemcpyAsync(fl, c->p_maxpass, sizeof(fl), hipMemcpyDeviceToHost,
+	HIPC(hipMemcpyAsync(fl, counters, sizeof(fl), hipMemcpyDeviceToHost,
 			(hipStream_t) c->stream));
 	HIPC(hipStreamSynchronize((hipStream_t) c->stream));
 	if (max_passes)
@@ -1841,13 +1857,13 @@ static int walk_step(ssx_ctx *c, const ssx_particles *p, double t_now,
 int ssx_particles_step(ssx_ctx *c, const ssx_particles *p, double t_now,
 		double dt, int *max_passes)
 {
-	return walk_step(c, p, t_now, dt, max_passes, 0);
+	return ssx_walk_dev(c, p, t_now, dt, max_passes, 0, NULL);
 }
 
 int ssx_tracers_step(ssx_ctx *c, const ssx_particles *p, double t_now,
 		double dt, int *max_passes)
 {
-	return walk_step(c, p, t_now, dt, max_passes, 1);
+	return ssx_walk_dev(c, p, t_now, dt, max_passes, 1, NULL);
 }
 
 /* host-array variants: stage through device memory held by the context */
@@ -1890,8 +1906,14 @@ int ssx_particles_set_velocity_host(ssx_ctx *c, const double *u,
 	return SSX_OK;
 }
 
-static int walk_step_host(ssx_ctx *c, const ssx_particles *h, double t_now,
-		double dt, int *max_passes, int tracers)
+size_t ssx_walk_stage_bytes(size_t n)
+{
+	return n * (8 * sizeof(double) + 3 * sizeof(int));
+}
+
+int ssx_walk_host(ssx_ctx *c, const ssx_particles *h, double t_now,
+		double dt, int *max_passes, int tracers, void *stage,
+		size_t stage_cap, int *counters)
 {
 	TRY(chk(c, 0));
 	if (!h || h->n < 0 || (h->n > 0 && (!h->el || !h->old_el || !h->x
@@ -1907,8 +1929,18 @@ static int walk_step_host(ssx_ctx *c, const ssx_particles *h, double t_now,
 		return SSX_OK;
 	}
 	/* layout: 8 double columns x..tau, then el, old_el, reinit */
-	TRY(stage_bytes(c, n * (8 * sizeof(double) + 3 * sizeof(int))));
-	double *dd = (double *) c->p_stage;
+	if (stage) {
+		if (ssx_walk_stage_bytes(n) > stage_cap) {
+			ssx_set_error("particle step: %zu particles need %zu bytes of "
+					"staging, more than the %zu available", n,
+					ssx_walk_stage_bytes(n), stage_cap);
+			return SSX_EINVAL;
+		}
+	} else {
+		TRY(stage_bytes(c, ssx_walk_stage_bytes(n)));
+		stage = c->p_stage;
+	}
+	double *dd = (double *) stage;
 	int *di = (int *) (dd + 8 * n);
 	double *hcol[8] = { h->x, h->y, h->z, h->u, h->v, h->w, h->t, h->tau };
 	hipStream_t st = (hipStream_t) c->stream;
@@ -1922,7 +1954,8 @@ static int walk_step_host(ssx_ctx *c, const ssx_particles *h, double t_now,
 			hipMemcpyHostToDevice, st));
 	ssx_particles d = { h->n, di, di + n, dd, dd + n, dd + 2 * n, dd + 3 * n,
 		dd + 4 * n, dd + 5 * n, dd + 6 * n, dd + 7 * n, di + 2 * n };
-	const int rc = walk_step(c, &d, t_now, dt, max_passes, tracers);
+	const int rc = ssx_walk_dev(c, &d, t_now, dt, max_passes, tracers,
+			counters);
 	if (rc != SSX_OK && rc != SSX_EINVAL)
 		return rc;
 	/* results come back also when some particles named no element */
@@ -1943,13 +1976,13 @@ static int walk_step_host(ssx_ctx *c, const ssx_particles *h, double t_now,
 int ssx_particles_step_host(ssx_ctx *c, const ssx_particles *h, double t_now,
 		double dt, int *max_passes)
 {
-	return walk_step_host(c, h, t_now, dt, max_passes, 0);
+	return ssx_walk_host(c, h, t_now, dt, max_passes, 0, NULL, 0, NULL);
 }
 
 int ssx_tracers_step_host(ssx_ctx *c, const ssx_particles *h, double t_now,
 		double dt, int *max_passes)
 {
-	return walk_step_host(c, h, t_now, dt, max_passes, 1);
+	return ssx_walk_host(c, h, t_now, dt, max_passes, 1, NULL, 0, NULL);
 }
 
 /* ------------------------------------------------------------------ */

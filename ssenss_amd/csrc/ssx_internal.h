@@ -100,6 +100,7 @@ typedef struct {
 
 struct ssx_ctx {
 	int nel, nlev, n;
+	int device;                  /* HIP device current at ssx_create      */
 	int nmg[SSX_LMAX];
 	ssx_level lev[SSX_LMAX];
 	double lambda;
@@ -399,6 +400,19 @@ int ssxk_axpby(void *stream, size_t n, double a, const double *x,
 int ssxk_mul(void *stream, size_t n, const double *x, const double *w,
 		double *z);
 /* per-block partials of sum(x*s*y) into part[slot*NPART + blk] */
+/* one step of the particle walk (inertial particles, or RK2 tracers) on
+ * device particle arrays; counters: two device ints (largest pass count,
+ * particles naming no element), NULL for the context's own, which then
+ * also allows the particle tables to be set up on first use */
+int ssx_walk_dev(ssx_ctx *c, const ssx_particles *p, double t_now, double dt,
+		int *max_passes, int tracers, int *counters);
+/* the same on host particle arrays staged through `stage` (stage_cap
+ * bytes; NULL: the context's own staging, grown on demand) */
+int ssx_walk_host(ssx_ctx *c, const ssx_particles *h, double t_now,
+		double dt, int *max_passes, int tracers, void *stage,
+		size_t stage_cap, int *counters);
+/* device bytes ssx_walk_host stages for n particles */
+size_t ssx_walk_stage_bytes(size_t n);
 int ssxk_dot(void *stream, size_t n, const double *x, const double *s,
 		const double *y, double *part, int slot);
 /* scal[dst] = sum of part[slot*NPART ...] (fixed order) */

@@ -231,6 +231,21 @@ class Context:
                                                 dt, C.byref(mp)))
         return mp.value
 
+    def particles_step_host_noalloc(self, p, t_now, dt):
+        """particles_step_host through ssx_particles_step_host_noalloc (no
+        device allocation; at most NOALLOC_MAX_PARTICLES particles)"""
+        for k in ("el", "old_el", "reinit"):
+            assert p[k].dtype == np.int32 and p[k].flags.c_contiguous
+        for k in ("x", "y", "z", "u", "v", "w", "t", "tau"):
+            assert p[k].dtype == np.float64 and p[k].flags.c_contiguous
+        ps = ParticlesStruct(int(p["x"].size), *(p[k].ctypes.data for k in
+                             ("el", "old_el", "x", "y", "z", "u", "v", "w",
+                              "t", "tau", "reinit")))
+        mp = C.c_int()
+        check(self._lib.ssx_particles_step_host_noalloc(
+            self.h, C.byref(ps), t_now, dt, C.byref(mp)))
+        return mp.value
+
     def tracers_step_host(self, p, t_now, dt):
         """RK2 tracers on host arrays (ssx_tracers_step_host); p as
         particles_step_host, tau optional"""

@@ -16,6 +16,7 @@ OPT_COARSE_ASSEMBLED = 6
 OPT_GATHER_DSS = 7
 AX_NVARIANTS = 8
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
+NOALLOC_MAX_PARTICLES = 131072
 (DATA_GFAC, DATA_MASS, DATA_DIAG_P, DATA_DIAG_H, DATA_SCALE, DATA_MASK,
  DATA_INVMASS, DATA_X, DATA_Y, DATA_Z) = range(10)
 
@@ -40,6 +41,7 @@ EXPORTED_SYMBOLS = [
     "ssx_autotune", "ssx_ax_variant",
     "ssx_set_ax_variant", "ssx_batch_create", "ssx_batch_destroy",
     "ssx_batch_set_stream", "ssx_batch_cg",
+    "ssx_particles_step_host_noalloc",
 ]
 
 
@@ -163,6 +165,8 @@ def load():
                                  C.POINTER(C.c_int)]),
         "ssx_tracers_step_host": (i, [vp, C.POINTER(ParticlesStruct), d, d,
                                       C.POINTER(C.c_int)]),
+        "ssx_particles_step_host_noalloc": (
+            i, [vp, C.POINTER(ParticlesStruct), d, d, C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
