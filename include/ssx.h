@@ -313,7 +313,12 @@ int ssx_particles_step_host_noalloc(ssx_ctx *ctx, const ssx_particles *p,
  * stream).  ssx_batch_cg: f[i], u[i] device E-vectors of problem i (f
  * masked in place, u initial guess in / solution out), stopping test
  * sqrt(<r,r>_w) <= eps[i] or max_iter; stats (nullable) gets `count`
- * entries.  Returns SSX_ENOTCONV if any problem stopped above its eps. */
+ * entries.  Returns SSX_ENOTCONV if any problem stopped above its eps.
+ * The batch stream first waits (by an event) for the work already enqueued
+ * on every context's stream, so f and u may still be in flight there; the
+ * call returns after the batch stream has finished.  The contexts' Jacobi
+ * diagonal choice, gather-form option and operator launch shape are read
+ * again at every call. */
 typedef struct ssx_batch ssx_batch;
 int ssx_batch_create(ssx_batch **batch, ssx_ctx *const *ctx, const int *op,
 		const int *var, int count);
@@ -321,6 +326,31 @@ void ssx_batch_destroy(ssx_batch *batch);
 int ssx_batch_set_stream(ssx_batch *batch, void *hip_stream);
 int ssx_batch_cg(ssx_batch *batch, double *const *f, double *const *u,
 		const double *eps, int max_iter, int use_graph, ssx_stats *stats);
+/* Every problem of the batch with the reference's solver for it, all
+ * advancing together: precon[i] = SSX_PC_MG solves problem i with
+ * flexible_conjugate_gradient (solver.c:83-157) preconditioned by the
+ * p-multigrid V-cycle (precon.c:227-271) -- the pressure solve of
+ * navier_stokes.c:157, or an ensemble member; SSX_PC_JACOBI with
+ * conjugate_gradient (solver.c:33-81) -- the velocity solves of
+ * navier_stokes.c:219.  The multigrid problems take one FCG iteration at a
+ * time in lock-step; in each V-cycle the smoother CG of every level (15
+ * iterations, early exit at eps) and the coarsest-level CG (500, to eps)
+ * of all problems run as batched passes, one launch per CG stage for the
+ * whole batch (the operator: one per order, operator and precision), and
+ * the Jacobi problems iterate along in every pass until they converge.
+ * A multigrid problem uses its context's level vectors and FCG history and
+ * its smoother precision (SSX_OPT_SMOOTHER_PRECISION, with the coarsest
+ * level in FP64 stopping at max(eps, 10^-k |f|_w) as in ssx_fcg), so a
+ * context may carry at most one multigrid problem of a batch (any number
+ * of Jacobi ones) and must not be used by other calls meanwhile.  f, u,
+ * eps, max_iter and the return codes are those of ssx_batch_cg (max_iter
+ * bounds the FCG iterations of a multigrid problem and the CG iterations
+ * of a Jacobi one); stats get the FCG figures of ssx_fcg.  With no
+ * multigrid problem this is ssx_batch_cg.  The multigrid passes are
+ * launched eagerly (use_graph applies to an all-Jacobi batch). */
+int ssx_batch_solve(ssx_batch *batch, const int *precon, double *const *f,
+		double *const *u, const double *eps, int max_iter, int use_graph,
+		ssx_stats *stats);
 
 /* ---- operator launch autotuning ----
  * The element operator has SSX_AX_NVARIANTS launch shapes (elements per

@@ -909,6 +909,56 @@ static int cg_level_plain(ssx_ctx *c, int l, int op, int var, double *f,
 	return SSX_OK;
 }
 
+/* The start of conjugate_gradient (solver.c:44-56) at level l in FP64
+ * storage, on the level's r, z, p, ap: f masked, r = f - A u (r = f when
+ * x_zero: the guess is known to be zero), the null space removed, z =
+ * D^-1 r, p = z, and <r,z>_w into scal[o + CG_RDZ] through the partials
+ * `part` (SSX_NPART doubles).  rtol > 0: also <r,r>_w into scal[o + CG_RR]
+ * and the stopping tolerance max(eps, rtol |r|_w) into scal[o + CG_TOL]. */
+int ssx_cg_start(ssx_ctx *c, int l, int op, int var, double *f,
+		const double *u, int x_zero, double *part, double *scal, int o,
+		double eps, double rtol)
+{
+	ssx_level *L = &c->lev[l];
+	const size_t nt = L->ntot;
+	double *r = L->r, *z = L->z, *p = L->p, *ap = L->ap;
+	void *st = c->stream;
+	const int pres = var == SSX_PRESSURE;
+	const double *dinv = jdiag(c, l, var);
+	TRY(ssx_mask(c, l, var, f));
+	if (!x_zero && !pres && c->gather_upd && L->gP > 0) {
+		/* r = f - A u in one pass over the unassembled A u (bit-identical
+		 * to the DSS launch and axpby below).  z, p and <r,z> stay
+		 * separate here: the fully fused start (used by the FP32
+		 * smoother, where it is bit-identical) moved the FP64 32^3
+		 * solution by ~1e-16 after a few outer iterations */
+		TRY(opeval_raw(c, l, op, var, u));
+		TRY(ssxk_resid_g(st, 0, nt, L->n3, L->gP, L->gsoff, L->gtab, ap, f,
+				NULL, r, NULL, NULL, NULL, NULL));
+		TRY(ssxk_mul(st, nt, r, dinv, z));
+		TRY(ssxk_copy(st, p, z, nt));
+	} else {
+		if (x_zero) {
+			TRY(ssxk_copy(st, r, f, nt));
+		} else {
+			TRY(ssx_opeval(c, l, op, var, u, r));
+			TRY(ssxk_axpby(st, nt, 1.0, f, NULL, -1.0, r));
+		}
+		TRY(ssx_nullspace(c, l, var, r));
+		TRY(ssxk_mul(st, nt, r, dinv, z));
+		TRY(ssx_nullspace(c, l, var, z));
+		TRY(ssxk_copy(st, p, z, nt));
+	}
+	TRY(ssxk_dot(st, nt, r, L->scale, z, part, 0));
+	TRY(ssxk_reduce(st, part, 0, scal, o + CG_RDZ));
+	if (rtol > 0.0) {       /* stop at max(eps, rtol * |r0|_w) */
+		TRY(ssxk_dot(st, nt, r, L->scale, r, part, 0));
+		TRY(ssxk_reduce(st, part, 0, scal, o + CG_RR));
+		TRY(ssxk_cg_tol(st, scal, o + CG_TOL, o + CG_RR, eps, rtol));
+	}
+	return SSX_OK;
+}
+
 /*
  * The same iteration fused: Ax (from the second iteration on with the
  * direction update p = (z - c) + beta p of the previous one) and the sum of
@@ -934,39 +984,10 @@ static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
 	const double *dinv = jdiag(c, l, var);
 	const int o = SC_CG(l);
 	int *flags = c->cg_flags + 2 * l;
-	TRY(ssx_mask(c, l, var, f));
-	if (!x_zero && !pres && c->gather_upd && L->gP > 0) {
-		/* r = f - A u in one pass over the unassembled A u (bit-identical
-		 * to the DSS launch and axpby below).  z, p and <r,z> stay
-		 * separate here: the fully fused start (used by the FP32
-		 * smoother, where it is bit-identical) moved the FP64 32^3
-		 * solution by ~1e-16 after a few outer iterations */
-		TRY(opeval_raw(c, l, op, var, u));
-		TRY(ssxk_resid_g(st, 0, nt, L->n3, L->gP, L->gsoff, L->gtab, ap, f,
-				NULL, r, NULL, NULL, NULL, NULL));
-		TRY(ssxk_mul(st, nt, r, dinv, z));
-		TRY(ssxk_copy(st, p, z, nt));
-		TRY(dot_to(c, l, r, z, o + CG_RDZ));
-	} else {
-		if (x_zero) {
-			TRY(ssxk_copy(st, r, f, nt));
-		} else {
-			TRY(ssx_opeval(c, l, op, var, u, r));
-			TRY(ssxk_axpby(st, nt, 1.0, f, NULL, -1.0, r));
-		}
-		TRY(ssx_nullspace(c, l, var, r));
-		TRY(ssxk_mul(st, nt, r, dinv, z));
-		TRY(ssx_nullspace(c, l, var, z));
-		TRY(ssxk_copy(st, p, z, nt));
-		TRY(dot_to(c, l, r, z, o + CG_RDZ));
-	}
+	TRY(ssx_cg_start(c, l, op, var, f, u, x_zero, c->part, c->scal, o, eps,
+			rtol));
 	TRY(ssxk_set_ints(st, flags, 0, 2));
-	double eps_dev = eps;
-	if (rtol > 0.0) {       /* stop at max(eps, rtol * |r0|_w) */
-		TRY(dot_to(c, l, r, r, o + CG_RR));
-		TRY(ssxk_cg_tol(st, c->scal, o + CG_TOL, o + CG_RR, eps, rtol));
-		eps_dev = -1.0;
-	}
+	const double eps_dev = rtol > 0.0 ? -1.0 : eps;
 	if (c->persist && L->n <= 4) {
 		int grid = c->persist_grid;
 		if (grid <= 0) {
@@ -1240,6 +1261,42 @@ static int nullspace_lp(ssx_ctx *c, int l, int var, float *u)
 			L->nunique, u);
 }
 
+/* The start of conjugate_gradient (solver.c:44-56) at level l in FP32
+ * storage (the reduced-precision smoother), on the level's FP32 r, z, p,
+ * ap: f masked, r = f - A u, the null space removed, z = D^-1 r, p = z,
+ * and <r,z>_w (accumulated in FP64) into scal[o + CG_RDZ] through the
+ * partials `part` (SSX_NPART doubles). */
+int ssx_cg_start_lp(ssx_ctx *c, int l, int op, int var, float *f,
+		const float *u, double *part, double *scal, int o)
+{
+	ssx_level *L = &c->lev[l];
+	const size_t nt = L->ntot;
+	float *r = L->r_lp, *z = L->z_lp, *p = L->p_lp, *ap = L->ap_lp;
+	void *st = c->stream;
+	const int pres = var == SSX_PRESSURE;
+	const float *mask = pres ? NULL : L->mask_lp;
+	const float *dinv = jdiag_lp(c, l, var);
+	if (!pres && c->gather_upd && L->gP > 0) {
+		/* f = f mask, r = f - A u, z = D^-1 r, p = z and <r,z>_w in one
+		 * pass over the unassembled A u (bit-identical to the sequence
+		 * below) */
+		TRY(opeval_raw_lp(c, l, op, var, u));
+		TRY(ssxk_resid_g_lp(st, 1, nt, L->n3, L->gP, L->gsoff, L->gtab, ap,
+				f, mask, r, z, p, dinv, part));
+	} else {
+		if (mask)
+			TRY(ssxk_mul_lp(st, nt, f, mask, f));
+		TRY(opeval_lp(c, l, op, var, u, r));
+		TRY(ssxk_axpby_lp(st, nt, 1.0, f, NULL, -1.0, r));
+		TRY(nullspace_lp(c, l, var, r));
+		TRY(ssxk_mul_lp(st, nt, r, dinv, z));
+		TRY(nullspace_lp(c, l, var, z));
+		TRY(ssxk_copy_lp(st, p, z, nt));
+		TRY(ssxk_dot_lp(st, nt, r, L->scale_lp, z, part, 0));
+	}
+	return ssxk_reduce(st, part, 0, scal, o + CG_RDZ);
+}
+
 /* conjugate_gradient (solver.c:33-81) in FP32 storage: the fused six-kernel
  * iteration of cg_level_fused with FP64 scalars and inner products */
 static int cg_level_lp(ssx_ctx *c, int l, int op, int var, float *f,
@@ -1250,29 +1307,10 @@ static int cg_level_lp(ssx_ctx *c, int l, int op, int var, float *f,
 	float *r = L->r_lp, *z = L->z_lp, *p = L->p_lp, *ap = L->ap_lp;
 	void *st = c->stream;
 	const int pres = var == SSX_PRESSURE;
-	const float *mask = pres ? NULL : L->mask_lp;
 	const float *dinv = jdiag_lp(c, l, var);
 	const int o = SC_CG(l);
 	int *flags = c->cg_flags + 2 * l;
-	if (!pres && c->gather_upd && L->gP > 0) {
-		/* f = f mask, r = f - A u, z = D^-1 r, p = z and <r,z>_w in one
-		 * pass over the unassembled A u (bit-identical to the sequence
-		 * below) */
-		TRY(opeval_raw_lp(c, l, op, var, u));
-		TRY(ssxk_resid_g_lp(st, 1, nt, L->n3, L->gP, L->gsoff, L->gtab, ap,
-				f, mask, r, z, p, dinv, c->part));
-	} else {
-		if (mask)
-			TRY(ssxk_mul_lp(st, nt, f, mask, f));
-		TRY(opeval_lp(c, l, op, var, u, r));
-		TRY(ssxk_axpby_lp(st, nt, 1.0, f, NULL, -1.0, r));
-		TRY(nullspace_lp(c, l, var, r));
-		TRY(ssxk_mul_lp(st, nt, r, dinv, z));
-		TRY(nullspace_lp(c, l, var, z));
-		TRY(ssxk_copy_lp(st, p, z, nt));
-		TRY(ssxk_dot_lp(st, nt, r, L->scale_lp, z, c->part, 0));
-	}
-	TRY(ssxk_reduce(st, c->part, 0, c->scal, o + CG_RDZ));
+	TRY(ssx_cg_start_lp(c, l, op, var, f, u, c->part, c->scal, o));
 	TRY(ssxk_set_ints(st, flags, 0, 2));
 	const long long nba = c->nel;   /* operator partials: one per element */
 	/* The solution step stays in the update here: taking it in the next
@@ -1418,11 +1456,132 @@ static int mg_precon_lp(ssx_ctx *c, int op, int var, double eps,
 	return ssxk_dss_all(st, L->grp, L->ngrp, px, NULL, NULL, 1, NULL);
 }
 
+/* ------------------------------------------------------------------ */
+/* the V-cycle in stages, for batches that run the smoother and coarse  */
+/* CG of many problems in shared launches: the same kernels as          */
+/* mg_recurse / mg_recurse_lp, cut where those call the level CG        */
+/* ------------------------------------------------------------------ */
+
+int ssx_lp_ready(ssx_ctx *c, int prec)
+{
+	return build_lp(c, prec);
+}
+
+const double *ssx_jdiag(ssx_ctx *c, int l, int var)
+{
+	return jdiag(c, l, var);
+}
+
+const float *ssx_jdiag_lp(ssx_ctx *c, int l, int var)
+{
+	return jdiag_lp(c, l, var);
+}
+
+/* initial guess of the cycle (precon.c:229-235): px = D^-1 x; reduced
+ * precision: FP32 right-hand side and guess on level 0 */
+int ssx_vc_begin(ssx_ctx *c, int prec, int var, const double *x, double *px)
+{
+	ssx_level *L = &c->lev[0];
+	c->cnt[3]++;
+	if (prec == SSX_PREC_FP64)
+		return ssx_jacobi(c, 0, var, x, px);
+	return ssxk_d2f_jacobi(c->stream, L->ntot, x, L->fn_lp,
+			jdiag_lp(c, 0, var), L->e_lp);
+}
+
+/* descent from level l (not the coarsest) to l + 1: the residual (level 0)
+ * or the level's right-hand side, restricted and DSS-averaged, and a zero
+ * initial error on l + 1 (precon.c:246-260) */
+int ssx_vc_down(ssx_ctx *c, int prec, int l, int op, int var,
+		const double *x, const double *px)
+{
+	ssx_level *L = &c->lev[l], *C = &c->lev[l + 1];
+	void *st = c->stream;
+	if (prec == SSX_PREC_FP64) {
+		double *r = L->mg_r;
+		if (l == 0 && c->gather_upd && L->gP > 0) {   /* precon.c:302-309 */
+			TRY(opeval_raw(c, l, op, var, px));
+			TRY(ssxk_resid_g(st, 0, L->ntot, L->n3, L->gP, L->gsoff,
+					L->gtab, L->ap, (double *) x, NULL, r, NULL, NULL,
+					NULL, NULL));
+		} else if (l == 0) {
+			TRY(ssx_opeval(c, l, op, var, px, r));
+			TRY(ssxk_axpby(st, L->ntot, 1.0, x, NULL, -1.0, r));
+		} else {
+			TRY(ssxk_copy(st, r, L->mg_fn, L->ntot));
+		}
+		TRY(ssx_restrict(c, l + 1, r, C->mg_fn));
+		return ssxk_fill(st, C->mg_e, 0.0, C->ntot);
+	}
+	float *r = L->mgr_lp;
+	if (l == 0 && c->gather_upd && L->gP > 0) {
+		TRY(opeval_raw_lp(c, l, op, var, L->e_lp));
+		TRY(ssxk_resid_g_lp(st, 0, L->ntot, L->n3, L->gP, L->gsoff, L->gtab,
+				L->ap_lp, L->fn_lp, NULL, r, NULL, NULL, NULL, NULL));
+	} else if (l == 0) {
+		TRY(opeval_lp(c, l, op, var, L->e_lp, r));
+		TRY(ssxk_axpby_lp(st, L->ntot, 1.0, L->fn_lp, NULL, -1.0, r));
+	} else {
+		TRY(ssxk_copy_lp(st, r, L->fn_lp, L->ntot));
+	}
+	TRY(ssxk_restrict_lp(st, L->n, C->n, c->nel, L->prol_lp, r, C->fn_lp));
+	if (l + 1 == c->nlev - 1) {
+		/* the coarsest right-hand side in FP64, averaged in FP64 (as
+		 * mg_recurse_lp) */
+		TRY(ssxk_cvt(st, SSX_PREC_FP32, SSX_PREC_FP64, C->fn_lp, C->mg_fn,
+				C->ntot));
+		TRY(ssxk_dss_all(st, C->grp, C->ngrp, C->mg_fn, NULL, NULL, 1,
+				NULL));
+		return ssxk_fill(st, C->mg_e, 0.0, C->ntot);
+	}
+	TRY(ssxk_dss_all_lp(st, C->grp, C->ngrp, C->fn_lp, NULL, NULL, 1, NULL));
+	return ssxk_fill_lp(st, C->e_lp, 0.0, C->ntot);
+}
+
+/* after the coarsest solve (reduced precision): its FP64 correction in
+ * FP32 for the level above */
+int ssx_vc_coarse_done(ssx_ctx *c, int prec)
+{
+	ssx_level *L = &c->lev[c->nlev - 1];
+	if (prec == SSX_PREC_FP64 || c->nlev < 2)
+		return SSX_OK;
+	return ssxk_cvt(c->stream, SSX_PREC_FP64, SSX_PREC_FP32, L->mg_e,
+			L->e_lp, L->ntot);
+}
+
+/* ascent to level l: the prolonged correction of l + 1 added to the
+ * level's solution (precon.c:262-263) */
+int ssx_vc_up(ssx_ctx *c, int prec, int l, double *px)
+{
+	ssx_level *L = &c->lev[l], *C = &c->lev[l + 1];
+	if (prec == SSX_PREC_FP64)
+		return ssxk_prolong(c->stream, L->n, C->n, c->nel, L->prol, C->mg_e,
+				l == 0 ? px : L->mg_e, 1);
+	return ssxk_prolong_lp(c->stream, L->n, C->n, c->nel, L->prol_lp,
+			C->e_lp, L->e_lp, 1);
+}
+
+/* end of the cycle (reduced precision): the FP32 result averaged in FP64
+ * into px, an assembled vector (as mg_precon_lp) */
+int ssx_vc_end(ssx_ctx *c, int prec, double *px)
+{
+	ssx_level *L = &c->lev[0];
+	void *st = c->stream;
+	if (prec == SSX_PREC_FP64)
+		return SSX_OK;
+	if (c->gather_upd && L->gP > 0)
+		return ssxk_f2d_average_g(st, L->ntot, L->n3, L->gP, L->gsoff,
+				L->gtab, L->e_lp, px);
+	TRY(ssxk_cvt(st, SSX_PREC_FP32, SSX_PREC_FP64, L->e_lp, px, L->ntot));
+	return ssxk_dss_all(st, L->grp, L->ngrp, px, NULL, NULL, 1, NULL);
+}
+
 int ssx_mg_precon(ssx_ctx *c, int op, int var, double eps, const double *x,
 		double *px)
 {
 	TRY(chk(c, 0));
-	if (c->smooth_prec != SSX_PREC_FP64)
+	/* a single level is the coarsest, which is FP64 in every mode */
+	if (c->smooth_prec != SSX_PREC_FP64 && c->nlev > 1)
 		return mg_precon_lp(c, op, var, eps, x, px);
 	c->cnt[3]++;
 	TRY(ssx_jacobi(c, 0, var, x, px));      /* precon.c:230 */
@@ -1461,84 +1620,129 @@ int ssx_cg(ssx_ctx *c, int l, int op, int var, double *f, double *u,
 	return rc;
 }
 
-int ssx_fcg(ssx_ctx *c, int op, int var, int precon, double *f, double *u,
-		double eps, int max_iter, ssx_stats *stats)
+/* flexible_conjugate_gradient (solver.c:83-157) in three phases, so a
+ * batch can interleave many problems: begin (history buffers, f masked,
+ * r = f - A u, null space), then per iteration the preconditioner into
+ * fcg_z followed by ssx_fcg_iter (orthogonalisation, direction, A p, step
+ * and <r,r>_w into scal[SC_RR]; enqueued only), and end (the rotated
+ * history handed back to the context). */
+int ssx_fcg_begin(ssx_ctx *c, int op, int var, int precon, double *f,
+		const double *u, ssx_fcg_state *s)
 {
 	TRY(chk(c, 0));
 	const int mmax = precon == SSX_PC_MG ? 10 : 1;   /* solver.c:90-92 */
-	ssx_level *L = &c->lev[0];
-	const size_t nt = L->ntot;
-	void *st = c->stream;
+	const size_t nt = c->lev[0].ntot;
+	s->mmax = 0;             /* set once the history is complete */
 	for (int i = 0; i < mmax + 1; ++i)
 		if (!c->fcg_p[i])
 			TRY(dalloc(&c->fcg_p[i], nt));
 	for (int i = 0; i < mmax; ++i)
 		if (!c->fcg_ap[i])
 			TRY(dalloc(&c->fcg_ap[i], nt));
-	double *p[11], *ap[10];
 	for (int i = 0; i < 11; ++i)
-		p[i] = c->fcg_p[i];
+		s->p[i] = c->fcg_p[i];
 	for (int i = 0; i < 10; ++i)
-		ap[i] = c->fcg_ap[i];
+		s->ap[i] = c->fcg_ap[i];
+	s->mmax = mmax;
+	s->j = 0;
+	TRY(ssx_mask(c, 0, var, f));
+	TRY(ssx_opeval(c, 0, op, var, u, c->fcg_r));
+	TRY(ssxk_axpby(c->stream, nt, 1.0, f, NULL, -1.0, c->fcg_r));
+	return ssx_nullspace(c, 0, var, c->fcg_r);
+}
+
+int ssx_fcg_iter(ssx_ctx *c, int op, int var, double *u, ssx_fcg_state *s)
+{
+	ssx_level *L = &c->lev[0];
+	const size_t nt = L->ntot;
+	void *st = c->stream;
+	double **p = s->p, **ap = s->ap;
 	double *r = c->fcg_r, *z = c->fcg_z;
 	const double *mask = var == SSX_PRESSURE ? NULL : L->mask;
+	const long j = s->j;
+	const int mmax = s->mmax;
+	int m = (int) j, mm = (int) (j % (mmax + 1));
+	if (mm < 1)
+		mm = 1;
+	if (mm < m)
+		m = mm;
+	TRY(ssx_nullspace(c, 0, var, z));
+	/* beta_k = <z, Ap_k>_w / <p_k, Ap_k>_w, k < m, in one pass */
+	TRY(ssxk_multidot(st, nt, z, L->scale, (const double *const *) ap,
+			m, c->part, c->scal, SC_ZAP0, SC_PAP0, SC_BETA0));
+	double *tp = p[m];                 /* p[k] -> p[k+1], k < m */
+	for (int k = m; k > 0; --k)
+		p[k] = p[k - 1];
+	p[0] = tp;
+	/* p_0 = mask (z - sum_k beta_k p_{k+1}), one pass */
+	TRY(ssxk_multi_axpy(st, nt, z, (const double *const *) (p + 1), m,
+			c->scal, SC_BETA0, mask, p[0]));
+	int m2 = (int) (j + 1), mm2 = (int) ((j + 1) % (mmax + 1));
+	if (mm2 < 1)
+		mm2 = 1;
+	if (mm2 < m2)
+		m2 = mm2;
+	double *ta = ap[m2 - 1];           /* ap[k] -> ap[k+1], k < m2-1 */
+	for (int k = m2 - 1; k > 0; --k)
+		ap[k] = ap[k - 1];
+	ap[0] = ta;
+	TRY(ssx_opeval(c, 0, op, var, p[0], ap[0]));
+	for (int k = m2 - 1; k > 0; --k)
+		TRY(ssxk_scalar(st, c->scal, 2, SC_PAP0 + k, SC_PAP0 + k - 1, 0,
+				0.0));
+	/* <p_0, Ap_0>_w and <p_0, r>_w in one pass */
+	const double *pr2[2] = { ap[0], r };
+	TRY(ssxk_multidot(st, nt, p[0], L->scale, pr2, 2, c->part, c->scal,
+			SC_PAPN, -1, 0));
+	TRY(ssxk_scalar(st, c->scal, 2, SC_PAP0, SC_PAPN, 0, 0.0));
+	TRY(ssxk_scalar(st, c->scal, 0, SC_ALPHA, SC_PAPN + 1, SC_PAPN, 0.0));
+	/* u += alpha p_0; r -= alpha Ap_0; <r,r>_w */
+	TRY(ssxk_fcg_step(st, nt, c->scal, SC_ALPHA, p[0], ap[0], u, r,
+			L->scale, c->part));
+	TRY(ssxk_reduce(st, c->part, 0, c->scal, SC_RR));
+	s->j = j + 1;
+	return SSX_OK;
+}
+
+void ssx_fcg_end(ssx_ctx *c, const ssx_fcg_state *s)
+{
+	if (s->mmax == 0)        /* ssx_fcg_begin did not complete */
+		return;
+	/* keep the rotated history buffers owned by the context */
+	for (int i = 0; i < 11; ++i)
+		c->fcg_p[i] = s->p[i];
+	for (int i = 0; i < 10; ++i)
+		c->fcg_ap[i] = s->ap[i];
+}
+
+int ssx_fcg(ssx_ctx *c, int op, int var, int precon, double *f, double *u,
+		double eps, int max_iter, ssx_stats *stats)
+{
+	TRY(chk(c, 0));
+	ssx_fcg_state s;
+	s.mmax = 0;
 	long long c0[4];
 	memcpy(c0, c->cnt, sizeof(c0));
 	double t0 = now_s();
-	TRY(ssx_mask(c, 0, var, f));
-	TRY(ssx_opeval(c, 0, op, var, u, r));
-	TRY(ssxk_axpby(st, nt, 1.0, f, NULL, -1.0, r));
-	TRY(ssx_nullspace(c, 0, var, r));
-	long j = 0;
+	int rc = ssx_fcg_begin(c, op, var, precon, f, u, &s);
+	if (rc != SSX_OK) {
+		ssx_fcg_end(c, &s);
+		return rc;
+	}
 	int it = 0;
 	double err = 0.0;
-	int rc = SSX_OK;
 	do {
-		int m = (int) j, mm = (int) (j % (mmax + 1));
-		if (mm < 1)
-			mm = 1;
-		if (mm < m)
-			m = mm;
 		if (precon == SSX_PC_MG)
-			TRY(ssx_mg_precon(c, op, var, eps, r, z));
+			rc = ssx_mg_precon(c, op, var, eps, c->fcg_r, c->fcg_z);
 		else
-			TRY(ssx_jacobi(c, 0, var, r, z));
-		TRY(ssx_nullspace(c, 0, var, z));
-		/* beta_k = <z, Ap_k>_w / <p_k, Ap_k>_w, k < m, in one pass */
-		TRY(ssxk_multidot(st, nt, z, L->scale, (const double *const *) ap,
-				m, c->part, c->scal, SC_ZAP0, SC_PAP0, SC_BETA0));
-		double *tp = p[m];                 /* p[k] -> p[k+1], k < m */
-		for (int k = m; k > 0; --k)
-			p[k] = p[k - 1];
-		p[0] = tp;
-		/* p_0 = mask (z - sum_k beta_k p_{k+1}), one pass */
-		TRY(ssxk_multi_axpy(st, nt, z, (const double *const *) (p + 1), m,
-				c->scal, SC_BETA0, mask, p[0]));
-		int m2 = (int) (j + 1), mm2 = (int) ((j + 1) % (mmax + 1));
-		if (mm2 < 1)
-			mm2 = 1;
-		if (mm2 < m2)
-			m2 = mm2;
-		double *ta = ap[m2 - 1];           /* ap[k] -> ap[k+1], k < m2-1 */
-		for (int k = m2 - 1; k > 0; --k)
-			ap[k] = ap[k - 1];
-		ap[0] = ta;
-		TRY(ssx_opeval(c, 0, op, var, p[0], ap[0]));
-		for (int k = m2 - 1; k > 0; --k)
-			TRY(ssxk_scalar(st, c->scal, 2, SC_PAP0 + k, SC_PAP0 + k - 1, 0,
-					0.0));
-		/* <p_0, Ap_0>_w and <p_0, r>_w in one pass */
-		const double *pr2[2] = { ap[0], r };
-		TRY(ssxk_multidot(st, nt, p[0], L->scale, pr2, 2, c->part, c->scal,
-				SC_PAPN, -1, 0));
-		TRY(ssxk_scalar(st, c->scal, 2, SC_PAP0, SC_PAPN, 0, 0.0));
-		TRY(ssxk_scalar(st, c->scal, 0, SC_ALPHA, SC_PAPN + 1, SC_PAPN, 0.0));
-		/* u += alpha p_0; r -= alpha Ap_0; <r,r>_w */
-		TRY(ssxk_fcg_step(st, nt, c->scal, SC_ALPHA, p[0], ap[0], u, r,
-				L->scale, c->part));
-		TRY(ssxk_reduce(st, c->part, 0, c->scal, SC_RR));
-		double rr;
-		TRY(read_scal(c, SC_RR, &rr));
+			rc = ssx_jacobi(c, 0, var, c->fcg_r, c->fcg_z);
+		if (rc == SSX_OK)
+			rc = ssx_fcg_iter(c, op, var, u, &s);
+		double rr = 0.0;
+		if (rc == SSX_OK)
+			rc = read_scal(c, SC_RR, &rr);
+		if (rc != SSX_OK)
+			break;
 		err = sqrt(rr);
 		++it;
 		if (isnan(err) || isinf(err)) {
@@ -1546,12 +1750,8 @@ int ssx_fcg(ssx_ctx *c, int op, int var, int precon, double *f, double *u,
 			ssx_set_error("FCG: residual is %g", err);
 			break;
 		}
-	} while (err > eps && ++j < max_iter);
-	/* keep the rotated history buffers owned by the context */
-	for (int i = 0; i < 11; ++i)
-		c->fcg_p[i] = p[i];
-	for (int i = 0; i < 10; ++i)
-		c->fcg_ap[i] = ap[i];
+	} while (err > eps && s.j < max_iter);
+	ssx_fcg_end(c, &s);
 	if (rc == SSX_OK)
 		rc = ssx_synchronize(c);
 	if (stats) {
@@ -1682,8 +1882,13 @@ int ssx_solve_helmholtz_cg(ssx_ctx *c, double *f, double *u, int var,
  * (deriv.c:68-79 derivatives summed in index order, tripalg.c:165-222
  * determinant and cofactors), so that particle positions and owning
  * elements follow the reference bit for bit */
-static void particle_jacobians(const ssx_basis *b, int nel, const double *x,
-		const double *y, const double *z, double *jac[9], double *jinv[9])
+/* J (jac[3r+s] = D_s x_r), J^-1 and (det nullable) det J at every node of
+ * a basis, with the reference's cofactor formulas and operation order
+ * (compute_jacobian deriv.c:68-79, compute_trimat_determinant
+ * tripalg.c:165-182, invert_trimat :201-222) */
+void ssx_jacobians(const ssx_basis *b, int nel, const double *x,
+		const double *y, const double *z, double *const jac[9],
+		double *const jinv[9], double *det_out)
 {
 	const int n = b->n, n3 = n * n * n;
 	const double *D = b->d;
@@ -1730,6 +1935,8 @@ static void particle_jacobians(const ssx_basis *b, int nel, const double *x,
 					jac[r * 3 + s][off + q] = a[r][s];
 					jinv[r * 3 + s][off + q] = ai[r][s];
 				}
+			if (det_out)
+				det_out[off + q] = det;
 		}
 	}
 }
@@ -1749,7 +1956,7 @@ static int particles_init(ssx_ctx *c)
 			rc = SSX_ENOMEM;
 	}
 	if (rc == SSX_OK) {
-		particle_jacobians(&L->b, c->nel, L->h_x, L->h_y, L->h_z, hj, hi);
+		ssx_jacobians(&L->b, c->nel, L->h_x, L->h_y, L->h_z, hj, hi, NULL);
 		for (int a = 0; a < 9 && rc == SSX_OK; ++a) {
 			rc = up(&c->p_jac[a], hj[a], nt);
 			if (!rc)
@@ -1775,6 +1982,18 @@ static int particles_init(ssx_ctx *c)
 		}
 		HIPC(hipMemcpy(c->d_neigh, c->h_neigh, sizeof(int) * 6 * c->nel,
 				hipMemcpyHostToDevice));
+		/* the tracer walk enters a neighbour on the face opposite the one
+		 * it left: right when the neighbour across face s names this
+		 * element across face s^1 (faces W,E,S,N,B,T) */
+		c->p_oriented = 1;
+		for (int e = 0; e < c->nel && c->p_oriented; ++e)
+			for (int f = 0; f < 6; ++f) {
+				const int nb = c->h_neigh[6 * e + f];
+				if (nb >= 0 && c->h_neigh[6 * nb + (f ^ 1)] != e) {
+					c->p_oriented = 0;
+					break;
+				}
+			}
 	}
 	if (rc != SSX_OK)
 		ssx_set_error("particle set-up failed");
@@ -1825,6 +2044,12 @@ int ssx_walk_dev(ssx_ctx *c, const ssx_particles *p, double t_now,
 	} else {
 		TRY(particles_init(c));
 		counters = c->p_maxpass;
+	}
+	if (tracers && !c->p_oriented) {
+		ssx_set_error("tracer step: a neighbour across face s does not name "
+				"the element across the opposite face; the tracer walk "
+				"needs consistently oriented elements");
+		return SSX_EMESH;
 	}
 	const double *ur[9];
 	for (int a = 0; a < 9; ++a)

@@ -20,6 +20,7 @@
 
 #include "ssx_internal.h"
 #include "ssx_batch.h"
+#include "ssx_batch_host.h"
 
 #define HIPB(x)                                                              \
 	do {                                                                 \
@@ -36,38 +37,7 @@
 			return rc_;                                          \
 	} while (0)
 
-#define GRAPH_ITERS 4           /* iterations per captured graph */
-
-struct ssx_batch {
-	int np;
-	ssx_ctx **ctx;
-	int *op, *var;
-	void *stream, *own_stream;
-	/* per-problem work vectors and scratch (device) */
-	double **r, **z, **p, **ap;
-	double *part, *scal;
-	int *flags;
-	size_t *poff;              /* partial offsets per problem */
-	ssx_bprob *h_prob, *d_prob;
-	/* work tables: one operator table per (order, operator) group, rebuilt
-	 * for the problems still iterating whenever that set changes */
-	int norder, order_n[2 * (SSX_NMAX + 1)], order_op[2 * (SSX_NMAX + 1)];
-	int order_v[2 * (SSX_NMAX + 1)];
-	ssx_bwork *d_axtab[2 * (SSX_NMAX + 1)];
-	long long nax[2 * (SSX_NMAX + 1)];
-	ssx_bwork *d_dsstab, *d_updtab;
-	long long ndss, nupd;
-	ssx_bwork *htab;           /* host staging of the tables */
-	long long htab_n;
-	unsigned char *active;     /* problems in the current tables */
-	double *bpart;             /* set-up reductions [SSX_NPART] */
-	int *h_flags;
-	double *h_scal;
-	hipGraphExec_t gexec;
-	int have_graph;
-};
-
-static int balloc(void **p, size_t bytes)
+int ssx_balloc(void **p, size_t bytes)
 {
 	if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) {
 		*p = NULL;
@@ -91,6 +61,9 @@ void ssx_batch_destroy(ssx_batch *b)
 		return;
 	if (b->have_graph)
 		hipGraphExecDestroy(b->gexec);
+	if (b->have_event)
+		hipEventDestroy(b->ready);
+	ssx_bfcg_free(b);
 	for (int i = 0; i < b->np; ++i) {
 		if (b->r) hipFree(b->r[i]);
 		if (b->z) hipFree(b->z[i]);
@@ -163,17 +136,17 @@ static int build(ssx_batch *b)
 		d->nbu = (d->ntot + SSX_BATCH_CHUNK - 1) / SSX_BATCH_CHUNK;
 		b->poff[i + 1] = b->poff[i] + (size_t) (d->nel + d->nbd
 				+ 4 * d->nbu);
-		TRYB(balloc((void **) &b->r[i], sizeof(double) * L->ntot));
-		TRYB(balloc((void **) &b->z[i], sizeof(double) * L->ntot));
-		TRYB(balloc((void **) &b->p[i], sizeof(double) * L->ntot));
-		TRYB(balloc((void **) &b->ap[i], sizeof(double) * L->ntot));
+		TRYB(ssx_balloc((void **) &b->r[i], sizeof(double) * L->ntot));
+		TRYB(ssx_balloc((void **) &b->z[i], sizeof(double) * L->ntot));
+		TRYB(ssx_balloc((void **) &b->p[i], sizeof(double) * L->ntot));
+		TRYB(ssx_balloc((void **) &b->ap[i], sizeof(double) * L->ntot));
 		d->r = b->r[i]; d->z = b->z[i]; d->p = b->p[i]; d->ap = b->ap[i];
 	}
-	TRYB(balloc((void **) &b->part, sizeof(double) * b->poff[np]));
-	TRYB(balloc((void **) &b->scal, sizeof(double) * CG_NSLOT * np));
-	TRYB(balloc((void **) &b->flags, sizeof(int) * 2 * np));
-	TRYB(balloc((void **) &b->d_prob, sizeof(ssx_bprob) * np));
-	TRYB(balloc((void **) &b->bpart, sizeof(double) * SSX_NPART));
+	TRYB(ssx_balloc((void **) &b->part, sizeof(double) * b->poff[np]));
+	TRYB(ssx_balloc((void **) &b->scal, sizeof(double) * CG_NSLOT * np));
+	TRYB(ssx_balloc((void **) &b->flags, sizeof(int) * 2 * np));
+	TRYB(ssx_balloc((void **) &b->d_prob, sizeof(ssx_bprob) * np));
+	TRYB(ssx_balloc((void **) &b->bpart, sizeof(double) * SSX_NPART));
 	HIPB(hipHostMalloc((void **) &b->h_flags, sizeof(int) * 2 * np, 0));
 	HIPB(hipHostMalloc((void **) &b->h_scal, sizeof(double) * CG_NSLOT * np,
 			0));
@@ -208,7 +181,7 @@ static int build(ssx_batch *b)
 		b->poff[i + 1] = b->poff[i] + (size_t) (d->nel + d->nbd + 4 * d->nbu);
 	}
 	hipFree(b->part);
-	rc = balloc((void **) &b->part, sizeof(double) * b->poff[np]);
+	rc = ssx_balloc((void **) &b->part, sizeof(double) * b->poff[np]);
 	for (int i = 0; i < np && rc == SSX_OK; ++i)
 		b->h_prob[i].part = b->part + b->poff[i];
 	/* device tables at full size; host staging for rebuilds */
@@ -219,7 +192,7 @@ static int build(ssx_batch *b)
 			if (b->h_prob[i].n == b->order_n[k]
 					&& b->h_prob[i].op == b->order_op[k])
 				m += b->h_prob[i].nba;
-		rc = balloc((void **) &b->d_axtab[k], sizeof(ssx_bwork) * m);
+		rc = ssx_balloc((void **) &b->d_axtab[k], sizeof(ssx_bwork) * m);
 		tot += m;
 	}
 	long long md = 0, mu = 0;
@@ -228,9 +201,9 @@ static int build(ssx_batch *b)
 		mu += b->h_prob[i].nbu;
 	}
 	if (rc == SSX_OK)
-		rc = balloc((void **) &b->d_dsstab, sizeof(ssx_bwork) * md);
+		rc = ssx_balloc((void **) &b->d_dsstab, sizeof(ssx_bwork) * md);
 	if (rc == SSX_OK)
-		rc = balloc((void **) &b->d_updtab, sizeof(ssx_bwork) * mu);
+		rc = ssx_balloc((void **) &b->d_updtab, sizeof(ssx_bwork) * mu);
 	tot += md + mu;
 	b->htab = malloc(sizeof(ssx_bwork) * (size_t) (tot ? tot : 1));
 	b->htab_n = tot;
@@ -347,7 +320,16 @@ int ssx_batch_create(ssx_batch **out, ssx_ctx *const *ctx, const int *op,
 		return SSX_EHIP;
 	}
 	b->own_stream = b->stream = s;
+	if (hipEventCreateWithFlags(&b->ready, hipEventDisableTiming)
+			!= hipSuccess) {
+		ssx_set_error("hipEventCreate (batch)");
+		ssx_batch_destroy(b);
+		return SSX_EHIP;
+	}
+	b->have_event = 1;
 	int rc = build(b);
+	if (rc == SSX_OK)
+		rc = ssx_bfcg_init(b);
 	if (rc != SSX_OK) {
 		ssx_batch_destroy(b);
 		return rc;
@@ -409,7 +391,7 @@ static int capture(ssx_batch *b)
 
 /* per-problem set-up: mask f, r = f - A u, null space, z = D^-1 r,
  * p = z, <r,z>_w (solver.c:46-56) */
-static int setup_problem(ssx_batch *b, int i, double *f, double *u)
+int ssx_batch_setup_problem(ssx_batch *b, int i, double *f, double *u)
 {
 	const ssx_ctx *c = b->ctx[i];
 	const ssx_level *L = &c->lev[0];
@@ -440,6 +422,93 @@ static int setup_problem(ssx_batch *b, int i, double *f, double *u)
 	return ssxk_reduce(st, b->bpart, 0, sc, CG_RDZ);
 }
 
+/* Settings the batch takes from its contexts (Jacobi diagonal choice,
+ * gather-form DSS, operator launch shape) are re-read at every solve, so a
+ * batch follows ssx_set_diag_choice, SSX_OPT_GATHER_DSS and
+ * ssx_set_ax_variant made after ssx_batch_create.  A changed launch shape
+ * or DSS form rebuilds the work tables (and the captured graph). */
+int ssx_batch_sync_settings(ssx_batch *b)
+{
+	int rebuild = 0;
+	for (int i = 0; i < b->np; ++i) {
+		const ssx_ctx *c = b->ctx[i];
+		ssx_bprob *d = &b->h_prob[i];
+		d->dinv = jdiag0(c, b->var[i]);
+		const int gP = c->gather_upd ? c->lev[0].gP : 0;
+		if (gP != d->gP) {
+			d->gP = gP;
+			rebuild = 1;
+		}
+	}
+	for (int k = 0; k < b->norder; ++k) {
+		int first = -1;
+		for (int i = 0; i < b->np && first < 0; ++i)
+			if (b->h_prob[i].n == b->order_n[k]
+					&& b->h_prob[i].op == b->order_op[k])
+				first = i;
+		const int v = b->ctx[first]->lev[0].axv[SSX_PREC_FP64];
+		if (v == b->order_v[k])
+			continue;
+		const int t_old = ssxk_bax_teams(b->order_n[k], b->order_v[k]);
+		const int t_new = ssxk_bax_teams(b->order_n[k], v);
+		b->order_v[k] = v;
+		rebuild = 1;
+		for (int i = 0; i < b->np; ++i)
+			if (b->h_prob[i].n == b->order_n[k]
+					&& b->h_prob[i].op == b->order_op[k])
+				b->h_prob[i].nba = (b->h_prob[i].nel + t_new - 1) / t_new;
+		if (t_new < t_old) {    /* more operator workgroups: larger table */
+			long long m = 0;
+			for (int i = 0; i < b->np; ++i)
+				if (b->h_prob[i].n == b->order_n[k]
+						&& b->h_prob[i].op == b->order_op[k])
+					m += b->h_prob[i].nba;
+			HIPB(hipStreamSynchronize((hipStream_t) b->stream));
+			hipFree(b->d_axtab[k]);
+			TRYB(ssx_balloc((void **) &b->d_axtab[k], sizeof(ssx_bwork) * m));
+			long long tot = m;
+			for (int q = 0; q < b->norder; ++q)
+				if (q != k)
+					for (int i = 0; i < b->np; ++i)
+						if (b->h_prob[i].n == b->order_n[q]
+								&& b->h_prob[i].op == b->order_op[q])
+							tot += b->h_prob[i].nba;
+			for (int i = 0; i < b->np; ++i)
+				tot += b->h_prob[i].nbd + b->h_prob[i].nbu;
+			if (tot > b->htab_n) {
+				ssx_bwork *t = realloc(b->htab, sizeof(ssx_bwork) * tot);
+				if (!t) {
+					ssx_set_error("out of host memory (batch)");
+					return SSX_ENOMEM;
+				}
+				b->htab = t;
+				b->htab_n = tot;
+			}
+		}
+	}
+	if (rebuild)                    /* the next solve refills the tables */
+		memset(b->active, 0, b->np);
+	return SSX_OK;
+}
+
+/* The batch stream waits for the work already enqueued on every context's
+ * stream (which may still be writing f and u) before it reads them. */
+int ssx_batch_order_after_contexts(ssx_batch *b)
+{
+	hipStream_t st = (hipStream_t) b->stream;
+	for (int i = 0; i < b->np; ++i) {
+		hipStream_t cs = (hipStream_t) b->ctx[i]->stream;
+		int seen = cs == st;
+		for (int q = 0; q < i && !seen; ++q)
+			seen = (hipStream_t) b->ctx[q]->stream == cs;
+		if (seen)
+			continue;
+		HIPB(hipEventRecord(b->ready, cs));
+		HIPB(hipStreamWaitEvent(st, b->ready, 0));
+	}
+	return SSX_OK;
+}
+
 int ssx_batch_cg(ssx_batch *b, double *const *f, double *const *u,
 		const double *eps, int max_iter, int use_graph, ssx_stats *stats)
 {
@@ -464,10 +533,12 @@ int ssx_batch_cg(ssx_batch *b, double *const *f, double *const *u,
 		b->h_prob[i].eps = eps[i];
 		b->h_prob[i].maxit = max_iter;
 	}
+	TRYB(ssx_batch_sync_settings(b));
+	TRYB(ssx_batch_order_after_contexts(b));
 	HIPB(hipMemcpyAsync(b->d_prob, b->h_prob, sizeof(ssx_bprob) * np,
 			hipMemcpyHostToDevice, st));
 	for (int i = 0; i < np; ++i)
-		TRYB(setup_problem(b, i, f[i], u[i]));
+		TRYB(ssx_batch_setup_problem(b, i, f[i], u[i]));
 	TRYB(ssxk_set_ints(st, b->flags, 0, 2 * np < 64 ? 2 * np : 64));
 	for (int q = 64; q < 2 * np; q += 64)
 		TRYB(ssxk_set_ints(st, b->flags + q, 0,

@@ -194,9 +194,10 @@ k_bupdate(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
 // null-space constant, rdz, beta, the iteration count and the stopping
 // test sqrt(<r,r>_w) <= eps or count >= maxit (or NaN), as k_cg_reduce
 __global__ void __launch_bounds__(1024)
-k_breduce(int mode, const ssx_bprob *__restrict__ pr)
+k_breduce(int mode, const ssx_bprob *__restrict__ pr,
+		const int *__restrict__ list)
 {
-	const ssx_bprob *d = pr + blockIdx.x;
+	const ssx_bprob *d = pr + (list ? list[blockIdx.x] : (int) blockIdx.x);
 	if (d->flags[0])
 		return;
 	__shared__ double sh[4][1024];
@@ -261,8 +262,45 @@ k_breduce(int mode, const ssx_bprob *__restrict__ pr)
 	s[CG_RR] = rr;
 	const int it = d->flags[1] + 1;
 	d->flags[1] = it;
-	if (sqrt(rr) <= d->eps || it >= d->maxit || !(rr == rr))
+	const double tol = d->eps >= 0.0 ? d->eps : s[CG_TOL];
+	if (sqrt(rr) <= tol || it >= d->maxit || !(rr == rr))
 		d->flags[0] = 1;
+}
+
+// start of a pass: flags cleared and the scalars that the first fused
+// direction update and solution step read set to zero
+__global__ void k_btask_reset(ssx_bprob *__restrict__ pr,
+		const int *__restrict__ list, int nt)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= nt)
+		return;
+	const ssx_bprob *d = pr + list[t];
+	d->flags[0] = 0;
+	d->flags[1] = 0;
+	d->scal[CG_ALPHA] = 0.0;
+	d->scal[CG_BETA] = 0.0;
+	d->scal[CG_NULLC] = 0.0;
+}
+
+// u += alpha p over one update chunk of an FP64 task (the step the update
+// leaves to the next operator launch); the arithmetic of that launch
+__global__ void __launch_bounds__(256)
+k_bflush(const ssx_bprob *__restrict__ pr, const ssx_bwork *__restrict__ tab)
+{
+	const ssx_bwork w = tab[blockIdx.x];
+	const ssx_bprob *d = pr + w.prob;
+	const double a = d->scal[CG_ALPHA];
+	const long long ntot = d->ntot;
+	const SSX_GLOBAL double *__restrict__ p =
+		(const SSX_GLOBAL double *) d->p;
+	SSX_GLOBAL double *__restrict__ u = (SSX_GLOBAL double *) d->u;
+	const long long q0 = (long long) w.blk * SSX_BATCH_CHUNK;
+	for (int t = 0; t < SSX_BATCH_CHUNK / 256; ++t) {
+		const long long q = q0 + t * 256 + threadIdx.x;
+		if (q < ntot)
+			u[q] = a * p[q] + u[q];
+	}
 }
 
 template <int N, int OP, bool PU, int V>
@@ -344,7 +382,40 @@ extern "C" int ssxk_breduce(void *stream, int mode, const ssx_bprob *pr,
 		int np)
 {
 	hipLaunchKernelGGL(k_breduce, dim3(np), dim3(1024), 0,
-			(hipStream_t) stream, mode, pr);
+			(hipStream_t) stream, mode, pr, (const int *) nullptr);
 	LAUNCH_OK("k_breduce");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_breduce_list(void *stream, int mode, const ssx_bprob *pr,
+		const int *list, int nt)
+{
+	if (nt <= 0)
+		return SSX_OK;
+	hipLaunchKernelGGL(k_breduce, dim3(nt), dim3(1024), 0,
+			(hipStream_t) stream, mode, pr, list);
+	LAUNCH_OK("k_breduce");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_btask_reset(void *stream, const ssx_bprob *pr,
+		const int *list, int nt)
+{
+	if (nt <= 0)
+		return SSX_OK;
+	hipLaunchKernelGGL(k_btask_reset, dim3((nt + 255) / 256), dim3(256), 0,
+			(hipStream_t) stream, (ssx_bprob *) pr, list, nt);
+	LAUNCH_OK("k_btask_reset");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_bflush(void *stream, const ssx_bprob *pr,
+		const ssx_bwork *tab, long long ntab)
+{
+	if (ntab <= 0)
+		return SSX_OK;
+	hipLaunchKernelGGL(k_bflush, dim3((unsigned) ntab), dim3(256), 0,
+			(hipStream_t) stream, pr, tab);
+	LAUNCH_OK("k_bflush");
 	return SSX_OK;
 }

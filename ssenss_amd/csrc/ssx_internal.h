@@ -133,6 +133,7 @@ struct ssx_ctx {
 	int *h_neigh, *d_neigh;
 	double *p_jac[9], *p_jinv[9], *p_ur[3][3], *p_xi;
 	int *p_maxpass;
+	int p_oriented;              /* every face pair's neighbours point back */
 	void *p_stage;               /* device staging of host particle lists */
 	size_t p_stage_bytes;
 	long long cnt[4];
@@ -400,6 +401,40 @@ int ssxk_axpby(void *stream, size_t n, double a, const double *x,
 int ssxk_mul(void *stream, size_t n, const double *x, const double *w,
 		double *z);
 /* per-block partials of sum(x*s*y) into part[slot*NPART + blk] */
+/* flexible_conjugate_gradient in phases (ssx_fcg and the batch) */
+typedef struct ssx_fcg_state {
+	double *p[11], *ap[10];      /* direction and A-direction history    */
+	long j;                      /* iterations taken                      */
+	int mmax;                    /* history length (10 with p-MG)         */
+} ssx_fcg_state;
+int ssx_fcg_begin(ssx_ctx *c, int op, int var, int precon, double *f,
+		const double *u, ssx_fcg_state *s);
+int ssx_fcg_iter(ssx_ctx *c, int op, int var, double *u, ssx_fcg_state *s);
+void ssx_fcg_end(ssx_ctx *c, const ssx_fcg_state *s);
+
+/* J, J^-1 and det J of a mesh at the nodes of basis b (ssx_api.c) */
+void ssx_jacobians(const ssx_basis *b, int nel, const double *x,
+		const double *y, const double *z, double *const jac[9],
+		double *const jinv[9], double *det_out);
+
+/* level CG starts (ssx_api.c), shared by the single-problem and batched
+ * solvers */
+int ssx_cg_start(ssx_ctx *c, int l, int op, int var, double *f,
+		const double *u, int x_zero, double *part, double *scal, int o,
+		double eps, double rtol);
+int ssx_cg_start_lp(ssx_ctx *c, int l, int op, int var, float *f,
+		const float *u, double *part, double *scal, int o);
+/* V-cycle stages (prec: the smoother precision SSX_PREC_*) */
+int ssx_lp_ready(ssx_ctx *c, int prec);
+const double *ssx_jdiag(ssx_ctx *c, int l, int var);
+const float *ssx_jdiag_lp(ssx_ctx *c, int l, int var);
+int ssx_vc_begin(ssx_ctx *c, int prec, int var, const double *x, double *px);
+int ssx_vc_down(ssx_ctx *c, int prec, int l, int op, int var,
+		const double *x, const double *px);
+int ssx_vc_coarse_done(ssx_ctx *c, int prec);
+int ssx_vc_up(ssx_ctx *c, int prec, int l, double *px);
+int ssx_vc_end(ssx_ctx *c, int prec, double *px);
+
 /* one step of the particle walk (inertial particles, or RK2 tracers) on
  * device particle arrays; counters: two device ints (largest pass count,
  * particles naming no element), NULL for the context's own, which then

@@ -291,12 +291,14 @@ class Batch:
         handle = None if stream is None else C.c_void_p(stream.cuda_stream)
         check(self._lib.ssx_batch_set_stream(self.h, handle))
 
-    def cg(self, fs, us, eps, max_iter, use_graph=True, allow_notconv=False):
+    def cg(self, fs, us, eps, max_iter, use_graph=True, allow_notconv=False,
+           sync=True):
         """fs, us: lists of float64 device tensors; returns a list of
-        Stats.  Synchronises the device first (the batch runs on its own
-        stream)."""
+        Stats.  sync: synchronise the device first (torch's streams are not
+        the contexts'; the batch itself waits for the contexts' streams)."""
         import torch
-        torch.cuda.synchronize()
+        if sync:
+            torch.cuda.synchronize()
         n = self.n
         fp = (C.c_void_p * n)(*[f.data_ptr() for f in fs])
         up = (C.c_void_p * n)(*[u.data_ptr() for u in us])
@@ -304,6 +306,25 @@ class Batch:
         st = (Stats * n)()
         rc = self._lib.ssx_batch_cg(self.h, fp, up, ep, max_iter,
                                     1 if use_graph else 0, st)
+        if not (allow_notconv and rc == 4):
+            check(rc)
+        return list(st)
+
+    def solve(self, precon, fs, us, eps, max_iter, use_graph=True,
+              allow_notconv=False, sync=True):
+        """ssx_batch_solve: precon[i] PC_MG (FCG + p-multigrid) or
+        PC_JACOBI (Jacobi-PCG) per problem; fs, us as cg"""
+        import torch
+        if sync:
+            torch.cuda.synchronize()
+        n = self.n
+        pc = (C.c_int * n)(*[int(p) for p in precon])
+        fp = (C.c_void_p * n)(*[f.data_ptr() for f in fs])
+        up = (C.c_void_p * n)(*[u.data_ptr() for u in us])
+        ep = (C.c_double * n)(*[float(e) for e in eps])
+        st = (Stats * n)()
+        rc = self._lib.ssx_batch_solve(self.h, pc, fp, up, ep, max_iter,
+                                       1 if use_graph else 0, st)
         if not (allow_notconv and rc == 4):
             check(rc)
         return list(st)

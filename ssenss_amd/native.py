@@ -41,7 +41,7 @@ EXPORTED_SYMBOLS = [
     "ssx_autotune", "ssx_ax_variant",
     "ssx_set_ax_variant", "ssx_batch_create", "ssx_batch_destroy",
     "ssx_batch_set_stream", "ssx_batch_cg",
-    "ssx_particles_step_host_noalloc",
+    "ssx_particles_step_host_noalloc", "ssx_batch_solve",
 ]
 
 
@@ -155,6 +155,8 @@ def load():
         "ssx_batch_set_stream": (i, [vp, vp]),
         "ssx_batch_cg": (i, [vp, C.POINTER(vp), C.POINTER(vp), dp, i, i,
                              C.POINTER(Stats)]),
+        "ssx_batch_solve": (i, [vp, ip, C.POINTER(vp), C.POINTER(vp), dp, i,
+                                i, C.POINTER(Stats)]),
         "ssx_ax_variant": (i, [vp, i, i, dp]),
         "ssx_set_ax_variant": (i, [vp, i, i, i]),
         "ssx_particles_step_host": (i, [vp, C.POINTER(ParticlesStruct), d, d,

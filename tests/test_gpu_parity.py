@@ -660,6 +660,40 @@ def test_batch_many_small_problems_match_single_solves():
             c.close()
 
 
+def test_batch_waits_for_work_on_the_context_stream():
+    """the right-hand side is assembled by a DSS still queued on the
+    context's own stream when the batch starts on its stream: the batch
+    waits for it (its solution is that of the assembled right-hand side)"""
+    Z = golden("cube2_n8")
+    mesh = golden_mesh(Z)
+    rng = np.random.default_rng(5)
+    with S.Context(mesh) as ctx:             # its own non-blocking stream
+        raw = rng.standard_normal(mesh.ntot) * ctx.level_data(0, S.DATA_MASS)
+        ref_f = dev(raw)
+        ref_u = torch.zeros_like(ref_f)
+        torch.cuda.synchronize()
+        ctx.dss(0, ref_f)
+        ctx.synchronize()
+        eps = 1e-10 * np.linalg.norm(host(ref_f))
+        st1 = ctx.cg(0, S.OP_HELMHOLTZ, S.X_VELOCITY, ref_f, ref_u, eps, 5000)
+        batch = S.Batch([(ctx, S.OP_HELMHOLTZ, S.X_VELOCITY)])
+        try:
+            f = dev(raw)
+            u = torch.zeros_like(f)
+            torch.cuda.synchronize()
+            for _ in range(3):                # queued on the context stream
+                ctx.mask(0, S.X_VELOCITY, f)
+            ctx.dss(0, f)
+            st = batch.cg([f], [u], [eps], 5000, sync=False)
+        finally:
+            batch.close()
+        print(f"[gpu] batch after context-stream work: {st[0].iterations} "
+              f"its (single {st1.iterations})")
+        assert st[0].converged
+        assert abs(st[0].iterations - st1.iterations) <= 1
+        assert rel(host(u), host(ref_u)) <= 1e-8
+
+
 def test_batch_graph_replays_match_eager_launches():
     """the captured-graph iterations give bit for bit what eager launches
     give, replay after replay, with the program's own eager work and
