@@ -301,6 +301,51 @@ int ssx_tracers_step_host(ssx_ctx *ctx, const ssx_particles *p,
 int ssx_particles_step_host_noalloc(ssx_ctx *ctx, const ssx_particles *p,
 		double t_now, double dt, int *max_passes);
 
+/* ---- Navier-Stokes time step (navier_stokes.c:224-235) ----
+ * The reference's splitting scheme on the context's mesh, every field in
+ * device memory: OIFS convection (three RK4 sub-integrations with dt/2
+ * steps, navier_stokes.c:71-92, time.c:84-152) with the dealiased
+ * skew-symmetric convection operator at nconv = 3n/2 points per direction
+ * (convection.c:31-140, main.c:53); the pressure right-hand side
+ * (navier_stokes.c:115-146, stokes.c:107-130), EXT3 pressure estimate and
+ * FCG + p-MG pressure solve (ssx_fcg, :148-159); the velocity right-hand
+ * side, EXT3 estimate and the three Helmholtz Jacobi-PCG solves (:161-222;
+ * with batch_velocity as one ssx_batch_cg of three problems); BDF3/EXT3
+ * weights (time.c:51-73); solves to 1e-10 in the weighted norm
+ * (residual_tolerance, navier_stokes.c:94-99).  The velocity solves need
+ * the new pressure (navier_stokes.c:184), so a step's pressure solve runs
+ * before them.  Dirichlet values are zero (the reference's default
+ * boundary functions, structs.c:67-68); the body force g is constant
+ * (main.c:104-106).  The context's lambda must be (11/6)/(dt visc);
+ * n <= 10 (nconv <= 15). */
+typedef struct ssx_ns ssx_ns;
+typedef struct ssx_ns_params {
+	double dt, visc, dens;     /* params->dt, visc, dens                 */
+	double t;                  /* time of the newest velocity level      */
+	double g[3];               /* body force gx, gy, gz                  */
+	int batch_velocity;        /* 1: the velocity solves as one batch    */
+} ssx_ns_params;
+#define SSX_NS_U 0
+#define SSX_NS_V 1
+#define SSX_NS_W 2
+#define SSX_NS_P 3
+int ssx_ns_create(ssx_ns **ns, ssx_ctx *ctx, const ssx_ns_params *prm);
+void ssx_ns_destroy(ssx_ns *ns);
+/* initialise_from_function (poisson.c:245-282) from arrays: vel[l][c] the
+ * velocity component c (device E-vector) at t - (2-l) dt, l = 0, 1, 2;
+ * the convection and double-curl histories of the two older levels are
+ * formed as there, the pressure history is zero */
+int ssx_ns_init(ssx_ns *ns, const double *const vel[3][3]);
+/* one time step; stats (nullable): [0] pressure FCG, [1..3] velocity CG */
+int ssx_ns_step(ssx_ns *ns, ssx_stats stats[4]);
+/* device E-vector of a field (SSX_NS_U..P) at history level 0 (newest),
+ * 1 or 2; valid until the next ssx_ns_step */
+double *ssx_ns_field(ssx_ns *ns, int which, int level);
+/* copy that field into the device E-vector dst (ordered on the context's
+ * stream) */
+int ssx_ns_read(ssx_ns *ns, int which, int level, double *dst);
+double ssx_ns_time(const ssx_ns *ns);
+
 /* ---- batches of independent problems ----
  * Jacobi-PCG (conjugate_gradient, solver.c:33-81) on level 0 of `count`
  * contexts at once: problem i applies op[i] for variable var[i] of ctx[i]

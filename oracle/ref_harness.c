@@ -14,6 +14,16 @@
  *       up as main.c:60-79 does (read_grid replaced by filling the same
  *       fields: dataio.c:407-452 with file n = run n), and writes
  *       <dir>/ref_*.bin and <dir>/ref_scalars.txt.
+ *   ref_harness ns <dir>
+ *       Navier-Stokes time steps: the element structures as main.c:45-79
+ *       with the convective geometry (the grid interpolated to nconv =
+ *       3n/2 points, as read_grid does for conv_geom), the velocity history
+ *       from <dir>/ns_vel.bin (u,v,w at t0-2dt, t0-dt, t0) set up as
+ *       initialise_from_function (poisson.c:245-282) sets it from
+ *       functions, then <dir>/ns.bin's step count of main.c:102-107
+ *       (source term g, integrate_navier_stokes navier_stokes.c:224-235);
+ *       writes <dir>/ref_ns_%d.bin (u, v, w, p) after each step.
+ *       ns.bin: double dt, visc, dens, t0, gx, gy, gz; int nsteps.
  *
  * Mesh file (little endian): int n, nel; double dt, visc; int svvn;
  * double svva; int neigh[nel*6], edge_neigh[nel*12], edge_size[nel*12],
@@ -38,6 +48,11 @@
 #include "global.h"
 #include "solver.h"
 #include "particles.h"
+#include "time.h"
+#include "convection.h"
+#include "stokes.h"
+#include "navier_stokes.h"
+#include "test.h"
 #include <unistd.h>
 
 static void die(const char *m)
@@ -452,15 +467,169 @@ static void do_mesh(const char *dir)
 	free(fr);
 }
 
+/* the mesh file and both geometries (main.c:45-79) */
+static struct element **ns_setup(const char *dir, const double *prm,
+		int *n_out, int *nel_out)
+{
+	char path[1024];
+	snprintf(path, sizeof(path), "%s/mesh.bin", dir);
+	FILE *f = fopen(path, "rb");
+	if (!f)
+		die("mesh.bin");
+	int hdr[2];
+	if (fread(hdr, sizeof(int), 2, f) != 2)
+		die("mesh header");
+	const int n = hdr[0], nel = hdr[1];
+	double dtv[2], svva;
+	int svvn;
+	if (fread(dtv, sizeof(double), 2, f) != 2 || fread(&svvn, sizeof(int), 1, f)
+			!= 1 || fread(&svva, sizeof(double), 1, f) != 1)
+		die("mesh params");
+	int *neigh = rd(f, sizeof(int), (size_t) nel * 6);
+	int *eneigh = rd(f, sizeof(int), (size_t) nel * 12);
+	int *esize = rd(f, sizeof(int), (size_t) nel * 12);
+	int *cneigh = rd(f, sizeof(int), (size_t) nel * 8);
+	int *csize = rd(f, sizeof(int), (size_t) nel * 8);
+	const int n3 = n * n * n;
+	const size_t nt = (size_t) nel * n3;
+	double *X = rd(f, sizeof(double), nt), *Y = rd(f, sizeof(double), nt),
+		*Z = rd(f, sizeof(double), nt);
+	fclose(f);
+	struct communicator *comm = new_communicator(0, 1, MPI_COMM_WORLD,
+			MPI_INFO_NULL);
+	struct parameters *params = new_parameters(0, 1);
+	params->n = n;
+	params->nel = nel;
+	params->neltot = nel;
+	params->nconv = (3 * n) / 2;                 /* main.c:53 */
+	params->svvn = svvn;
+	params->svva = svva;
+	params->dt = prm[0];
+	params->visc = prm[1];
+	params->dens = prm[2];
+	params->t = prm[3];
+	params->gx = prm[4];
+	params->gy = prm[5];
+	params->gz = prm[6];
+	struct basis *gll = new_gll_basis(n);
+	struct basis *conv = new_gll_basis(params->nconv);
+	create_interpolation_matrices(gll, conv);
+	compute_svv_derivative_matrix(gll, params->svvn, params->svva);
+	struct geometry **geom = new_geometries(gll, nel, nel);
+	struct geometry **cgeom = new_geometries(conv, nel, nel);
+	double ***tmp = new_3d_array(n, n, n);
+	for (int e = 0; e < nel; ++e) {
+		struct geometry *gs[2] = { geom[e], cgeom[e] };
+		for (int k = 0; k < 2; ++k) {
+			struct geometry *g = gs[k];
+			g->id = e;
+			for (int s = 0; s < 6; ++s)
+				g->neigh_rot[s] = 0;
+			memcpy(g->edge_size, esize + e * 12, 12 * sizeof(int));
+			memcpy(g->corner_size, csize + e * 8, 8 * sizeof(int));
+			get_neighbour_elnums_procs(g->neigh_el, g->neigh_proc,
+					neigh + e * 6, nel, 1, 6);
+			get_neighbour_elnums_procs(g->neigh_edge_el,
+					g->neigh_edge_proc, eneigh + e * 12, nel, 1, 12);
+			get_neighbour_elnums_procs(g->neigh_corn_el,
+					g->neigh_corn_proc, cneigh + e * 8, nel, 1, 8);
+			set_boundary_conditions(g);
+		}
+		memcpy(**geom[e]->grid->x, X + (size_t) e * n3, n3 * sizeof(double));
+		memcpy(**geom[e]->grid->y, Y + (size_t) e * n3, n3 * sizeof(double));
+		memcpy(**geom[e]->grid->z, Z + (size_t) e * n3, n3 * sizeof(double));
+		compute_geometry_transformations(geom[e], gll);
+		/* dataio.c:433-441: the file's grid interpolated to the basis */
+		const double *src[3] = { X, Y, Z };
+		double ***dst[3] = { cgeom[e]->grid->x, cgeom[e]->grid->y,
+			cgeom[e]->grid->z };
+		for (int a = 0; a < 3; ++a) {
+			memcpy(**tmp, src[a] + (size_t) e * n3, n3 * sizeof(double));
+			interpolate_to_grid(conv, dst[a], gll, tmp);
+		}
+		compute_geometry_transformations(cgeom[e], conv);
+	}
+	free_3d_array(tmp);
+	struct element **sel = new_elements(geom, cgeom, gll, conv, params, comm);
+	connect_elements(sel);
+	initialise_preconditioners(sel);
+	free(neigh); free(eneigh); free(esize); free(cneigh); free(csize);
+	free(X); free(Y); free(Z);
+	*n_out = n;
+	*nel_out = nel;
+	return sel;
+}
+
+static void do_ns(const char *dir)
+{
+	char path[1024];
+	snprintf(path, sizeof(path), "%s/ns.bin", dir);
+	FILE *f = fopen(path, "rb");
+	if (!f)
+		die("ns.bin");
+	double prm[7];
+	int nsteps;
+	if (fread(prm, sizeof(double), 7, f) != 7
+			|| fread(&nsteps, sizeof(int), 1, f) != 1)
+		die("ns params");
+	fclose(f);
+	int n, nel;
+	struct element **sel = ns_setup(dir, prm, &n, &nel);
+	const int n3 = n * n * n;
+	const size_t nt = (size_t) nel * n3;
+	double *vel = rdvec(dir, "ns_vel.bin", 9 * nt);
+	/* initialise_from_function (poisson.c:245-282) with the levels
+	 * given as arrays instead of evaluated functions */
+	for (int j = 0; j < 3; ++j) {
+		for (int e = 0; e < nel; ++e) {
+			time_shift(sel[e]->u, sel[e]->old->u, sel[e]->old->old->u,
+					sel[e]->ntot);
+			for (int a = 0; a < 3; ++a)
+				memcpy(***sel[e]->u->ind[a], vel + (3 * j + a) * nt
+						+ (size_t) e * n3, n3 * sizeof(double));
+			if (j >= 2)
+				continue;
+			time_shift(sel[e]->conv, sel[e]->old->conv,
+					sel[e]->old->old->conv, sel[e]->ntot);
+			time_shift(sel[e]->rot, sel[e]->old->rot,
+					sel[e]->old->old->rot, sel[e]->ntot);
+			convection_operator(sel[e], sel[e]->u, sel[e]->u, sel[e]->conv);
+			double_curl_operator(sel[e], sel[e]->u, sel[e]->rot);
+		}
+	}
+	struct parameters *params = (*sel)->params;
+	double *out = malloc(sizeof(double) * 4 * nt);
+	for (int step = 0; step < nsteps; ++step) {
+		/* main.c:104-107 */
+		set_source_term(sel, X_SOURCE, one_func, params->gx);
+		set_source_term(sel, Y_SOURCE, one_func, params->gy);
+		set_source_term(sel, Z_SOURCE, one_func, params->gz);
+		integrate_navier_stokes(sel);
+		for (int e = 0; e < nel; ++e) {
+			for (int a = 0; a < 3; ++a)
+				memcpy(out + a * nt + (size_t) e * n3,
+						***sel[e]->u->ind[a], n3 * sizeof(double));
+			memcpy(out + 3 * nt + (size_t) e * n3, **sel[e]->p,
+					n3 * sizeof(double));
+		}
+		snprintf(path, sizeof(path), "ref_ns_%d.bin", step + 1);
+		wr(dir, path, out, 4 * nt);
+	}
+	free(out);
+	free(vel);
+}
+
 int main(int argc, char **argv)
 {
 	MPI_Init(&argc, &argv);
 	if (argc < 3)
-		die("usage: ref_harness basis <out> | mesh <dir>");
+		die("usage: ref_harness basis <out> | mesh <dir> | ns <dir>");
 	if (!strcmp(argv[1], "basis"))
 		do_basis(argv[2]);
 	else if (!strcmp(argv[1], "mesh"))
 		do_mesh(argv[2]);
+	else if (!strcmp(argv[1], "ns"))
+		do_ns(argv[2]);
 	else
 		die("unknown task");
 	MPI_Finalize();

@@ -16,6 +16,7 @@ from .native import (  # noqa: F401
     AX_NVARIANTS,
     DATA_GFAC, DATA_MASS, DATA_DIAG_P, DATA_DIAG_H, DATA_SCALE, DATA_MASK,
     DATA_INVMASS, DATA_X, DATA_Y, DATA_Z, EXPORTED_SYMBOLS,
+    NS_U, NS_V, NS_W, NS_P,
 )
 from .mesh import Mesh, box_mesh  # noqa: F401
-from .engine import Batch, Context  # noqa: F401
+from .engine import Batch, Context, NavierStokes  # noqa: F401

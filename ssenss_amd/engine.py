@@ -5,7 +5,7 @@ import ctypes as C
 
 import numpy as np
 
-from .native import ParticlesStruct, Stats, check, load
+from .native import NSParams, ParticlesStruct, Stats, check, load
 
 
 def _ptr(t):
@@ -328,3 +328,57 @@ class Batch:
         if not (allow_notconv and rc == 4):
             check(rc)
         return list(st)
+
+
+class NavierStokes:
+    """The reference's Navier-Stokes time step on a context's mesh
+    (ssx_ns_*), every field on the GPU."""
+
+    def __init__(self, ctx, dt, visc, dens=1.0, t=0.0, g=(0.0, 0.0, 0.0),
+                 batch_velocity=True):
+        self._lib = load()
+        self._ctx = ctx                                 # keep it alive
+        prm = NSParams(float(dt), float(visc), float(dens), float(t),
+                       (C.c_double * 3)(*[float(x) for x in g]),
+                       1 if batch_velocity else 0)
+        h = C.c_void_p()
+        check(self._lib.ssx_ns_create(C.byref(h), ctx.h, C.byref(prm)))
+        self.h = h
+        self.ntot = ctx.ntot(0)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.ssx_ns_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def init(self, levels):
+        """levels[l][c]: float64 device tensor of velocity component c at
+        t - (2 - l) dt (l = 0 oldest)"""
+        arr = (C.c_void_p * 9)(*[levels[l][c].data_ptr() for l in range(3)
+                                 for c in range(3)])
+        check(self._lib.ssx_ns_init(self.h, arr))
+
+    def step(self):
+        st = (Stats * 4)()
+        check(self._lib.ssx_ns_step(self.h, st))
+        return list(st)
+
+    def read(self, which, level=0):
+        import torch
+        out = torch.empty(self.ntot, dtype=torch.float64, device="cuda")
+        check(self._lib.ssx_ns_read(self.h, which, level,
+                                    C.c_void_p(out.data_ptr())))
+        return out
+
+    @property
+    def time(self):
+        return self._lib.ssx_ns_time(self.h)

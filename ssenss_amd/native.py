@@ -42,7 +42,18 @@ EXPORTED_SYMBOLS = [
     "ssx_set_ax_variant", "ssx_batch_create", "ssx_batch_destroy",
     "ssx_batch_set_stream", "ssx_batch_cg",
     "ssx_particles_step_host_noalloc", "ssx_batch_solve",
+    "ssx_ns_create", "ssx_ns_destroy", "ssx_ns_init", "ssx_ns_step",
+    "ssx_ns_field", "ssx_ns_read", "ssx_ns_time",
 ]
+
+
+NS_U, NS_V, NS_W, NS_P = 0, 1, 2, 3
+
+
+class NSParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("visc", C.c_double),
+                ("dens", C.c_double), ("t", C.c_double),
+                ("g", C.c_double * 3), ("batch_velocity", C.c_int)]
 
 
 class SsxError(RuntimeError):
@@ -157,6 +168,13 @@ def load():
                              C.POINTER(Stats)]),
         "ssx_batch_solve": (i, [vp, ip, C.POINTER(vp), C.POINTER(vp), dp, i,
                                 i, C.POINTER(Stats)]),
+        "ssx_ns_create": (i, [C.POINTER(vp), vp, C.POINTER(NSParams)]),
+        "ssx_ns_destroy": (None, [vp]),
+        "ssx_ns_init": (i, [vp, C.POINTER(vp)]),
+        "ssx_ns_step": (i, [vp, C.POINTER(Stats)]),
+        "ssx_ns_field": (vp, [vp, i, i]),
+        "ssx_ns_read": (i, [vp, i, i, vp]),
+        "ssx_ns_time": (d, [vp]),
         "ssx_ax_variant": (i, [vp, i, i, dp]),
         "ssx_set_ax_variant": (i, [vp, i, i, i]),
         "ssx_particles_step_host": (i, [vp, C.POINTER(ParticlesStruct), d, d,
