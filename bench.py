@@ -117,9 +117,10 @@ def particle_case(S, torch, stream, npart=1_000_000, seed=7):
     and z, walls in y.  Velocity: Poiseuille u = 1 - y^2 plus a seeded
     divergence-free perturbation of 10% amplitude, set three times (the
     3-level history).  npart particles at uniform random element and
-    reference position, velocity 0, response time tau ~ U[0.05, 0.5],
-    advanced over one step dt = 2e-3 (one element is about 0.2 long in x,
-    so few particles change element)."""
+    reference position, reference-space velocity (3, 0, 0) (about the
+    flow's: centreline speed 1 times 2/h for elements h ~ 0.5 long in x),
+    response time tau ~ U[0.05, 0.5], advanced over one step dt = 0.05,
+    so that a share of them changes element."""
     from ssenss_amd.mesh import box_mesh, PERIODIC, NOSLIP
     mesh = box_mesh(10, (24, 16, 24), lo=(0.0, -1.0, 0.0),
                     hi=(4 * np.pi, 1.0, 4 * np.pi / 3),
@@ -137,7 +138,7 @@ def particle_case(S, torch, stream, npart=1_000_000, seed=7):
     for _ in range(3):
         ctx.particles_set_velocity(*uv)
     rng = np.random.default_rng(seed)
-    dt, T = 2e-3, 1.0
+    dt, T = 0.05, 1.0
     ip = lambda a: torch.from_numpy(a.astype(np.int32)).cuda()
     fp = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
     el = rng.integers(0, mesh.nel, npart)
@@ -148,7 +149,7 @@ def particle_case(S, torch, stream, npart=1_000_000, seed=7):
     def fresh():
         return {"el": ip(base["el"]), "old_el": ip(base["el"]),
                 "x": fp(base["xyz"][0]), "y": fp(base["xyz"][1]),
-                "z": fp(base["xyz"][2]), "u": fp(zero), "v": fp(zero),
+                "z": fp(base["xyz"][2]), "u": fp(zero + 3.0), "v": fp(zero),
                 "w": fp(zero), "t": fp(zero + (T - dt)),
                 "tau": fp(base["tau"]), "reinit": ip(np.zeros(npart))}
     ctx.particles_step(fresh(), T, dt)              # warm-up
@@ -253,6 +254,296 @@ def batch_case(S, torch, nprob=16, seed=11, mrange=(8, 16),
                 abs(a - b) for a, b in zip(single_its, res[True][1]))}
 
 
+def channel_field(mesh, lx, lz, amp, seed):
+    """Poiseuille u = 1 - y^2 plus a seeded divergence-free perturbation:
+    (u', v') = amp (d phi/dy, -d phi/dx), w' = 0, with
+    phi = (1 - y^2)^2 sum_m a_m sin(k_m x + s_m) cos(l_m z), k_m, l_m
+    multiples of 2 pi / lx, 2 pi / lz (periodic), zero with its y-derivative
+    on the walls y = +-1"""
+    rng = np.random.default_rng(seed)
+    X, Y, Z = (a.ravel() for a in (mesh.x, mesh.y, mesh.z))
+    u = 1.0 - Y ** 2
+    v = np.zeros_like(u)
+    env, denv = (1 - Y ** 2) ** 2, -4 * Y * (1 - Y ** 2)
+    for _ in range(4):
+        k = 2 * np.pi / lx * rng.integers(1, 4)
+        l = 2 * np.pi / lz * rng.integers(0, 3)
+        a, ph = rng.standard_normal(), rng.uniform(0, 2 * np.pi)
+        sx, cx = np.sin(k * X + ph), np.cos(k * X + ph)
+        u += amp * a * denv * sx * np.cos(l * Z) / 4
+        v -= amp * a * env * k * cx * np.cos(l * Z) / 4
+    return u, v, np.zeros_like(u)
+
+
+def ns_tgv_case(S, torch, stream, steps=10):
+    """The Taylor-Green configuration: periodic [0, 2 pi]^3 as 4^3
+    hexahedra at N = 7 (n = 8, 32,768 nodes), u = sin x cos y cos z,
+    v = -cos x sin y cos z, w = 0 plus seeded uniform noise of amplitude
+    1e-3 (DSS-averaged), Re = 1600, dt = 1e-3, `steps` Navier-Stokes steps
+    with the FCG + p-MG pressure solve (1e-10) and batched Helmholtz
+    velocity solves.  Per-step wall time; the step's iteration counts."""
+    from ssenss_amd.mesh import box_mesh, PERIODIC
+    visc, dt = 1 / 1600, 1e-3
+    mesh = box_mesh(8, (4, 4, 4), lo=(0, 0, 0), hi=(2 * np.pi,) * 3,
+                    bc=(PERIODIC,) * 6, dt=dt, visc=visc)
+    ctx = S.Context(mesh)
+    ctx.set_stream(stream)
+    X, Y, Z = (a.ravel() for a in (mesh.x, mesh.y, mesh.z))
+    rng = np.random.default_rng(1600)
+    levels = []
+    for lev in range(3):
+        d = np.exp(-3 * visc * (lev - 2) * dt)
+        comps = []
+        for c in (d * np.sin(X) * np.cos(Y) * np.cos(Z),
+                  -d * np.cos(X) * np.sin(Y) * np.cos(Z), np.zeros_like(X)):
+            t = torch.from_numpy(c + 1e-3 * rng.uniform(-1, 1, c.shape)).cuda()
+            ctx.dss_average(0, t)
+            comps.append(t)
+        levels.append(comps)
+    res = {}
+    with S.NavierStokes(ctx, dt, visc, 1.0, 0.0, (0, 0, 0), True) as ns:
+        ns.init(levels)
+        torch.cuda.synchronize()
+        times, its = [], []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            st = ns.step()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+            its.append([x.iterations for x in st])
+        u = [ns.read(k) for k in range(3)]
+        ke = 0.5 * sum(ctx.dot(0, c, c) for c in u) / ctx.dot(
+            0, torch.ones_like(u[0]), torch.ones_like(u[0]))
+        res = {"workload": "Taylor-Green, periodic 4^3 hexahedra, N=7, "
+                           "Re=1600, dt=1e-3", "local_dofs": mesh.ntot,
+               "steps": steps, "seconds_per_step_first": times[0],
+               "seconds_per_step": float(np.median(times[1:])),
+               "iterations_per_step": its[-1],
+               "kinetic_energy_final": float(ke)}
+    ctx.close()
+    return res
+
+
+def ns_channel_case(S, torch, stream, steps=3, ntracers=1_000_000,
+                    prec="fp32"):
+    """The channel configuration: 24x16x24 hexahedra at N = 9 (n = 10,
+    9.2M nodes) on [0, 4 pi] x [-1, 1] x [0, 4 pi / 3], tanh wall
+    stretching 2.0 in y, periodic in x and z, visc = 1/180 (Re_tau = 180
+    with u_tau = h = 1), body force gx = 2 visc (the laminar profile's
+    balance), dt = 2e-3; Poiseuille profile plus 10% divergence-free noise
+    at all three history levels.  Each step: OIFS convection, FCG +
+    p-MG pressure solve with the FP32 smoother, the three velocity
+    Helmholtz solves as one batch, then the velocity history updated and
+    `ntracers` RK2 tracers advanced over the step.  The first step is a
+    warm-up (graph capture, reduced-precision copies)."""
+    from ssenss_amd.mesh import box_mesh, PERIODIC, NOSLIP
+    visc, dt = 1 / 180, 2e-3
+    lx, lz = 4 * np.pi, 4 * np.pi / 3
+    mesh = box_mesh(10, (24, 16, 24), lo=(0.0, -1.0, 0.0), hi=(lx, 1.0, lz),
+                    bc=(PERIODIC, PERIODIC, NOSLIP, NOSLIP, PERIODIC,
+                        PERIODIC), deform=2, amp=2.0, dt=dt, visc=visc)
+    t0 = time.perf_counter()
+    ctx = S.Context(mesh)
+    ctx.set_stream(stream)
+    tset = time.perf_counter() - t0
+    code = {"fp64": S.PREC_FP64, "fp32": S.PREC_FP32, "bf16": S.PREC_BF16}
+    ctx.set_option(S.OPT_SMOOTHER_PRECISION, code[prec])
+    u0 = channel_field(mesh, lx, lz, 0.1, 180)
+    levels = []
+    for _ in range(3):
+        comps = []
+        for c in u0:
+            t = torch.from_numpy(np.ascontiguousarray(c)).cuda()
+            ctx.dss_average(0, t)
+            comps.append(t)
+        levels.append(comps)
+    rng = np.random.default_rng(181)
+    ip = lambda a: torch.from_numpy(a.astype(np.int32)).cuda()
+    fp = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    el = rng.integers(0, mesh.nel, ntracers)
+    xyz = rng.uniform(-1, 1, (3, ntracers))
+    zero = np.zeros(ntracers)
+    tr = {"el": ip(el), "old_el": ip(el), "x": fp(xyz[0]), "y": fp(xyz[1]),
+          "z": fp(xyz[2]), "u": fp(zero), "v": fp(zero), "w": fp(zero),
+          "t": fp(zero), "reinit": ip(zero)}
+    res = {}
+    with S.NavierStokes(ctx, dt, visc, 1.0, 0.0, (2 * visc, 0, 0),
+                        True) as ns:
+        ns.init(levels)
+        for lev in levels:                       # tracer velocity history
+            ctx.particles_set_velocity(*lev)
+        torch.cuda.synchronize()
+        ts, tt, its, passes = [], [], [], []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            st = ns.step()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            ctx.particles_set_velocity(*[ns.read(k) for k in range(3)])
+            passes.append(ctx.tracers_step(tr, ns.time, dt))
+            t2 = time.perf_counter()
+            ts.append(t1 - t0)
+            tt.append(t2 - t1)
+            its.append([x.iterations for x in st])
+        moved = int((tr["el"].cpu().numpy() != el).sum())
+        res = {"workload": "channel 24x16x24 hexahedra, N=9, tanh 2.0, "
+                           "Re_tau 180, dt 2e-3", "local_dofs": mesh.ntot,
+               "context_setup_s": tset, "pressure_smoother": prec,
+               "steps": steps, "seconds_per_step_first": ts[0],
+               "seconds_per_step": float(np.median(ts[1:])),
+               "iterations_per_step": its[-1],
+               "tracers_rk2": {"tracers": ntracers,
+                               "seconds_per_step": float(np.median(tt[1:])),
+                               "max_passes": max(passes),
+                               "changed_element_total": moved}}
+    ctx.close()
+    return res
+
+
+def batch_mg_case(S, torch, nprob=64, seed=64, mrange=(8, 24),
+                  points=(5, 6, 8, 10, 12), prec="fp32"):
+    """The ensemble configuration: nprob independent Helmholtz problems on
+    randomly deformed boxes (amplitude U(0, 0.15)), element counts m^3 with
+    m uniform in mrange, orders N in {4, 5, 7, 9, 11} (n = N + 1), lambda ~
+    U[1, 100], right-hand side the weak form of a seeded N(0, 1) nodal
+    field, FCG + p-multigrid to 1e-10 relative with the FP64 outer loop and
+    the reduced-precision smoother `prec`.  All problems at once
+    (ssx_batch_solve: each V-cycle's level CGs of all problems in shared
+    launches) against the same problems solved one by one (ssx_fcg); both
+    after a warm-up solve of each problem."""
+    from ssenss_amd.mesh import box_mesh
+    code = {"fp64": S.PREC_FP64, "fp32": S.PREC_FP32, "bf16": S.PREC_BF16}
+    rng = np.random.default_rng(seed)
+    ctxs, rhs, shapes = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(nprob):
+        m = int(rng.integers(mrange[0], mrange[1] + 1))
+        n = int(rng.choice(list(points)))
+        lam = rng.uniform(1, 100)
+        mesh = box_mesh(n, (m, m, m), deform=1, amp=rng.uniform(0, 0.15),
+                        dt=(11.0 / 6.0) / lam, visc=1.0)
+        c = S.Context(mesh)
+        c.set_stream(torch.cuda.current_stream())
+        c.set_option(S.OPT_SMOOTHER_PRECISION, code[prec])
+        # right-hand side: DSS(w detJ f) of a seeded N(0,1) nodal field,
+        # drawn on the GPU (the mesh arrays stay on the host only for the
+        # context set-up)
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(int(rng.integers(1 << 30)))
+        massv = torch.from_numpy(c.level_data(0, S.DATA_MASS)).cuda()
+        b = torch.randn(c.ntot(0), dtype=torch.float64, device="cuda",
+                        generator=gen) * massv
+        c.dss(0, b)
+        del mesh
+        ctxs.append(c)
+        rhs.append(b)
+        shapes.append((m, n))
+    tset = time.perf_counter() - t0
+    log(f"[bench] ensemble of {nprob}: set-up {tset:.1f} s")
+    eps = [1e-10 * float(np.sqrt(c.dot(0, b, b))) for c, b in zip(ctxs, rhs)]
+    dofs = sum(c.ntot(0) for c in ctxs)
+    t_seq, single_its = None, []
+    for rep in range(2):                 # the first run is a warm-up
+        single_its = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for c, b, e in zip(ctxs, rhs, eps):
+            f = b.clone()
+            u = torch.zeros_like(f)
+            st = c.fcg(S.OP_HELMHOLTZ, S.X_VELOCITY, S.PC_MG, f, u, e, 500)
+            single_its.append(st.iterations)
+        torch.cuda.synchronize()
+        t_seq = time.perf_counter() - t0
+    log(f"[bench] ensemble one by one {t_seq:.3f} s")
+    batch = S.Batch([(c, S.OP_HELMHOLTZ, S.X_VELOCITY) for c in ctxs])
+    t_b, b_its = None, []
+    for rep in range(2):
+        fs = [b.clone() for b in rhs]
+        us = [torch.zeros_like(f) for f in fs]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = batch.solve([S.PC_MG] * nprob, fs, us, eps, 500)
+        t_b = time.perf_counter() - t0
+        b_its = [x.iterations for x in st]
+    log(f"[bench] ensemble batched {t_b:.3f} s")
+    batch.close()
+    for c in ctxs:
+        c.close()
+    return {"problems": nprob, "local_dofs": dofs,
+            "elements_per_dir": list(mrange), "orders": sorted(
+                {n - 1 for _, n in shapes}),
+            "smoother_precision": prec, "context_setup_s": tset,
+            "sequential_s": t_seq, "batched_s": t_b,
+            "speedup_batched": t_seq / t_b,
+            "iterations_single": single_its, "iterations_batched": b_its,
+            "max_iteration_difference": max(
+                abs(a - b) for a, b in zip(single_its, b_its))}
+
+
+def reference_cpu(seconds):
+    """The compiled reference (oracle/_ref/ref_harness, built from the
+    SSENSS sources by oracle/Makefile) timed on the host: its Poisson
+    element operator over all elements for about `seconds`, and one
+    Dirichlet FCG + p-MG solve to 1e-10 relative, on a 8^3-element sample
+    of the deformed cube at n = 8, one core (MKL sequential, one rank);
+    None where the harness or its libraries are absent."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.access(exe, os.X_OK):
+        return None
+    from ssenss_amd.mesh import box_mesh
+    from oracle import pyoracle as P
+    sm = box_mesh(8, (8, 8, 8), deform=1, amp=0.1)
+    f = smooth_source(sm.x.ravel(), sm.y.ravel(), sm.z.ravel())
+    o = P.Oracle(sm)
+    _, rho, _, _ = P.gll_basis(8)
+    w3 = (rho[None, None, :] * rho[None, :, None] * rho[:, None, None]).ravel()
+    b = o.dss(0, np.tile(w3, sm.nel) * o.data(0, P.DETJ) * f)
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "mesh.bin"), "wb") as fh:
+            fh.write(np.array([sm.n, sm.nel], np.int32).tobytes())
+            fh.write(np.array([1.0, 1.0], np.float64).tobytes())
+            fh.write(np.array([(2 * sm.n) // 3], np.int32).tobytes())
+            fh.write(np.array([0.0], np.float64).tobytes())
+            for a in (sm.neigh, sm.edge_neigh, sm.edge_size, sm.corn_neigh,
+                      sm.corn_size):
+                fh.write(np.ascontiguousarray(a, np.int32).tobytes())
+            for a in (sm.x, sm.y, sm.z):
+                fh.write(np.ascontiguousarray(a, np.float64).tobytes())
+        b.tofile(os.path.join(d, "in_f.bin"))
+        env = dict(os.environ, MKL_NUM_THREADS="1",
+                   MKL_THREADING_LAYER="SEQUENTIAL", OMP_NUM_THREADS="1")
+        try:
+            r = subprocess.run([exe, "time", d, str(seconds)], env=env,
+                               capture_output=True, text=True, timeout=300)
+        except (OSError, subprocess.SubprocessError) as e:
+            log(f"[bench] reference harness did not run: {e}")
+            return None
+    kv = r.stdout.split()
+    if r.returncode != 0 or "ax_seconds" not in kv:
+        log(f"[bench] reference harness failed: {r.stderr[-300:]}")
+        return None
+    vals = {kv[i]: float(kv[i + 1]) for i in range(0, len(kv) - 1, 2)}
+    # the port on the same sample, one thread and all threads
+    port = {}
+    for th in (1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()
+                                     or 1))):
+        P.set_threads(th)
+        o.set_diag_choice(0)
+        eps = 1e-10 * np.sqrt(o.dot(0, b, b))
+        t0 = time.perf_counter()
+        _, it, _, _ = o.fcg(P.OP_POISSON, P.X_VELOCITY, P.PC_MG, b,
+                            np.zeros(sm.ntot), eps, 500)
+        port[P.get_threads()] = (time.perf_counter() - t0, it)
+    o.close()
+    return {"local_dofs": sm.ntot, "ax_seconds": vals["ax_seconds"],
+            "ax_reps": int(vals["ax_reps"]),
+            "solve_seconds": vals["solve_seconds"],
+            "port_solve_seconds": {str(k): v[0] for k, v in port.items()},
+            "port_iterations": {str(k): v[1] for k, v in port.items()}}
+
+
 def cpu_baseline(mesh, seconds):
     """The oracle (C restatement of the reference CPU path) on a bounded
     sample: Poisson Ax on a 12^3-element piece of the same deformed cube
@@ -293,6 +584,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-particles", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-ns", action="store_true",
+                    help="skip the Navier-Stokes workloads")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 64-problem p-MG ensemble")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the N = 5 / 7 / 11 mixed-precision sweep")
     ap.add_argument("--profile", action="store_true",
@@ -392,13 +687,22 @@ def main():
             f"{particles['seconds']*1e3:.2f} ms per step, "
             f"{particles['max_passes']} passes; RK2 tracers "
             f"{particles['tracers_rk2']['seconds']*1e3:.2f} ms per step")
-    batchres = batch_small = None
+    ns_tgv = ns_chan = None
+    if not args.no_ns:
+        ns_tgv = ns_tgv_case(S, torch, stream)
+        log(f"[bench] NS Taylor-Green: {ns_tgv['seconds_per_step']*1e3:.1f} "
+            f"ms per step")
+        ns_chan = ns_channel_case(S, torch, stream)
+        log(f"[bench] NS channel: {ns_chan['seconds_per_step']:.3f} s per "
+            f"step, tracers {ns_chan['tracers_rk2']['seconds_per_step']*1e3:.2f}"
+            f" ms")
+    ensemble = None if args.no_ensemble else batch_mg_case(S, torch)
+    batch_small = None
     if not args.no_batch:
-        batchres = batch_case(S, torch)
         # many small problems, where one problem leaves most of the GPU idle
         batch_small = batch_case(S, torch, nprob=64, seed=12, mrange=(3, 6),
                                  points=(5, 6, 8))
-    for br in (batchres, batch_small):
+    for br in (batch_small,):
         if br:
             log(f"[bench] batch of {br['problems']} ({br['local_dofs']} "
                 f"nodes): sequential {br['sequential_s']:.3f} s, batched "
@@ -421,7 +725,30 @@ def main():
                              log_prefix=f"N={n-1} {me}^3 "))
             sctx.close()
 
-    cpu = None if args.no_cpu else cpu_baseline(mesh, args.cpu_seconds)
+    cpu, cpu_ttr = None, None
+    if not args.no_cpu:
+        ref = reference_cpu(3.0)
+        if ref:
+            cpu = {"value": ref["local_dofs"] / ref["ax_seconds"] / 1e9,
+                   "unit": "GDOF/s", "cores": 1, "kind": "reference",
+                   "sample": f"the compiled reference's Poisson element "
+                             f"operator over a 8^3-element deformed cube "
+                             f"at n=8 ({ref['local_dofs']} local nodes), "
+                             f"{ref['ax_reps']} applications, one core"}
+            gpu = solves.get("fp64")
+            cpu_ttr = {
+                "sample": "Dirichlet Poisson FCG + p-MG to 1e-10 relative, "
+                          "8^3 deformed cube, n=8",
+                "local_dofs": ref["local_dofs"],
+                "reference_1core_s": ref["solve_seconds"],
+                "port_s_by_threads": ref["port_solve_seconds"],
+                "port_iterations": ref["port_iterations"],
+                "reference_ns_per_dof": ref["solve_seconds"]
+                / ref["local_dofs"] * 1e9,
+                "gpu_fp64_ns_per_dof": (gpu["time_to_residual_s"] / nt * 1e9
+                                        if gpu else None)}
+        else:
+            cpu = cpu_baseline(mesh, args.cpu_seconds)
     out = {
         "metric": "Ax GDOF/s + p-MG PCG time to 1e-10 residual, FP64 vs "
                   "mixed, 1 MI355X",
@@ -439,6 +766,7 @@ def main():
                    "mg_orders": mg_orders},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_time_to_residual": cpu_ttr,
         "helmholtz_ax_gdofs": nt / t_helm / 1e9,
         "dss_us": t_dss * 1e6,
         "opeval_gdofs": nt / t_op / 1e9,
@@ -447,8 +775,10 @@ def main():
         "solve_mixed_bf16": solves.get("bf16"),
         "ax_launch_variants": variants,
         "particles": particles,
-        "batch_uneven": batchres,
+        "batch_mg_ensemble": ensemble,
         "batch_small": batch_small,
+        "ns_taylor_green": ns_tgv,
+        "ns_channel": ns_chan,
         "mixed_precision_sweep": sweep or None,
     }
     print(json.dumps(out), flush=True)

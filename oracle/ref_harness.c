@@ -14,6 +14,14 @@
  *       up as main.c:60-79 does (read_grid replaced by filling the same
  *       fields: dataio.c:407-452 with file n = run n), and writes
  *       <dir>/ref_*.bin and <dir>/ref_scalars.txt.
+ *   ref_harness time <dir> <seconds>
+ *       the reference's own code timed on the element structures of
+ *       <dir>/mesh.bin: the Poisson element operator over all elements
+ *       (discrete_laplace_operator, poisson.c:34-56) repeated for about
+ *       <seconds>, then one Dirichlet Poisson FCG + p-MG solve (Poisson
+ *       diagonal, X_VELOCITY mask) of <dir>/in_f.bin to 1e-10 ||b||_w from
+ *       a zero guess; prints "ax_seconds <per application> ax_reps <k>
+ *       solve_seconds <s> residual <r> eps <eps>".
  *   ref_harness ns <dir>
  *       Navier-Stokes time steps: the element structures as main.c:45-79
  *       with the convective geometry (the grid interpolated to nconv =
@@ -467,6 +475,51 @@ static void do_mesh(const char *dir)
 	free(fr);
 }
 
+static double wall(void)
+{
+	return MPI_Wtime();
+}
+
+static struct element **ns_setup(const char *dir, const double *prm,
+		int *n_out, int *nel_out);
+
+static void do_time(const char *dir, double seconds)
+{
+	const double prm[7] = { 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0 };
+	int n, nel;
+	struct element **sel = ns_setup(dir, prm, &n, &nel);
+	const size_t nt = (size_t) nel * n * n * n;
+	double *fr = rdvec(dir, "in_f.bin", nt);
+	double ****F = vec4(n, nel), ****U = vec4(n, nel), ****AU = vec4(n, nel);
+	set4(U, fr, nt);
+	for (int e = 0; e < nel; ++e)           /* warm-up */
+		discrete_laplace_operator(sel[e], U[e], AU[e]);
+	int reps = 0;
+	double t0 = wall(), t1;
+	do {
+		for (int e = 0; e < nel; ++e)
+			discrete_laplace_operator(sel[e], U[e], AU[e]);
+		++reps;
+		t1 = wall();
+	} while (t1 - t0 < seconds);
+	const double ax = (t1 - t0) / reps;
+	set4(F, fr, nt);
+	memset(***U, 0, nt * sizeof(double));
+	const double eps = 1e-10 * sqrt(global_dot_product(sel, F, F));
+	const int nl = (*sel)->precon->nlevels;
+	for (int e = 0; e < nel; ++e)
+		for (int l = 0; l < nl; ++l)
+			sel[e]->precon->helmholtz_diag[l] =
+				sel[e]->precon->poisson_diag[l];
+	t0 = wall();
+	const double err = flexible_conjugate_gradient(sel, F, U, X_VELOCITY,
+			discrete_laplace_operator, multigrid_preconditioner, eps, 5000);
+	t1 = wall();
+	printf("ax_seconds %.6e ax_reps %d solve_seconds %.6f residual %.6e "
+			"eps %.6e\n", ax, reps, t1 - t0, err, eps);
+	free(fr);
+}
+
 /* the mesh file and both geometries (main.c:45-79) */
 static struct element **ns_setup(const char *dir, const double *prm,
 		int *n_out, int *nel_out)
@@ -623,13 +676,16 @@ int main(int argc, char **argv)
 {
 	MPI_Init(&argc, &argv);
 	if (argc < 3)
-		die("usage: ref_harness basis <out> | mesh <dir> | ns <dir>");
+		die("usage: ref_harness basis <out> | mesh <dir> | ns <dir> | "
+				"time <dir> <seconds>");
 	if (!strcmp(argv[1], "basis"))
 		do_basis(argv[2]);
 	else if (!strcmp(argv[1], "mesh"))
 		do_mesh(argv[2]);
 	else if (!strcmp(argv[1], "ns"))
 		do_ns(argv[2]);
+	else if (!strcmp(argv[1], "time") && argc > 3)
+		do_time(argv[2], atof(argv[3]));
 	else
 		die("unknown task");
 	MPI_Finalize();
