@@ -596,6 +596,13 @@ def main():
                     "FCG iterations with FP64 and FP32 smoothers, then a 1 GiB "
                     "device copy (known bytes, to calibrate FETCH_SIZE/"
                     "WRITE_SIZE)")
+    ap.add_argument("--profile-case", default="cube",
+                    choices=("cube", "particles", "ns"),
+                    help="with --profile: the workload whose kernels run "
+                    "(cube: the solver kernels above; particles: the "
+                    "1M-particle channel step and RK2 tracers; ns: two "
+                    "channel Navier-Stokes steps), each followed by the 1 GiB "
+                    "calibration copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this "
@@ -606,6 +613,26 @@ def main():
     import torch
     import ssenss_amd as S
     from ssenss_amd.mesh import box_mesh
+
+    def calibration_copy():
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)                                   # 1 GiB each way
+        torch.cuda.synchronize()
+
+    if args.profile and args.profile_case != "cube":
+        stream = torch.cuda.current_stream()
+        if args.profile_case == "particles":
+            r = particle_case(S, torch, stream)
+            log(f"[bench] particles {r['seconds']*1e3:.2f} ms, tracers "
+                f"{r['tracers_rk2']['seconds']*1e3:.2f} ms")
+        else:
+            r = ns_channel_case(S, torch, stream, steps=2)
+            log(f"[bench] NS channel {r['seconds_per_step']:.3f} s per step")
+        calibration_copy()
+        log("[bench] profile mode done")
+        return
 
     m = args.elems
     log(f"[bench] building {m}^3 mesh n={args.n}")
@@ -643,11 +670,7 @@ def main():
         # coarse-level CG
         solve_case(S, torch, ctx, mesh, 2, precs=("fp64", "fp32"),
                    log_prefix="profile ")
-        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)                                   # 1 GiB each way
-        torch.cuda.synchronize()
+        calibration_copy()
         log("[bench] profile mode done")
         ctx.close()
         return

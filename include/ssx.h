@@ -209,6 +209,11 @@ int ssx_set_diag_choice(ssx_ctx *ctx, int choice);
  * per-copy partner lists, so the DSS launch and its write-back go away;
  * 0 runs the group DSS as a launch of its own.  Bit-identical results. */
 #define SSX_OPT_GATHER_DSS 7
+/* SSX_OPT_PERSIST_ABORT_TEST: test hook of the persistent CG's failure
+ * path: 1 makes the next persistent launch start with its abort word set,
+ * so every block leaves at its first grid barrier, as when a barrier wait
+ * gives up; the solve then returns SSX_EHIP.  Cleared by that launch. */
+#define SSX_OPT_PERSIST_ABORT_TEST 8
 #define SSX_PREC_FP64 0
 #define SSX_PREC_FP32 1
 #define SSX_PREC_BF16 2

@@ -644,6 +644,9 @@ int ssx_set_option(ssx_ctx *c, int option, int value)
 		}
 		c->lp_coarse_rtol = pow(10.0, -value);
 		return SSX_OK;
+	case SSX_OPT_PERSIST_ABORT_TEST:
+		c->persist_abort_test = value != 0;
+		return SSX_OK;
 	case SSX_OPT_PERSISTENT_CG:
 		if (value < 0 || value > 1024) {
 			ssx_set_error("persistent CG option %d out of range", value);
@@ -999,6 +1002,12 @@ static int cg_level_fused(ssx_ctx *c, int l, int op, int var, double *f,
 				grid = 1024;
 		}
 		TRY(ssxk_set_ints(st, (int *) c->pbar, 0, 4));
+		if (c->persist_abort_test) {
+			/* test hook: the launch starts aborted, so every block leaves
+			 * at its first barrier, as when a barrier wait gives up */
+			TRY(ssxk_set_ints(st, (int *) c->pbar + 2, 1, 1));
+			c->persist_abort_test = 0;
+		}
 		TRY(ssxk_cg_persist(st, L->n, grid, c->nel, op, !pres, pres,
 				max_iter, eps_dev, c->lambda, (double) L->nunique, L->gf,
 				L->mass, L->dmat, dinv, L->scale, mask, L->bc, L->grp,

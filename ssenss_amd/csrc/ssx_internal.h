@@ -123,6 +123,7 @@ struct ssx_ctx {
 	int gather_upd;              /* SSX_OPT_GATHER_UPDATE                */
 	unsigned *redctr;            /* device: last-workgroup counter       */
 	int persist_grid;            /* blocks of the cooperative CG launch  */
+	int persist_abort_test;      /* SSX_OPT_PERSIST_ABORT_TEST           */
 	unsigned *pbar;              /* device: grid-barrier words [4]       */
 	size_t npart;                /* doubles in part                      */
 	/* FCG history (level 0) */

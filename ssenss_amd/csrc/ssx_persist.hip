@@ -294,8 +294,9 @@ k_cg_persist(PersistArgs a)
 
 // Cooperative launch of the persistent CG on one level.  grid: blocks (at
 // most what the device keeps co-resident).  Returns SSX_EHIP if the launch
-// is refused, SSX_ENUMERIC if a barrier wait gave up (flags[0] == -1 is
-// left for the caller to read).
+// is refused.  A barrier wait that gave up makes every block leave at its
+// next barrier and leaves flags[0] == -1; the caller reads it and reports
+// SSX_EHIP (u, r and p are then those of an unfinished iteration).
 extern "C" int ssxk_cg_persist(void *stream, int n, int grid, int nel,
 		int op, int apply_mask, int pressure, int maxit, double eps,
 		double lambda, double nunique, const double *gf, const double *mass,

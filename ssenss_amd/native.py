@@ -14,6 +14,7 @@ OPT_LP_COARSE_RTOL = 4
 OPT_FUSED_REDUCE = 5
 OPT_COARSE_ASSEMBLED = 6
 OPT_GATHER_DSS = 7
+OPT_PERSIST_ABORT_TEST = 8
 AX_NVARIANTS = 8
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 NOALLOC_MAX_PARTICLES = 131072
