@@ -597,12 +597,13 @@ def main():
                     "device copy (known bytes, to calibrate FETCH_SIZE/"
                     "WRITE_SIZE)")
     ap.add_argument("--profile-case", default="cube",
-                    choices=("cube", "particles", "ns"),
+                    choices=("cube", "solve32", "particles", "ns"),
                     help="with --profile: the workload whose kernels run "
-                    "(cube: the solver kernels above; particles: the "
-                    "1M-particle channel step and RK2 tracers; ns: two "
-                    "channel Navier-Stokes steps), each followed by the 1 GiB "
-                    "calibration copy")
+                    "(cube: the solver kernels above; solve32: one whole "
+                    "FCG + p-MG solve of the cube with the FP32 smoother; "
+                    "particles: the 1M-particle channel step and RK2 "
+                    "tracers; ns: two channel Navier-Stokes steps), each "
+                    "followed by the 1 GiB calibration copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this "
@@ -621,7 +622,7 @@ def main():
         dst.copy_(src)                                   # 1 GiB each way
         torch.cuda.synchronize()
 
-    if args.profile and args.profile_case != "cube":
+    if args.profile and args.profile_case not in ("cube", "solve32"):
         stream = torch.cuda.current_stream()
         if args.profile_case == "particles":
             r = particle_case(S, torch, stream)
@@ -660,6 +661,13 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / 1e3 / steps
 
+    if args.profile and args.profile_case == "solve32":
+        solve_case(S, torch, ctx, mesh, args.max_iter, precs=("fp32",),
+                   log_prefix="profile ")
+        calibration_copy()
+        log("[bench] profile mode done")
+        ctx.close()
+        return
     if args.profile:
         for _ in range(args.steps):
             ctx.ax(0, S.OP_POISSON, u, au)

@@ -214,6 +214,12 @@ int ssx_set_diag_choice(ssx_ctx *ctx, int choice);
  * so every block leaves at its first grid barrier, as when a barrier wait
  * gives up; the solve then returns SSX_EHIP.  Cleared by that launch. */
 #define SSX_OPT_PERSIST_ABORT_TEST 8
+/* SSX_OPT_PARTICLE_WALK: how ssx_particles_step / ssx_tracers_step (and
+ * ssx_particles_step_host) take the particles, after a counting sort by
+ * owning element: 1 (default) one relocation pass per launch, the
+ * particles still moving compacted between passes; 0 the whole walk of
+ * each particle in one lane of one launch.  Bit-identical results. */
+#define SSX_OPT_PARTICLE_WALK 9
 #define SSX_PREC_FP64 0
 #define SSX_PREC_FP32 1
 #define SSX_PREC_BF16 2
@@ -293,6 +299,13 @@ int ssx_tracers_step(ssx_ctx *ctx, const ssx_particles *p, double t_now,
 		double dt, int *max_passes);
 int ssx_tracers_step_host(ssx_ctx *ctx, const ssx_particles *p,
 		double t_now, double dt, int *max_passes);
+/* How many particles entered each relocation pass of the last particle or
+ * tracer step that ran pass by pass (SSX_OPT_PARTICLE_WALK 1):
+ * counts[k], k < SSX_PASS_COUNTS (pass 0: every particle of the step).
+ * No counterpart in the reference (a diagnostic of the walk's load).  All
+ * zero before such a step. */
+#define SSX_PASS_COUNTS 12
+int ssx_particles_pass_counts(ssx_ctx *ctx, int counts[SSX_PASS_COUNTS]);
 /* ssx_particles_step_host without any device allocation.  No counterpart
  * in the reference: for callers that cannot allocate (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  The

@@ -403,6 +403,7 @@ int ssx_create(ssx_ctx **out, const ssx_mesh *m)
 	c->diag_choice = -1;
 	c->fused = 1;
 	c->persist = 0;
+	c->p_elem_walk = 1;
 	c->fused_red = 0;
 	c->coarse_csr = 1;
 	c->gather_upd = 1;
@@ -567,6 +568,10 @@ void ssx_destroy(ssx_ctx *c)
 	hipFree(c->d_f);
 	hipFree(c->d_u);
 	hipFree(c->p_stage);
+	hipFree(c->p_perm);
+	hipFree(c->p_bucket);
+	hipFree(c->p_work);
+	hipFree(c->p_pcnt);
 	free(c->h_neigh);
 	hipFree(c->d_neigh);
 	for (int a = 0; a < 9; ++a) {
@@ -646,6 +651,13 @@ int ssx_set_option(ssx_ctx *c, int option, int value)
 		return SSX_OK;
 	case SSX_OPT_PERSIST_ABORT_TEST:
 		c->persist_abort_test = value != 0;
+		return SSX_OK;
+	case SSX_OPT_PARTICLE_WALK:
+		if (value != 0 && value != 1) {
+			ssx_set_error("particle walk option %d not 0 or 1", value);
+			return SSX_EINVAL;
+		}
+		c->p_elem_walk = value;
 		return SSX_OK;
 	case SSX_OPT_PERSISTENT_CG:
 		if (value < 0 || value > 1024) {
@@ -2028,6 +2040,40 @@ int ssx_particles_set_velocity(ssx_ctx *c, const double *u, const double *v,
 
 /* one step of the particle walk on device arrays: inertial particles
  * (particles.c) or, with tracers, RK2 tracers (tau not used) */
+/* the processing-order buffers for n particles (grown on demand) */
+static int particle_order_bufs(ssx_ctx *c, long n)
+{
+	if (!c->p_bucket && hipMalloc((void **) &c->p_bucket, sizeof(int)
+			* (size_t) ssxk_particle_order_ints(c->nel, c->lev[0].n))
+			!= hipSuccess) {
+		c->p_bucket = NULL;
+		ssx_set_error("hipMalloc of the particle buckets");
+		return SSX_ENOMEM;
+	}
+	if (!c->p_pcnt && hipMalloc((void **) &c->p_pcnt,
+			2 * sizeof(int) * (SSX_WALK_PASSES + 1)) != hipSuccess) {
+		c->p_pcnt = NULL;
+		ssx_set_error("hipMalloc of the particle pass counts");
+		return SSX_ENOMEM;
+	}
+	if (c->p_perm_cap >= n)
+		return SSX_OK;
+	hipFree(c->p_perm);
+	hipFree(c->p_work);
+	c->p_perm = c->p_work = NULL;
+	c->p_perm_cap = 0;
+	if (hipMalloc((void **) &c->p_perm, sizeof(int) * (size_t) n)
+			!= hipSuccess || hipMalloc((void **) &c->p_work,
+			5 * sizeof(int) * (size_t) n) != hipSuccess) {
+		hipFree(c->p_perm);
+		c->p_perm = NULL;
+		ssx_set_error("hipMalloc of the particle order (%ld)", n);
+		return SSX_ENOMEM;
+	}
+	c->p_perm_cap = n;
+	return SSX_OK;
+}
+
 int ssx_walk_dev(ssx_ctx *c, const ssx_particles *p, double t_now,
 		double dt, int *max_passes, int tracers, int *counters)
 {
@@ -2042,6 +2088,7 @@ int ssx_walk_dev(ssx_ctx *c, const ssx_particles *p, double t_now,
 		ssx_set_error("particle step: dt must be positive");
 		return SSX_EINVAL;
 	}
+	const int own = counters == NULL;
 	if (counters) {
 		/* callers with their own counters allocate nothing: the tables
 		 * must exist already (ssx_particles_set_velocity sets them up) */
@@ -2063,17 +2110,49 @@ int ssx_walk_dev(ssx_ctx *c, const ssx_particles *p, double t_now,
 	const double *ur[9];
 	for (int a = 0; a < 9; ++a)
 		ur[a] = c->p_ur[a / 3][a % 3];
+	/* The allocating entries take the particles element by element (a
+	 * counting sort into context buffers), one relocation pass per launch
+	 * with the unfinished particles compacted between passes (option 1,
+	 * the default; 0: the whole walk in one launch).  Callers with their
+	 * own counters allocate nothing and take the particles as stored, the
+	 * whole walk in one launch.  The results are the same bits in every
+	 * form. */
+	const int *perm = NULL;
 	TRY(ssxk_set_ints(c->stream, counters, 0, 2));
+	if (own && p->n > 0) {
+		TRY(particle_order_bufs(c, p->n));
+		TRY(ssxk_particle_order(c->stream, p->n, p->el, p->z, c->p_xi,
+				c->nel, c->lev[0].n, c->p_bucket, c->p_perm));
+		perm = c->p_perm;
+		if (c->p_elem_walk > 0) {
+			c->p_pass_np = p->n;
+			if (tracers)
+				TRY(ssxk_tracers_passes(c->stream, c->lev[0].n, c->nel,
+						c->p_xi, ur, c->d_neigh, t_now, dt, p->n, perm,
+						c->p_work, c->p_pcnt, p->el, p->old_el, p->x, p->y,
+						p->z, p->u, p->v, p->w, p->t, p->reinit, counters));
+			else
+				TRY(ssxk_particles_passes(c->stream, c->lev[0].n, c->nel,
+						c->p_xi, ur, (const double *const *) c->p_jac,
+						(const double *const *) c->p_jinv, c->d_neigh,
+						t_now, dt, p->n, perm, c->p_bucket, c->p_work,
+						c->p_pcnt, p->el,
+						p->old_el, p->x, p->y, p->z, p->u, p->v, p->w, p->t,
+						p->tau, p->reinit, counters));
+			goto launched;
+		}
+	}
 	if (tracers)
 		TRY(ssxk_tracers_step(c->stream, c->lev[0].n, c->nel, c->p_xi, ur,
-				c->d_neigh, t_now, dt, p->n, p->el, p->old_el, p->x, p->y,
-				p->z, p->u, p->v, p->w, p->t, p->reinit, counters));
+				c->d_neigh, t_now, dt, p->n, perm, p->el, p->old_el, p->x,
+				p->y, p->z, p->u, p->v, p->w, p->t, p->reinit, counters));
 	else
 		TRY(ssxk_particles_step(c->stream, c->lev[0].n, c->nel, c->p_xi, ur,
 				(const double *const *) c->p_jac,
 				(const double *const *) c->p_jinv, c->d_neigh, t_now, dt,
-				p->n, p->el, p->old_el, p->x, p->y, p->z, p->u, p->v, p->w,
-				p->t, p->tau, p->reinit, counters));
+				p->n, perm, p->el, p->old_el, p->x, p->y, p->z, p->u, p->v,
+				p->w, p->t, p->tau, p->reinit, counters));
+launched:;
 	int fl[2] = { 0, 0 };
 	HIPC(hipMemcpyAsync(fl, counters, sizeof(fl), hipMemcpyDeviceToHost,
 			(hipStream_t) c->stream));
@@ -2085,6 +2164,27 @@ int ssx_walk_dev(ssx_ctx *c, const ssx_particles *p, double t_now,
 				fl[1]);
 		return SSX_EINVAL;
 	}
+	return SSX_OK;
+}
+
+int ssx_particles_pass_counts(ssx_ctx *c, int counts[SSX_PASS_COUNTS])
+{
+	TRY(chk(c, 0));
+	if (!counts) {
+		ssx_set_error("ssx_particles_pass_counts: null array");
+		return SSX_EINVAL;
+	}
+	memset(counts, 0, SSX_PASS_COUNTS * sizeof(int));
+	if (!c->p_pcnt || !c->p_pass_np)
+		return SSX_OK;
+	int d[SSX_WALK_PASSES + 1];
+	hipStream_t st = (hipStream_t) c->stream;
+	HIPC(hipMemcpyAsync(d, c->p_pcnt, sizeof(d), hipMemcpyDeviceToHost, st));
+	HIPC(hipStreamSynchronize(st));
+	/* d[k] counts the particles that entered pass k, k >= 1 */
+	counts[0] = (int) c->p_pass_np;
+	for (int k = 1; k < SSX_PASS_COUNTS && k < SSX_WALK_PASSES; ++k)
+		counts[k] = d[k];
 	return SSX_OK;
 }
 

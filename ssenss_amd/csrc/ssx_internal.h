@@ -137,6 +137,11 @@ struct ssx_ctx {
 	int p_oriented;              /* every face pair's neighbours point back */
 	void *p_stage;               /* device staging of host particle lists */
 	size_t p_stage_bytes;
+	int *p_perm, *p_bucket;      /* processing order by element, buckets  */
+	int *p_work, *p_pcnt;        /* pass lists and state (5 x cap), counts */
+	long p_perm_cap;
+	int p_elem_walk;             /* SSX_OPT_PARTICLE_WALK                 */
+	long p_pass_np;              /* particles of the last pass-by-pass step */
 	long long cnt[4];
 	double ax_us[SSX_LMAX][3][SSX_AX_NVARIANTS];  /* autotuner timings  */
 };
@@ -366,16 +371,45 @@ int ssxk_copy_lp(void *stream, void *dst, const void *src, size_t n);
 int ssxk_particles_ur(void *stream, size_t nt, const double *const jinv[9],
 		const double *u, const double *v, const double *w, double *u0,
 		double *u1, double *u2);
+/* perm: the order the particles are taken in (NULL: as stored) */
 int ssxk_tracers_step(void *stream, int n_pts, int nel, const double *xi,
 		const double *const ur[9], const int *neigh, double T, double dt,
-		long np, int *el, int *old_el, double *x, double *y, double *z,
-		double *u, double *v, double *w, double *t, int *reinit, int *flags);
+		long np, const int *perm, int *el, int *old_el, double *x, double *y,
+		double *z, double *u, double *v, double *w, double *t, int *reinit,
+		int *flags);
+/* perm = particle indices sorted by owning element and, for n > 5, the
+ * z plane their interpolation stencil starts at (counting sort; cnt:
+ * ssxk_particle_order_ints ints; z: the particles' reference z) */
+long ssxk_particle_order_buckets(int nel, int n_pts);
+long ssxk_particle_order_ints(int nel, int n_pts);
+int ssxk_particle_order(void *stream, long np, const int *el,
+		const double *z, const double *xi, int nel, int n_pts, int *cnt,
+		int *perm);
+/* The relocation loop of a particle or tracer runs at most 12 iterations
+ * (after the 11th the particle is re-seeded and stops). */
+#define SSX_WALK_PASSES SSX_PASS_COUNTS
+/* The walk one loop iteration per pass: pass 0 over perm, passes 1 .. 11
+ * over the particles still moving, compacted (the particles' pass 1 sorted
+ * again, with bucket, the order's scratch).  work: 5 np ints, cnt:
+ * 2 (SSX_WALK_PASSES + 1) ints, the first SSX_WALK_PASSES + 1 the number
+ * of particles entering each pass. */
+int ssxk_particles_passes(void *stream, int n_pts, int nel, const double *xi,
+		const double *const ur[9], const double *const jac[9],
+		const double *const jinv[9], const int *neigh, double T, double dt,
+		long np, const int *perm, int *bucket, int *work, int *cnt, int *el,
+		int *old_el, double *x, double *y, double *z, double *u, double *v,
+		double *w, double *t, const double *tau, int *reinit, int *flags);
+int ssxk_tracers_passes(void *stream, int n_pts, int nel, const double *xi,
+		const double *const ur[9], const int *neigh, double T, double dt,
+		long np, const int *perm, int *work, int *cnt, int *el, int *old_el,
+		double *x, double *y, double *z, double *u, double *v, double *w,
+		double *t, int *reinit, int *flags);
 int ssxk_particles_step(void *stream, int n_pts, int nel, const double *xi,
 		const double *const ur[9], const double *const jac[9],
 		const double *const jinv[9], const int *neigh, double T, double dt,
-		long np, int *el, int *old_el, double *x, double *y, double *z,
-		double *u, double *v, double *w, double *t, const double *tau,
-		int *reinit, int *flags);
+		long np, const int *perm, int *el, int *old_el, double *x, double *y,
+		double *z, double *u, double *v, double *w, double *t,
+		const double *tau, int *reinit, int *flags);
 /* whole Jacobi-PCG solve of one level with n <= 4 in one cooperative
  * launch (ssx_persist.hip); setup (r, z, p, <r,z>_w, flags) done by the
  * caller, bar[0..2] zeroed */

@@ -216,6 +216,13 @@ class Context:
         check(self._lib.ssx_particles_set_velocity_host(
             self.h, *(_dp(a) for a in arrs)))
 
+    def particles_pass_counts(self):
+        """particles that entered each relocation pass of the last
+        pass-by-pass step (ssx_particles_pass_counts)"""
+        a = (C.c_int * 12)()
+        check(self._lib.ssx_particles_pass_counts(self.h, a))
+        return list(a)
+
     def particles_step_host(self, p, t_now, dt):
         """p: dict of contiguous numpy arrays (int32 el, old_el, reinit;
         float64 x, y, z, u, v, w, t, tau), updated in place"""

@@ -15,6 +15,7 @@ OPT_FUSED_REDUCE = 5
 OPT_COARSE_ASSEMBLED = 6
 OPT_GATHER_DSS = 7
 OPT_PERSIST_ABORT_TEST = 8
+OPT_PARTICLE_WALK = 9
 AX_NVARIANTS = 8
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 NOALLOC_MAX_PARTICLES = 131072
@@ -45,6 +46,7 @@ EXPORTED_SYMBOLS = [
     "ssx_particles_step_host_noalloc", "ssx_batch_solve",
     "ssx_ns_create", "ssx_ns_destroy", "ssx_ns_init", "ssx_ns_step",
     "ssx_ns_field", "ssx_ns_read", "ssx_ns_time",
+    "ssx_particles_pass_counts",
 ]
 
 
@@ -188,6 +190,7 @@ def load():
                                       C.POINTER(C.c_int)]),
         "ssx_particles_step_host_noalloc": (
             i, [vp, C.POINTER(ParticlesStruct), d, d, C.POINTER(C.c_int)]),
+        "ssx_particles_pass_counts": (i, [vp, C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
