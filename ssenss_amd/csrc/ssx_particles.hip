@@ -695,14 +695,18 @@ k_particles_a(const PField F, long np, int pass, long coop_max,
 		double *__restrict__ t, const double *__restrict__ tau,
 		int *__restrict__ reinit_out, int *__restrict__ flags)
 {
+	// pass 0 (a grid the size of the list) in XCD-contiguous block order:
+	// each XCD walks one range of the sorted list, so an element's fields
+	// are fetched into one L2.  The later passes' grids outnumber their
+	// lists, and round-robin keeps every XCD busy.
+	const long lb = pass == 0 ? (long) xcd_block() : (long) blockIdx.x;
 	const long m = pass == 0 ? np : (long) ca[pass];
-	if ((long) blockIdx.x * 64 >= m || (pass > 0 && m <= coop_max))
+	if (lb * 64 >= m || (pass > 0 && m <= coop_max))
 		return;
 	if (threadIdx.x < F.n)
 		s_xi[threadIdx.x] = F.xi[threadIdx.x];
 	__syncthreads();
-	for (long i = (long) blockIdx.x * 64 + threadIdx.x; i < m;
-			i += (long) gridDim.x * 64) {
+	for (long i = lb * 64 + threadIdx.x; i < m; i += (long) gridDim.x * 64) {
 		const long q = in ? (long) in[i] : i;
 		bool more = false;
 		PState p;
@@ -745,14 +749,14 @@ k_particles_b(const PField F, int pass, long coop_max,
 		double *__restrict__ t, const double *__restrict__ tau,
 		int *__restrict__ reinit_out, int *__restrict__ flags)
 {
+	const long lb = blockIdx.x;
 	const long m = cm[pass];
-	if ((long) blockIdx.x * 64 >= m || (pass > 0 && ca[pass] <= coop_max))
+	if (lb * 64 >= m || (pass > 0 && ca[pass] <= coop_max))
 		return;
 	if (threadIdx.x < F.n)
 		s_xi[threadIdx.x] = F.xi[threadIdx.x];
 	__syncthreads();
-	for (long i = (long) blockIdx.x * 64 + threadIdx.x; i < m;
-			i += (long) gridDim.x * 64) {
+	for (long i = lb * 64 + threadIdx.x; i < m; i += (long) gridDim.x * 64) {
 		const long q = mid[i];
 		PState p;
 		p.el = el[q];
@@ -790,7 +794,7 @@ k_particles_step(const PField F, long n, const int *__restrict__ perm,
 	if (threadIdx.x < F.n)
 		s_xi[threadIdx.x] = F.xi[threadIdx.x];
 	__syncthreads();
-	const long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+	const long i = (long) xcd_block() * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
 	particle_walk<NF>(F, perm ? (long) perm[i] : i, el, old_el, x, y, z,
@@ -1016,7 +1020,7 @@ k_tracers_step(const PField F, long n, const int *__restrict__ perm,
 	if (threadIdx.x < F.n)
 		s_xi[threadIdx.x] = F.xi[threadIdx.x];
 	__syncthreads();
-	const long i0 = (long) blockIdx.x * blockDim.x + threadIdx.x;
+	const long i0 = (long) xcd_block() * blockDim.x + threadIdx.x;
 	if (i0 >= n)
 		return;
 	tracer_walk<NF>(F, perm ? (long) perm[i0] : i0, el, old_el, x, y, z,
@@ -1035,14 +1039,14 @@ k_tracers_pass(const PField F, long np, int pass, long coop_max,
 		double *__restrict__ t, int *__restrict__ reinit_out,
 		int *__restrict__ flags)
 {
+	const long lb = pass == 0 ? (long) xcd_block() : (long) blockIdx.x;
 	const long m = pass == 0 ? np : (long) cnt[pass];
-	if ((long) blockIdx.x * 64 >= m || (pass > 0 && m <= coop_max))
+	if (lb * 64 >= m || (pass > 0 && m <= coop_max))
 		return;
 	if (threadIdx.x < F.n)
 		s_xi[threadIdx.x] = F.xi[threadIdx.x];
 	__syncthreads();
-	for (long i = (long) blockIdx.x * 64 + threadIdx.x; i < m;
-			i += (long) gridDim.x * 64)
+	for (long i = lb * 64 + threadIdx.x; i < m; i += (long) gridDim.x * 64)
 		tracer_pass_one<NF>(F, pass, list ? (long) list[i] : i, next, cnt,
 				pst, el, old_el, x, y, z, u, v, w, t, reinit_out, flags);
 }
