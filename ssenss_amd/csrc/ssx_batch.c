@@ -222,24 +222,57 @@ static int fill_tables(ssx_batch *b, const unsigned char *active)
 	hipStream_t st = (hipStream_t) b->stream;
 	ssx_bwork *t = b->htab;
 	long long off = 0;
+	/* Problems on one mesh (same factors, same blocks: the velocity
+	 * components of a time step) take their operator blocks interleaved
+	 * in runs of 8: workgroup w runs on XCD w % 8, so block q of every such
+	 * problem lands on one XCD, close in time, and the element's factors
+	 * are read from HBM once for all of them. */
+	unsigned char *grouped = calloc((size_t) b->np, 1);
+	int *mem = malloc(sizeof(int) * (size_t) (b->np ? b->np : 1));
+	if (!grouped || !mem) {
+		free(grouped);
+		free(mem);
+		ssx_set_error("out of host memory (batch)");
+		return SSX_ENOMEM;
+	}
 	for (int k = 0; k < b->norder; ++k) {
 		long long m = 0;
+		memset(grouped, 0, (size_t) b->np);
 		for (int i = 0; i < b->np; ++i) {
 			const ssx_bprob *d = &b->h_prob[i];
-			if (!active[i] || d->n != b->order_n[k] || d->op != b->order_op[k])
+			if (!active[i] || grouped[i] || d->n != b->order_n[k]
+					|| d->op != b->order_op[k])
 				continue;
-			for (long long q = 0; q < d->nba; ++q) {
-				t[off + m].prob = i;
-				t[off + m].blk = (int) q;
-				++m;
+			int nm = 0;                 /* i's group: same n, op, mesh */
+			for (int j = i; j < b->np; ++j) {
+				const ssx_bprob *e = &b->h_prob[j];
+				if (active[j] && !grouped[j] && e->n == d->n
+						&& e->op == d->op && e->gf == d->gf
+						&& e->nba == d->nba) {
+					grouped[j] = 1;
+					mem[nm++] = j;
+				}
 			}
+			for (long long q0 = 0; q0 < d->nba; q0 += 8)
+				for (int g = 0; g < nm; ++g)
+					for (long long q = q0; q < q0 + 8 && q < d->nba; ++q) {
+						t[off + m].prob = mem[g];
+						t[off + m].blk = (int) q;
+						++m;
+					}
 		}
 		b->nax[k] = m;
-		if (m)
-			HIPB(hipMemcpyAsync(b->d_axtab[k], t + off,
-					sizeof(ssx_bwork) * m, hipMemcpyHostToDevice, st));
+		if (m && hipMemcpyAsync(b->d_axtab[k], t + off, sizeof(ssx_bwork) * m,
+				hipMemcpyHostToDevice, st) != hipSuccess) {
+			free(grouped);
+			free(mem);
+			ssx_set_error("hipMemcpyAsync of a batch work table failed");
+			return SSX_EHIP;
+		}
 		off += m;
 	}
+	free(grouped);
+	free(mem);
 	for (int pass = 0; pass < 2; ++pass) {
 		long long m = 0;
 		for (int i = 0; i < b->np; ++i) {

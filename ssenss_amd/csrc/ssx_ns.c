@@ -62,6 +62,10 @@ int ssxk_ns_conv(void *st, size_t nt, int n, const double *dm,
 		const double *const u[3], const double *const v[3],
 		const double *const Jv[9], const double *const ti[9],
 		const double *mass, double *const out[3]);
+size_t ssxk_ns_conv_full_bytes(int n, int nc);
+int ssxk_ns_conv_full(void *st, int nel, int n, int nc, const double *prol,
+		const double *dm, const double *const u[3], const double *const v[3],
+		const double *const ti[9], const double *mass, double *const out[3]);
 int ssxk_ns_surf(void *st, int nel, int n, const int *bc, int dirichlet,
 		const double *wn, const double *g, double *f);
 int ssxk_ns_zero_dir(void *st, size_t nt, int n, const int *bc,
@@ -89,6 +93,7 @@ struct ssx_ns {
 	double *rkf[4][3], *rky[4][3];
 	/* scratch at nconv */
 	double *cu[3], *cv[3], *cJ[9], *co[3];
+	int conv_elem;               /* convection one workgroup per element */
 	int inited;
 	long long cnt_solve[4];
 };
@@ -332,6 +337,17 @@ int ssx_ns_create(ssx_ns **out, ssx_ctx *c, const ssx_ns_params *prm)
 	for (int d = 0; d < 3; ++d)
 		ns->g[d] = prm->g[d];
 	ns->batched = prm->batch_velocity;
+	/* the whole convection operator one element per workgroup where its
+	 * LDS fits the device, else the transfer kernels and the node-per-lane
+	 * operator with the Jacobian of v stored (cJ) */
+	{
+		int dev = 0, lim = 0;
+		const size_t need = ssxk_ns_conv_full_bytes(n, nc);
+		ns->conv_elem = hipGetDevice(&dev) == hipSuccess
+			&& hipDeviceGetAttribute(&lim,
+					hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess
+			&& need <= (size_t) lim;
+	}
 	int rc = ns_geometry(ns);
 	const size_t nt = ns->nt, ntc = ns->ntc;
 	for (int l = 0; l < 3 && rc == SSX_OK; ++l)
@@ -346,7 +362,8 @@ int ssx_ns_create(ssx_ns **out, ssx_ctx *c, const ssx_ns_params *prm)
 		double **v[] = { &ns->p[d], &ns->ut[d], &ns->rhs[d], &ns->a[d],
 			&ns->b[d], &ns->cu[d], &ns->cv[d], &ns->co[d] };
 		const size_t sz[] = { nt, nt, nt, nt, nt, ntc, ntc, ntc };
-		for (int k = 0; k < 8 && rc == SSX_OK; ++k)
+		/* the nc-point fields only for the unfused convection */
+		for (int k = 0; k < (ns->conv_elem ? 5 : 8) && rc == SSX_OK; ++k)
 			rc = nalloc(v[k], sz[k]);
 		for (int s = 0; s < 4 && rc == SSX_OK; ++s) {
 			rc = nalloc(&ns->rkf[s][d], nt);
@@ -356,7 +373,7 @@ int ssx_ns_create(ssx_ns **out, ssx_ctx *c, const ssx_ns_params *prm)
 	}
 	for (int k = 0; k < 9 && rc == SSX_OK; ++k) {
 		rc = nalloc(&ns->J[k], nt);
-		if (rc == SSX_OK)
+		if (rc == SSX_OK && !ns->conv_elem)
 			rc = nalloc(&ns->cJ[k], ntc);
 	}
 	/* zero histories (the element constructor's zeroed arrays) */
@@ -404,6 +421,10 @@ static int ns_convection(ssx_ns *ns, double *const u[3], double *const v[3],
 {
 	void *st = ns->c->stream;
 	const int n = ns->n, nc = ns->nc, nel = ns->nel;
+	if (ns->conv_elem)          /* the whole operator per element */
+		return ssxk_ns_conv_full(st, nel, n, nc, ns->prol_c, ns->dm_c,
+				(const double *const *) u, (const double *const *) v,
+				(const double *const *) ns->ti_c, ns->mass_c, out);
 	for (int d = 0; d < 3; ++d) {
 		TRYN(ssxk_prolong(st, nc, n, nel, ns->prol_c, u[d], ns->cu[d], 0));
 		TRYN(ssxk_prolong(st, nc, n, nel, ns->prol_c, v[d], ns->cv[d], 0));

@@ -218,6 +218,275 @@ k_ns_conv(long long nt, int n, const double *__restrict__ dm, P3 u, P3 v,
 	}
 }
 
+// The same operator, one workgroup per element: the three u components
+// and one v component at a time are staged in LDS, so the reference
+// derivatives of v and of u_b v along every line come from LDS instead of
+// per-node global gathers, and the Jacobian of v is never stored.  Per
+// node and component the arithmetic is k_ns_conv's (dref's sums, then the
+// same combinations).  LDS: 4 n^3 + n^2 doubles (n <= 15).
+__global__ void __launch_bounds__(1024)
+k_ns_conv_elem(int n, const double *__restrict__ dm, P3 u, P3 v, P9 ti,
+		const double *__restrict__ mass, M3 out)
+{
+	extern __shared__ double sm[];
+	const int n2 = n * n, n3 = n2 * n;
+	double *su = sm;                    // u_0, u_1, u_2
+	double *sv = sm + 3 * n3;           // v_c
+	double *sd = sm + 4 * n3;           // D
+	const long long b0 = (long long) blockIdx.x * n3;
+	for (int q = threadIdx.x; q < n2; q += blockDim.x)
+		sd[q] = dm[q];
+	for (int a = 0; a < 3; ++a)
+		for (int q = threadIdx.x; q < n3; q += blockDim.x)
+			su[a * n3 + q] = u.p[a][b0 + q];
+	for (int c = 0; c < 3; ++c) {
+		__syncthreads();                // sv free (c > 0), su/sd ready
+		for (int q = threadIdx.x; q < n3; q += blockDim.x)
+			sv[q] = v.p[c][b0 + q];
+		__syncthreads();
+		for (int r = threadIdx.x; r < n3; r += blockDim.x) {
+			const int i = r % n, j = (r / n) % n, k = r / n2;
+			const long long q = b0 + r;
+			// D_b v_c, b = xi, eta, zeta (dref's order)
+			double jv[3] = { 0.0, 0.0, 0.0 };
+			double tb[3] = { 0.0, 0.0, 0.0 };
+			const double *rx = sv + (k * n + j) * n;
+			const double *ry = sv + k * n2 + i;
+			const double *rz = sv + j * n + i;
+			const double *ux = su + (k * n + j) * n;
+			const double *uy = su + n3 + k * n2 + i;
+			const double *uz = su + 2 * n3 + j * n + i;
+			for (int l = 0; l < n; ++l) {
+				jv[0] += sd[i * n + l] * rx[l];
+				jv[1] += sd[j * n + l] * ry[l * n];
+				jv[2] += sd[k * n + l] * rz[l * n2];
+				tb[0] += sd[i * n + l] * (ux[l] * rx[l]);
+				tb[1] += sd[j * n + l] * (uy[l * n] * ry[l * n]);
+				tb[2] += sd[k * n + l] * (uz[l * n2] * rz[l * n2]);
+			}
+			double tv[9];
+#pragma unroll
+			for (int a = 0; a < 9; ++a)
+				tv[a] = ti.p[a][q];
+			const double wj = mass[q];
+			double cv = 0.0;
+#pragma unroll
+			for (int a = 0; a < 3; ++a) {
+				double dv = 0.0;
+#pragma unroll
+				for (int b = 0; b < 3; ++b) {
+					const double t = tv[3 * a + b] * jv[b];
+					dv = b == 0 ? t : t + dv;
+				}
+				const double t = su[a * n3 + r] * dv;
+				cv = a == 0 ? t : t + cv;
+			}
+			cv = cv * wj;
+			double cs = 0.0;
+#pragma unroll
+			for (int a = 0; a < 3; ++a) {
+				double d = 0.0;
+#pragma unroll
+				for (int b = 0; b < 3; ++b) {
+					const double t = tv[3 * a + b] * tb[b];
+					d = b == 0 ? t : t + d;
+				}
+				cs = a == 0 ? d : cs + d;
+			}
+			cs = cs * wj;
+			out.p[c][q] = 0.5 * cs + 0.5 * cv;
+		}
+	}
+}
+
+// The whole dealiased operator of one element in one workgroup
+// (antialiased_convection_operator, convection.c:37-69): u and v are
+// interpolated from n to nc points into LDS (interpolate_to_grid, the three
+// contractions of k_prolong), the skew-symmetric operator is formed at the
+// nc points as in k_ns_conv_elem, and each component is integrated back to
+// n points (integrate_to_grid, the contractions of k_restrict).  Every sum
+// runs in the order of those kernels.  prol[p*nc+i] = l_p(xi_i) (p < n
+// coarse, i < nc fine).
+#define CONV_SCR(n, nc) ((n) * (n) * (n) + (n) * (n) * (nc) + (n) * (nc) * (nc))
+
+// dst[nc^3] = the interpolation to nc points of src[n^3] (global); scr:
+// CONV_SCR doubles
+__device__ __forceinline__ void conv_prolong(int n, int nc,
+		const double *__restrict__ sp, const double *__restrict__ src,
+		double *__restrict__ dst, double *__restrict__ scr)
+{
+	double *in = scr, *t1 = scr + n * n * n, *t2 = t1 + n * n * nc;
+	const int n3 = n * n * n;
+	for (int q = threadIdx.x; q < n3; q += blockDim.x)
+		in[q] = src[q];
+	__syncthreads();
+	for (int q = threadIdx.x; q < n * n * nc; q += blockDim.x) {
+		const int i = q % nc, kj = q / nc;
+		double s = 0.0;
+		for (int p = 0; p < n; ++p)
+			s += sp[p * nc + i] * in[kj * n + p];
+		t1[q] = s;
+	}
+	__syncthreads();
+	for (int q = threadIdx.x; q < n * nc * nc; q += blockDim.x) {
+		const int i = q % nc, jq = (q / nc) % nc, k = q / (nc * nc);
+		double s = 0.0;
+		for (int l = 0; l < n; ++l)
+			s += t1[(k * n + l) * nc + i] * sp[l * nc + jq];
+		t2[q] = s;
+	}
+	__syncthreads();
+	for (int q = threadIdx.x; q < nc * nc * nc; q += blockDim.x) {
+		const int ji = q % (nc * nc), k = q / (nc * nc);
+		double s = 0.0;
+		for (int l = 0; l < n; ++l)
+			s += t2[l * nc * nc + ji] * sp[l * nc + k];
+		dst[q] = s;
+	}
+	__syncthreads();
+}
+
+// dst[n^3] (global) = the integration to n points of src[nc^3] (LDS)
+__device__ __forceinline__ void conv_restrict(int n, int nc,
+		const double *__restrict__ sp, const double *__restrict__ src,
+		double *__restrict__ dst, double *__restrict__ scr)
+{
+	double *t1 = scr, *t2 = scr + nc * nc * n;
+	for (int q = threadIdx.x; q < nc * nc * n; q += blockDim.x) {
+		const int p = q % n, kj = q / n;
+		double s = 0.0;
+		for (int c = 0; c < nc; ++c)
+			s += sp[p * nc + c] * src[kj * nc + c];
+		t1[q] = s;
+	}
+	__syncthreads();
+	for (int q = threadIdx.x; q < nc * n * n; q += blockDim.x) {
+		const int i = q % n, jq = (q / n) % n, k = q / (n * n);
+		double s = 0.0;
+		for (int l = 0; l < nc; ++l)
+			s += t1[(k * nc + l) * n + i] * sp[jq * nc + l];
+		t2[q] = s;
+	}
+	__syncthreads();
+	for (int q = threadIdx.x; q < n * n * n; q += blockDim.x) {
+		const int ji = q % (n * n), k = q / (n * n);
+		double s = 0.0;
+		for (int l = 0; l < nc; ++l)
+			s += t2[l * n * n + ji] * sp[k * nc + l];
+		dst[q] = s;
+	}
+	__syncthreads();
+}
+
+#define CONV_PTS 4                  // nc^3 / 1024 rounded up (nc <= 15)
+
+// the skew-symmetric operator at node r of the element (k_ns_conv_elem's
+// arithmetic): su the three u components, sv v_c, sd D, all at nc points
+__device__ __forceinline__ double conv_point(int nc, int r,
+		const double *__restrict__ su, const double *__restrict__ sv,
+		const double *__restrict__ sd, const P9 &ti,
+		const double *__restrict__ mass, long long q)
+{
+	const int c2 = nc * nc, c3 = c2 * nc;
+	const int i = r % nc, j = (r / nc) % nc, k = r / c2;
+	double jv[3] = { 0.0, 0.0, 0.0 };
+	double tb[3] = { 0.0, 0.0, 0.0 };
+	const double *rx = sv + (k * nc + j) * nc;
+	const double *ry = sv + k * c2 + i;
+	const double *rz = sv + j * nc + i;
+	const double *ux = su + (k * nc + j) * nc;
+	const double *uy = su + c3 + k * c2 + i;
+	const double *uz = su + 2 * c3 + j * nc + i;
+	for (int l = 0; l < nc; ++l) {
+		jv[0] += sd[i * nc + l] * rx[l];
+		jv[1] += sd[j * nc + l] * ry[l * nc];
+		jv[2] += sd[k * nc + l] * rz[l * c2];
+		tb[0] += sd[i * nc + l] * (ux[l] * rx[l]);
+		tb[1] += sd[j * nc + l] * (uy[l * nc] * ry[l * nc]);
+		tb[2] += sd[k * nc + l] * (uz[l * c2] * rz[l * c2]);
+	}
+	double tv[9];
+#pragma unroll
+	for (int a = 0; a < 9; ++a)
+		tv[a] = ti.p[a][q];
+	const double wj = mass[q];
+	double cv = 0.0;
+#pragma unroll
+	for (int a = 0; a < 3; ++a) {
+		double dv = 0.0;
+#pragma unroll
+		for (int b = 0; b < 3; ++b) {
+			const double t = tv[3 * a + b] * jv[b];
+			dv = b == 0 ? t : t + dv;
+		}
+		const double t = su[a * c3 + r] * dv;
+		cv = a == 0 ? t : t + cv;
+	}
+	cv = cv * wj;
+	double cs = 0.0;
+#pragma unroll
+	for (int a = 0; a < 3; ++a) {
+		double d = 0.0;
+#pragma unroll
+		for (int b = 0; b < 3; ++b) {
+			const double t = tv[3 * a + b] * tb[b];
+			d = b == 0 ? t : t + d;
+		}
+		cs = a == 0 ? d : cs + d;
+	}
+	cs = cs * wj;
+	return 0.5 * cs + 0.5 * cv;
+}
+
+__global__ void __launch_bounds__(1024)
+k_ns_conv_full(int n, int nc, const double *__restrict__ prol,
+		const double *__restrict__ dm, P3 u, P3 v, P9 ti,
+		const double *__restrict__ mass, M3 out)
+{
+	extern __shared__ double sm[];
+	const int c2 = nc * nc, c3 = c2 * nc, n3 = n * n * n;
+	double *sp = sm;                    // n*nc
+	double *sd = sp + n * nc;           // nc^2
+	double *su = sd + c2;               // u_0, u_1, u_2 at nc
+	double *sv = su + 3 * c3;           // v_c at nc, then the result
+	double *scr = sv + c3;              // transfer scratch
+	const long long e = blockIdx.x;
+	for (int q = threadIdx.x; q < n * nc; q += blockDim.x)
+		sp[q] = prol[q];
+	for (int q = threadIdx.x; q < c2; q += blockDim.x)
+		sd[q] = dm[q];
+	__syncthreads();
+	for (int a = 0; a < 3; ++a)
+		conv_prolong(n, nc, sp, u.p[a] + e * n3, su + a * c3, scr);
+	const long long b0 = e * c3;
+	for (int c = 0; c < 3; ++c) {
+		conv_prolong(n, nc, sp, v.p[c] + e * n3, sv, scr);
+		// this lane's nodes r = threadIdx.x + 1024 m, results in named
+		// registers (no array indexed by m)
+		double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+#pragma unroll 1
+		for (int m = 0; m < CONV_PTS; ++m) {
+			const int r = threadIdx.x + m * 1024;
+			if (r >= c3)
+				break;
+			const double val = conv_point(nc, r, su, sv, sd, ti, mass, b0 + r);
+			r0 = m == 0 ? val : r0;
+			r1 = m == 1 ? val : r1;
+			r2 = m == 2 ? val : r2;
+			r3 = m == 3 ? val : r3;
+		}
+		__syncthreads();                // every read of sv is done
+#pragma unroll
+		for (int m = 0; m < CONV_PTS; ++m) {
+			const int r = threadIdx.x + m * 1024;
+			if (r < c3)
+				sv[r] = m == 0 ? r0 : m == 1 ? r1 : m == 2 ? r2 : r3;
+		}
+		__syncthreads();
+		conv_restrict(n, nc, sp, sv, out.p[c] + e * n3, scr);
+	}
+}
+
 // f += (w_b w_a) n_s g on the face nodes of every face s of an element
 // whose Dirichlet bit equals `dirichlet` (add_surface_integral,
 // poisson.c:145-210: all nodes of face W, then E, S, N, B, T).  One
@@ -393,6 +662,70 @@ extern "C" int ssxk_ns_conv(void *st, size_t nt, int n, const double *dm,
 			(hipStream_t) st, (long long) nt, n, dm, pu, pv, pj, pt, mass,
 			po);
 	NS_LAUNCH("k_ns_conv");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_ns_conv_elem(void *st, int nel, int n, const double *dm,
+		const double *const u[3], const double *const v[3],
+		const double *const ti[9], const double *mass, double *const out[3])
+{
+	const size_t bytes = sizeof(double) * (4 * (size_t) n * n * n
+			+ (size_t) n * n);
+	if (n < 2 || bytes > 160 * 1024) {
+		ssx_set_error("k_ns_conv_elem: n = %d does not fit LDS", n);
+		return SSX_EINVAL;
+	}
+	P3 pu = { { u[0], u[1], u[2] } }, pv = { { v[0], v[1], v[2] } };
+	P9 pt;
+	for (int k = 0; k < 9; ++k)
+		pt.p[k] = ti[k];
+	M3 po = { { out[0], out[1], out[2] } };
+	if (hipFuncSetAttribute((const void *) k_ns_conv_elem,
+			hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes)
+			!= hipSuccess) {
+		ssx_set_error("k_ns_conv_elem: %zu bytes of LDS refused", bytes);
+		return SSX_EHIP;
+	}
+	hipLaunchKernelGGL(k_ns_conv_elem, dim3(nel), dim3(1024), bytes,
+			(hipStream_t) st, n, dm, pu, pv, pt, mass, po);
+	NS_LAUNCH("k_ns_conv_elem");
+	return SSX_OK;
+}
+
+// LDS of k_ns_conv_full, bytes
+extern "C" size_t ssxk_ns_conv_full_bytes(int n, int nc)
+{
+	const size_t scr_p = CONV_SCR((size_t) n, (size_t) nc);
+	const size_t scr_r = (size_t) nc * nc * n + (size_t) nc * n * n;
+	return sizeof(double) * ((size_t) n * nc + (size_t) nc * nc
+			+ 4 * (size_t) nc * nc * nc + (scr_p > scr_r ? scr_p : scr_r));
+}
+
+extern "C" int ssxk_ns_conv_full(void *st, int nel, int n, int nc,
+		const double *prol, const double *dm, const double *const u[3],
+		const double *const v[3], const double *const ti[9],
+		const double *mass, double *const out[3])
+{
+	const size_t bytes = ssxk_ns_conv_full_bytes(n, nc);
+	if (n < 2 || nc < n || nc * nc * nc > CONV_PTS * 1024
+			|| bytes > 160 * 1024) {
+		ssx_set_error("k_ns_conv_full: n = %d, nc = %d unsupported", n, nc);
+		return SSX_EINVAL;
+	}
+	P3 pu = { { u[0], u[1], u[2] } }, pv = { { v[0], v[1], v[2] } };
+	P9 pt;
+	for (int k = 0; k < 9; ++k)
+		pt.p[k] = ti[k];
+	M3 po = { { out[0], out[1], out[2] } };
+	if (hipFuncSetAttribute((const void *) k_ns_conv_full,
+			hipFuncAttributeMaxDynamicSharedMemorySize, (int) bytes)
+			!= hipSuccess) {
+		ssx_set_error("k_ns_conv_full: %zu bytes of LDS refused", bytes);
+		return SSX_EHIP;
+	}
+	hipLaunchKernelGGL(k_ns_conv_full, dim3(nel), dim3(1024), bytes,
+			(hipStream_t) st, n, nc, prol, dm, pu, pv, pt, mass, po);
+	NS_LAUNCH("k_ns_conv_full");
 	return SSX_OK;
 }
 
