@@ -325,17 +325,19 @@ def ns_tgv_case(S, torch, stream, steps=10):
 
 
 def ns_channel_case(S, torch, stream, steps=3, ntracers=1_000_000,
-                    prec="fp32"):
+                    prec="fp32", coarse_direct=16384):
     """The channel configuration: 24x16x24 hexahedra at N = 9 (n = 10,
     9.2M nodes) on [0, 4 pi] x [-1, 1] x [0, 4 pi / 3], tanh wall
     stretching 2.0 in y, periodic in x and z, visc = 1/180 (Re_tau = 180
     with u_tau = h = 1), body force gx = 2 visc (the laminar profile's
     balance), dt = 2e-3; Poiseuille profile plus 10% divergence-free noise
     at all three history levels.  Each step: OIFS convection, FCG +
-    p-MG pressure solve with the FP32 smoother, the three velocity
-    Helmholtz solves as one batch, then the velocity history updated and
-    `ntracers` RK2 tracers advanced over the step.  The first step is a
-    warm-up (graph capture, reduced-precision copies)."""
+    p-MG pressure solve with the FP32 smoother (its coarsest level,
+    9792 unique nodes, solved directly when coarse_direct >= that:
+    SSX_OPT_COARSE_DIRECT), the three velocity Helmholtz solves as one
+    batch, then the velocity history updated and `ntracers` RK2 tracers
+    advanced over the step.  The first step is a warm-up (graph capture,
+    reduced-precision copies, the coarse inverse)."""
     from ssenss_amd.mesh import box_mesh, PERIODIC, NOSLIP
     visc, dt = 1 / 180, 2e-3
     lx, lz = 4 * np.pi, 4 * np.pi / 3
@@ -348,6 +350,7 @@ def ns_channel_case(S, torch, stream, steps=3, ntracers=1_000_000,
     tset = time.perf_counter() - t0
     code = {"fp64": S.PREC_FP64, "fp32": S.PREC_FP32, "bf16": S.PREC_BF16}
     ctx.set_option(S.OPT_SMOOTHER_PRECISION, code[prec])
+    ctx.set_option(S.OPT_COARSE_DIRECT, coarse_direct)
     u0 = channel_field(mesh, lx, lz, 0.1, 180)
     levels = []
     for _ in range(3):
@@ -389,6 +392,7 @@ def ns_channel_case(S, torch, stream, steps=3, ntracers=1_000_000,
         res = {"workload": "channel 24x16x24 hexahedra, N=9, tanh 2.0, "
                            "Re_tau 180, dt 2e-3", "local_dofs": mesh.ntot,
                "context_setup_s": tset, "pressure_smoother": prec,
+               "coarse_direct_max_unknowns": coarse_direct,
                "steps": steps, "seconds_per_step_first": ts[0],
                "seconds_per_step": float(np.median(ts[1:])),
                "iterations_per_step": its[-1],

@@ -220,6 +220,15 @@ int ssx_set_diag_choice(ssx_ctx *ctx, int choice);
  * particles still moving compacted between passes; 0 the whole walk of
  * each particle in one lane of one launch.  Bit-identical results. */
 #define SSX_OPT_PARTICLE_WALK 9
+/* SSX_OPT_COARSE_DIRECT: m > 0 solves the p-multigrid coarsest level
+ * directly, with its assembled operator inverted once (dense, rocSOLVER
+ * Cholesky), when it has at most m unique nodes and the smoother runs in
+ * reduced precision (SSX_OPT_SMOOTHER_PRECISION != FP64, whose coarse solve
+ * stops at a relative tolerance anyway).  The inverse takes 8 m^2 bytes of
+ * device memory (m = 16384: 2 GiB).  0 (default): Jacobi-PCG as the
+ * reference does.  A coarse operator that is not positive definite (other
+ * than the pressure's constant null space) falls back to the CG. */
+#define SSX_OPT_COARSE_DIRECT 10
 #define SSX_PREC_FP64 0
 #define SSX_PREC_FP32 1
 #define SSX_PREC_BF16 2

@@ -572,6 +572,7 @@ void ssx_destroy(ssx_ctx *c)
 	hipFree(c->p_bucket);
 	hipFree(c->p_work);
 	hipFree(c->p_pcnt);
+	ssx_coarse_direct_free(c);
 	free(c->h_neigh);
 	hipFree(c->d_neigh);
 	for (int a = 0; a < 9; ++a) {
@@ -651,6 +652,14 @@ int ssx_set_option(ssx_ctx *c, int option, int value)
 		return SSX_OK;
 	case SSX_OPT_PERSIST_ABORT_TEST:
 		c->persist_abort_test = value != 0;
+		return SSX_OK;
+	case SSX_OPT_COARSE_DIRECT:
+		if (value < 0 || value > 46340) {
+			ssx_set_error("direct coarse solve: %d unknowns out of range",
+					value);
+			return SSX_EINVAL;
+		}
+		c->coarse_direct = value;
 		return SSX_OK;
 	case SSX_OPT_PARTICLE_WALK:
 		if (value != 0 && value != 1) {

@@ -26,6 +26,8 @@
 #include <string.h>
 
 #include <hip/hip_runtime_api.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include "ssx_internal.h"
 
@@ -321,6 +323,115 @@ static int unull(ssx_ctx *c, double *x)
 			(double) cs->nu, x);
 }
 
+/* ---- direct coarsest solve (SSX_OPT_COARSE_DIRECT) ----
+ * The assembled operator, masked rows and columns replaced by the identity
+ * (and for a singular one, the pressure or an all-periodic Poisson level,
+ * the rank-one term <1,.>/nu added, so that the inverse maps zero-mean
+ * data to the zero-mean solution), is factored by
+ * rocSOLVER's Cholesky and inverted once per (operator, lambda, null
+ * space); a solve is then one dense product over the unique nodes. */
+void ssx_coarse_direct_free(ssx_ctx *c)
+{
+	hipFree(c->cd_inv);
+	c->cd_inv = NULL;
+	c->cd_n = 0;
+	if (c->cd_blas)
+		rocblas_destroy_handle((rocblas_handle) c->cd_blas);
+	c->cd_blas = NULL;
+}
+
+static int coarse_direct_build(ssx_ctx *c, int l, int op, int pres)
+{
+	ssx_coarse *cs = &c->lev[l].cs;
+	if (c->cd_inv && c->cd_n == cs->nu && c->cd_op == op
+			&& c->cd_pres == pres && c->cd_lambda == c->lambda)
+		return c->cd_bad ? SSX_ENUMERIC : SSX_OK;
+	const int nu = cs->nu;
+	hipStream_t st = (hipStream_t) c->stream;
+	if (c->cd_n != nu) {
+		hipFree(c->cd_inv);
+		c->cd_inv = NULL;
+		c->cd_n = 0;
+		if (hipMalloc((void **) &c->cd_inv, sizeof(double) * (size_t) nu
+				* nu) != hipSuccess) {
+			c->cd_inv = NULL;
+			ssx_set_error("hipMalloc of the %d x %d coarse inverse", nu, nu);
+			return SSX_ENOMEM;
+		}
+		c->cd_n = nu;
+	}
+	if (!c->cd_blas) {
+		rocblas_handle h;
+		if (rocblas_create_handle(&h) != rocblas_status_success) {
+			ssx_set_error("rocblas_create_handle failed");
+			return SSX_EHIP;
+		}
+		c->cd_blas = h;
+	}
+	rocblas_handle h = (rocblas_handle) c->cd_blas;
+	if (rocblas_set_stream(h, st) != rocblas_status_success) {
+		ssx_set_error("rocblas_set_stream failed");
+		return SSX_EHIP;
+	}
+	c->cd_op = op;
+	c->cd_pres = pres;
+	c->cd_lambda = c->lambda;
+	c->cd_bad = 1;                  /* until the factorisation succeeds */
+	/* singular: the pressure, and the Poisson operator of a level with no
+	 * Dirichlet node (all periodic), both with the constants as null
+	 * space */
+	int singular = pres;
+	if (!pres && op == SSX_OP_POISSON) {
+		TRYQ(udot(c, cs->mask, cs->ones, SC_TMP));
+		double msum = 0.0;
+		HIPQ(hipMemcpyAsync(&msum, c->scal + SC_TMP, sizeof(double),
+				hipMemcpyDeviceToHost, st));
+		HIPQ(hipStreamSynchronize(st));
+		singular = msum >= (double) nu - 0.5;
+	}
+	c->cd_sing = singular;
+	const double *val = op == SSX_OP_HELMHOLTZ ? cs->valh : cs->valp;
+	TRYQ(ssxk_csr_dense(st, nu, cs->rowptr, cs->col, val,
+			pres ? NULL : cs->mask, singular ? 1.0 / nu : 0.0, c->cd_inv));
+	/* the level's CG flags (its CG is what this replaces) hold the infos */
+	rocblas_int *info = (rocblas_int *) (c->cg_flags + 2 * l);
+	rocblas_int hinfo[2] = { 0, 0 };
+	if (rocsolver_dpotrf(h, rocblas_fill_lower, nu, c->cd_inv, nu, info)
+			!= rocblas_status_success
+			|| rocsolver_dpotri(h, rocblas_fill_lower, nu, c->cd_inv, nu,
+					info + 1) != rocblas_status_success) {
+		ssx_set_error("rocSOLVER Cholesky of the coarse operator failed");
+		return SSX_EHIP;
+	}
+	HIPQ(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost,
+			st));
+	HIPQ(hipStreamSynchronize(st));
+	if (hinfo[0] != 0 || hinfo[1] != 0)
+		return SSX_ENUMERIC;        /* not definite: keep the CG */
+	TRYQ(ssxk_dense_sym(st, nu, c->cd_inv));
+	c->cd_bad = 0;
+	return SSX_OK;
+}
+
+/* x = A^-1 f over the unique nodes (the right-hand side masked, and for
+ * a singular operator made zero-mean first) */
+static int coarse_direct_solve(ssx_ctx *c, ssx_level *L, int pres,
+		const double *f, double *x)
+{
+	ssx_coarse *cs = &L->cs;
+	void *st = c->stream;
+	const int nu = cs->nu;
+	TRYQ(ssxk_gather_u(st, nu, cs->rep, f, cs->f));
+	if (!pres)
+		TRYQ(ssxk_mul(st, (size_t) nu, cs->f, cs->mask, cs->f));
+	if (c->cd_sing)
+		TRYQ(unull(c, cs->f));
+	TRYQ(ssxk_dense_mv(st, nu, c->cd_inv, cs->f, cs->x));
+	if (c->cd_sing)
+		TRYQ(unull(c, cs->x));
+	return ssxk_scatter_u(st, L->ntot, cs->cmap, cs->x, x);
+}
+
 int ssx_coarse_cg(ssx_ctx *c, int l, int op, int var, const double *f,
 		double *x, double eps, double rtol, int max_iter, int x_zero,
 		int *iters, double *err_out)
@@ -330,6 +441,22 @@ int ssx_coarse_cg(ssx_ctx *c, int l, int op, int var, const double *f,
 	if (cs->nu <= 0) {
 		ssx_set_error("level %d has no assembled operator", l);
 		return SSX_EINVAL;
+	}
+	/* the direct solve replaces the CG of the reduced-precision cycle (a
+	 * relative tolerance, a zero start) on small levels */
+	if (c->coarse_direct > 0 && cs->nu <= c->coarse_direct && rtol > 0.0
+			&& x_zero) {
+		const int pres = var == SSX_PRESSURE;
+		const int rc = coarse_direct_build(c, l, op, pres);
+		if (rc == SSX_OK) {
+			TRYQ(coarse_direct_solve(c, L, pres, f, x));
+			if (iters)
+				*iters = 1;
+			*err_out = 0.0;
+			return SSX_OK;
+		}
+		if (rc != SSX_ENUMERIC)
+			return rc;
 	}
 	void *st = c->stream;
 	const int nu = cs->nu;

@@ -141,6 +141,11 @@ struct ssx_ctx {
 	int *p_work, *p_pcnt;        /* pass lists and state (5 x cap), counts */
 	long p_perm_cap;
 	int p_elem_walk;             /* SSX_OPT_PARTICLE_WALK                 */
+	int coarse_direct;           /* SSX_OPT_COARSE_DIRECT: max unknowns   */
+	double *cd_inv;              /* dense inverse of the coarsest operator */
+	int cd_n, cd_op, cd_pres, cd_bad, cd_sing;
+	double cd_lambda;
+	void *cd_blas;               /* rocblas handle of the factorisation   */
 	long p_pass_np;              /* particles of the last pass-by-pass step */
 	long long cnt[4];
 	double ax_us[SSX_LMAX][3][SSX_AX_NVARIANTS];  /* autotuner timings  */
@@ -150,6 +155,8 @@ struct ssx_ctx {
 int ssx_coarse_build(ssx_ctx *c, int l, const double *gf, const double *mass,
 		const double *dp, const double *dh, const double *mask);
 void ssx_coarse_free(ssx_coarse *cs);
+/* frees the direct coarse solve's inverse and rocblas handle */
+void ssx_coarse_direct_free(ssx_ctx *c);
 /* Jacobi-PCG (solver.c:33-81) of level l in unique-node space: f, x
  * E-vectors of the level (f averaged over copies; x_zero: x is zero on
  * entry); stops at max(eps, rtol |r0|) or max_iter */
@@ -167,6 +174,11 @@ int ssxk_csr_pupd(void *stream, int nu, const int *rowptr, const int *col,
 		const double *z, const double *mask, const double *scal, int ib,
 		int ic, int first, double *ap, double *part, const int *done);
 int ssxk_csr_blocks(int nu);
+int ssxk_csr_dense(void *stream, int nu, const int *rowptr, const int *col,
+		const double *val, const double *mask, double reg, double *A);
+int ssxk_dense_sym(void *stream, int nu, double *A);
+int ssxk_dense_mv(void *stream, int nu, const double *A, const double *x,
+		double *y);
 /* Two-launch coarse CG iteration j (0-based) over unique nodes.
  * ssxk_csr_cg_spmv: every workgroup first replays the stopping test and
  * beta of iteration j-1 from the update partials pu (nbu per quantity,

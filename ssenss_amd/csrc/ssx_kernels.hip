@@ -1399,6 +1399,89 @@ extern "C" int ssxk_scatter_u(void *stream, size_t ntot, const int *cmap,
 	return SSX_OK;
 }
 
+// ---------------------------------------------------------------------
+// Dense coarse operator and its inverse (the direct coarsest solve)
+// ---------------------------------------------------------------------
+
+// A (nu x nu, row i at A + i*nu) = the CSR operator with masked rows and
+// columns replaced by the identity; reg > 0 adds reg to every entry (the
+// rank-one term that makes a singular pressure operator definite)
+__global__ void __launch_bounds__(256)
+k_csr_dense(int nu, const int *__restrict__ rowptr,
+		const int *__restrict__ col, const double *__restrict__ val,
+		const double *__restrict__ mask, double reg, double *__restrict__ A)
+{
+	const int i = blockIdx.x;
+	double *row = A + (size_t) i * nu;
+	for (int j = threadIdx.x; j < nu; j += 256)
+		row[j] = reg;
+	__syncthreads();
+	const bool mi = mask && mask[i] == 0.0;
+	if (mi) {
+		if (threadIdx.x == 0)
+			row[i] = 1.0;
+		return;
+	}
+	for (int k = rowptr[i] + threadIdx.x; k < rowptr[i + 1]; k += 256) {
+		const int j = col[k];
+		if (!(mask && mask[j] == 0.0))
+			row[j] = val[k] + reg;
+	}
+}
+
+// the lower triangle from the upper one: rocSOLVER's potri leaves the
+// inverse in the lower triangle of the column-major matrix, which is the
+// upper triangle of this row-major one
+__global__ void __launch_bounds__(256)
+k_dense_sym(int nu, double *__restrict__ A)
+{
+	const int i = blockIdx.x;
+	for (int j = i + 1 + threadIdx.x; j < nu; j += 256)
+		A[(size_t) j * nu + i] = A[(size_t) i * nu + j];
+}
+
+// y = A x, one workgroup per row, a fixed tree per row (deterministic)
+__global__ void __launch_bounds__(256)
+k_dense_mv(int nu, const double *__restrict__ A, const double *__restrict__ x,
+		double *__restrict__ y)
+{
+	__shared__ double sh[256];
+	const double *row = A + (size_t) blockIdx.x * nu;
+	double s = 0.0;
+	for (int j = threadIdx.x; j < nu; j += 256)
+		s += row[j] * x[j];
+	const double t = block_sum256(s, sh);
+	if (threadIdx.x == 0)
+		y[blockIdx.x] = t;
+}
+
+extern "C" int ssxk_csr_dense(void *stream, int nu, const int *rowptr,
+		const int *col, const double *val, const double *mask, double reg,
+		double *A)
+{
+	hipLaunchKernelGGL(k_csr_dense, dim3(nu), dim3(256), 0,
+			(hipStream_t) stream, nu, rowptr, col, val, mask, reg, A);
+	LAUNCH_CHECK("k_csr_dense");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_dense_sym(void *stream, int nu, double *A)
+{
+	hipLaunchKernelGGL(k_dense_sym, dim3(nu), dim3(256), 0,
+			(hipStream_t) stream, nu, A);
+	LAUNCH_CHECK("k_dense_sym");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_dense_mv(void *stream, int nu, const double *A,
+		const double *x, double *y)
+{
+	hipLaunchKernelGGL(k_dense_mv, dim3(nu), dim3(256), 0,
+			(hipStream_t) stream, nu, A, x, y);
+	LAUNCH_CHECK("k_dense_mv");
+	return SSX_OK;
+}
+
 extern "C" int ssxk_csr_pupd(void *stream, int nu, const int *rowptr,
 		const int *col, const double *val, const double *pold, double *pnew,
 		const double *z, const double *mask, const double *scal, int ib,
