@@ -43,6 +43,21 @@ static inline int ssx_ax_teams(int n, int v)
 	return per / n2 > 0 ? per / n2 : 1;
 }
 
+// the six factors of one node, q = plane * N^2 + lane, as stored (the
+// conversion of narrow factors is left to their use)
+template <bool NT, int N3, typename G>
+__device__ __forceinline__ void ax_factors(const G *__restrict__ g, int q,
+		G f[6])
+{
+#pragma unroll
+	for (int a = 0; a < 6; ++a) {
+		if constexpr (NT)
+			f[a] = __builtin_nontemporal_load(g + a * N3 + q);
+		else
+			f[a] = g[a * N3 + q];
+	}
+}
+
 // the CG direction update fused into the operator (PUPD): p = (z - c) +
 // beta p (masked), with beta at scal[ib] and c at scal[ic] (ic < 0: none)
 struct PUpd {
@@ -99,35 +114,50 @@ __device__ __forceinline__ double ax_team(int eblk, int nel,
 	const int bits = apply_mask ? eb : 0;
 	const bool mi = ((bits & 1) && i == 0) || ((bits & 2) && i == N - 1)
 		|| ((bits & 4) && j == 0) || ((bits & 8) && j == N - 1);
+	// Every load of the element is issued before any is used: the address
+	// of an idle lane (e >= nel) is clamped to element 0, so the loads need
+	// no branch, and a branch around each would make the compiler wait for
+	// each load before the next one goes out.  Stores stay behind act.
 	T uc[N], acc[N];
 	if (PU >= 1) {
 		const T bb = (T) scal[ib];
 		const T cc = ic >= 0 ? (T) -scal[ic] : T(0);
 		const T aa = PU == 2 ? (T) scal[ia] : T(0);
+		T po[N], zz[N], uo[PU == 2 ? N : 1];
 #pragma unroll
 		for (int k = 0; k < N; ++k) {
-			T v = T(0);
+			const size_t q = eoff + k * N2 + tid;
+			po[k] = pio[q];
+			zz[k] = zin[q];
+			if (PU == 2)
+				uo[k] = uio[q];
+		}
+#pragma unroll
+		for (int k = 0; k < N; ++k) {
+			const size_t q = eoff + k * N2 + tid;
+			const T zv = ic >= 0 ? cc * T(1) + zz[k] : zz[k];
+			T v = bb * po[k] + zv;
+			const bool m = mi || ((bits & 16) && k == 0)
+				|| ((bits & 32) && k == N - 1);
+			if (apply_mask)
+				v = v * (m ? T(0) : T(1));
 			if (act) {
-				const size_t q = eoff + k * N2 + tid;
-				const T po = pio[q];
 				if (PU == 2)
-					uio[q] = aa * po + uio[q];
-				const T zv = ic >= 0 ? cc * T(1) + zin[q] : zin[q];
-				v = bb * po + zv;
-				const bool m = mi || ((bits & 16) && k == 0)
-					|| ((bits & 32) && k == N - 1);
-				if (apply_mask)
-					v = v * (m ? T(0) : T(1));
+					uio[q] = aa * po[k] + uo[PU == 2 ? k : 0];
 				pio[q] = v;
 			}
+			v = act ? v : T(0);
 			uc[k] = v;
 			su[(k * N + j) * P + i] = v;
 			acc[k] = T(0);
 		}
 	} else {
 #pragma unroll
+		for (int k = 0; k < N; ++k)
+			uc[k] = u[eoff + k * N2 + tid];
+#pragma unroll
 		for (int k = 0; k < N; ++k) {
-			uc[k] = act ? u[eoff + k * N2 + tid] : T(0);
+			uc[k] = act ? uc[k] : T(0);
 			su[(k * N + j) * P + i] = uc[k];
 			acc[k] = T(0);
 		}
@@ -147,8 +177,9 @@ __device__ __forceinline__ double ax_team(int eblk, int nel,
 		}
 	}
 
-#pragma unroll 1
-	for (int k = 0; k < N; ++k) {
+	// One plane of the element: the factors of the plane are the lane's
+	// registers fc, loaded one plane ahead (below).
+	auto plane_step = [&](int k, const G *fc) {
 		T gx = T(0), gy = T(0), gz = T(0);
 		const T *plane = su + k * N * P;
 #pragma unroll
@@ -157,26 +188,14 @@ __device__ __forceinline__ double ax_team(int eblk, int nel,
 			gy += dyf[l] * plane[l * P + i];
 			gz += gDn[k * N + l] * uc[l];      // uniform: scalar loads
 		}
-		T t0 = T(0), t1 = T(0), t2 = T(0);
-		if (act) {
-			const int q = k * N2 + tid;
-			T gxx, gxy, gxz, gyy, gyz, gzz;
-			if (NT) {
-				gxx = ldv(__builtin_nontemporal_load(g + q));
-				gxy = ldv(__builtin_nontemporal_load(g + N3 + q));
-				gxz = ldv(__builtin_nontemporal_load(g + 2 * N3 + q));
-				gyy = ldv(__builtin_nontemporal_load(g + 3 * N3 + q));
-				gyz = ldv(__builtin_nontemporal_load(g + 4 * N3 + q));
-				gzz = ldv(__builtin_nontemporal_load(g + 5 * N3 + q));
-			} else {
-				gxx = ldg(g, q); gxy = ldg(g, N3 + q);
-				gxz = ldg(g, 2 * N3 + q); gyy = ldg(g, 3 * N3 + q);
-				gyz = ldg(g, 4 * N3 + q); gzz = ldg(g, 5 * N3 + q);
-			}
-			t0 = gxx * gx + gxy * gy + gxz * gz;
-			t1 = gxy * gx + gyy * gy + gyz * gz;
-			t2 = gxz * gx + gyz * gy + gzz * gz;
-		}
+		const T gxx = ldv(fc[0]), gxy = ldv(fc[1]), gxz = ldv(fc[2]);
+		const T gyy = ldv(fc[3]), gyz = ldv(fc[4]), gzz = ldv(fc[5]);
+		T t0 = gxx * gx + gxy * gy + gxz * gz;
+		T t1 = gxy * gx + gyy * gy + gyz * gz;
+		T t2 = gxz * gx + gyz * gy + gzz * gz;
+		t0 = act ? t0 : T(0);
+		t1 = act ? t1 : T(0);
+		t2 = act ? t2 : T(0);
 		sa[j * P + i] = t0;
 		sb[j * P + i] = t1;
 		__syncthreads();
@@ -191,6 +210,27 @@ __device__ __forceinline__ double ax_team(int eblk, int nel,
 		for (int kk = 0; kk < N; ++kk)
 			acc[kk] += (kk == k ? r : T(0)) + gDs[k * N + kk] * t2;
 		__syncthreads();
+	};
+	// The six factors of plane k + 1 are loaded before plane k is computed,
+	// so two planes' loads are in flight per lane instead of one.  The loop
+	// takes two planes a turn with two register sets, so no loaded value is
+	// copied (a copy would wait for its load); a load past the last plane
+	// reads the last plane again, unused.
+	G fa[6], fb[6];
+	ax_factors<NT, N3>(g, tid, fa);
+#pragma unroll 1
+	for (int k = 0; k < N; k += 2) {
+		ax_factors<NT, N3>(g, (k + 1 < N ? k + 1 : N - 1) * N2 + tid, fb);
+		plane_step(k, fa);
+		ax_factors<NT, N3>(g, (k + 2 < N ? k + 2 : N - 1) * N2 + tid, fa);
+		if (k + 1 < N)
+			plane_step(k + 1, fb);
+	}
+	T ms[OP == 1 ? N : 1];
+	if (OP == 1) {
+#pragma unroll
+		for (int k = 0; k < N; ++k)
+			ms[k] = mass[eoff + k * N2 + tid];
 	}
 	double pd = 0.0;
 	if (act) {
@@ -199,7 +239,7 @@ __device__ __forceinline__ double ax_team(int eblk, int nel,
 			const int q = k * N2 + tid;
 			T v = acc[k];
 			if (OP == 1)
-				v += lambda * (mass[eoff + q] * uc[k]);
+				v += lambda * (ms[OP == 1 ? k : 0] * uc[k]);
 			const bool m = mi || ((bits & 16) && k == 0)
 				|| ((bits & 32) && k == N - 1);
 			v = m ? T(0) : v;

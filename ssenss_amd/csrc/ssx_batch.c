@@ -175,6 +175,26 @@ static int build(ssx_batch *b)
 		const int teams = ssxk_bax_teams(n, b->order_v[k]);
 		b->h_prob[i].nba = (b->h_prob[i].nel + teams - 1) / teams;
 	}
+	/* An order whose problems share a mesh (the velocity components of a
+	 * time step) has its operator blocks interleaved so that the factors
+	 * read for one problem are found in L2 by the others: its factor loads
+	 * and result stores must not be non-temporal (launch-shape bit 1, which
+	 * the single-problem autotuner may have chosen).  The other bits keep
+	 * the block count, and every shape gives the same bits. */
+	for (int k = 0; k < b->norder; ++k) {
+		int shared = 0;
+		for (int i = 0; i < np && !shared; ++i) {
+			const ssx_bprob *d = &b->h_prob[i];
+			if (d->n != b->order_n[k] || d->op != b->order_op[k])
+				continue;
+			for (int j = i + 1; j < np && !shared; ++j) {
+				const ssx_bprob *e = &b->h_prob[j];
+				shared = e->n == d->n && e->op == d->op && e->gf == d->gf;
+			}
+		}
+		if (shared)
+			b->order_v[k] &= ~2;
+	}
 	int rc = SSX_OK;
 	for (int i = 0; i < np; ++i) {
 		const ssx_bprob *d = &b->h_prob[i];
