@@ -1017,10 +1017,248 @@ k_prolong(int nf, int nc, const T *__restrict__ prol,
 	}
 }
 
+// The same transfers with one wavefront per element and the sizes known at
+// compile time, for the p-multigrid's own level pairs (nc = (nf-1)/2 + 1).
+// Every load of the element is issued before the first product (the
+// accumulated fine values into registers, indexes clamped to the element,
+// so no branch separates them), the three stages are separated by wave
+// barriers only, and a block holds four elements.  Each output is the same
+// sum in the same order as above, so results are identical bit for bit.
+__device__ __forceinline__ void wave_sync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// a block barrier for LDS data only: loads of global memory still in
+// flight stay in flight across it
+__device__ __forceinline__ void lds_sync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+	__builtin_amdgcn_s_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+#define XW 4                            // elements (wavefronts) per block
+
+template <typename T, int NF, int NC, bool ACC>
+__global__ void __launch_bounds__(64 * XW)
+k_prolong_w(int nel, const T *__restrict__ prol, const T *__restrict__ coarse,
+		T *__restrict__ fine)
+{
+	constexpr int F3 = NF * NF * NF, C3 = NC * NC * NC;
+	constexpr int S1 = NC * NC * NF, S2 = NC * NF * NF;
+	constexpr int PC = (C3 + 63) / 64, PF = (F3 + 63) / 64;
+	constexpr int WS = C3 + S1 + S2;
+	extern __shared__ char smraw[];
+	T *sp = (T *) smraw;                  // nc*nf
+	const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	T *su = sp + NC * NF + wv * WS, *t1 = su + C3, *t2 = t1 + S1;
+	const int e = blockIdx.x * XW + wv;
+	const bool act = e < nel;
+	const size_t ec = (size_t) (act ? e : 0);
+	static_assert(NC * NF <= 64 * XW, "one matrix entry per lane");
+	// the matrix by uniform (scalar) loads, each lane keeping its entry: its
+	// LDS store then waits for no vector load of the element
+	T pv = T(0);
+#pragma unroll
+	for (int q = 0; q < NC * NF; ++q) {
+		const T m = prol[q];
+		pv = (int) threadIdx.x == q ? m : pv;
+	}
+	T cv[PC], fv[PF];
+#pragma unroll
+	for (int m = 0; m < PC; ++m) {
+		const int q = lane + 64 * m;
+		cv[m] = coarse[ec * C3 + (q < C3 ? q : C3 - 1)];
+	}
+	if constexpr (ACC) {
+#pragma unroll
+		for (int m = 0; m < PF; ++m) {
+			const int q = lane + 64 * m;
+			fv[m] = fine[ec * F3 + (q < F3 ? q : F3 - 1)];
+		}
+	}
+	if (threadIdx.x < NC * NF)
+		sp[threadIdx.x] = pv;
+#pragma unroll
+	for (int m = 0; m < PC; ++m) {
+		const int q = lane + 64 * m;
+		if (q < C3)
+			su[q] = cv[m];
+	}
+	lds_sync();
+#pragma unroll
+	for (int m = 0; m < (S1 + 63) / 64; ++m) {
+		const int q = lane + 64 * m;
+		if (q < S1) {
+			const int i = q % NF, kj = q / NF;
+			T s = T(0);
+#pragma unroll
+			for (int p = 0; p < NC; ++p)
+				s += sp[p * NF + i] * su[kj * NC + p];
+			t1[q] = s;
+		}
+	}
+	wave_sync();
+#pragma unroll
+	for (int m = 0; m < (S2 + 63) / 64; ++m) {
+		const int q = lane + 64 * m;
+		if (q < S2) {
+			const int i = q % NF, jq = (q / NF) % NF, k = q / (NF * NF);
+			T s = T(0);
+#pragma unroll
+			for (int l = 0; l < NC; ++l)
+				s += t1[(k * NC + l) * NF + i] * sp[l * NF + jq];
+			t2[q] = s;
+		}
+	}
+	wave_sync();
+#pragma unroll
+	for (int m = 0; m < PF; ++m) {
+		const int q = lane + 64 * m;
+		if (q < F3) {
+			const int ji = q % (NF * NF), k = q / (NF * NF);
+			T s = T(0);
+#pragma unroll
+			for (int l = 0; l < NC; ++l)
+				s += t2[l * NF * NF + ji] * sp[l * NF + k];
+			if (act)
+				fine[ec * F3 + q] = ACC ? s + fv[m] : s;
+		}
+	}
+}
+
+template <typename T, int NF, int NC>
+__global__ void __launch_bounds__(64 * XW)
+k_restrict_w(int nel, const T *__restrict__ prol, const T *__restrict__ fine,
+		T *__restrict__ coarse)
+{
+	constexpr int F3 = NF * NF * NF, C3 = NC * NC * NC;
+	constexpr int S1 = NF * NF * NC, S2 = NF * NC * NC;
+	constexpr int PF = (F3 + 63) / 64;
+	constexpr int WS = F3 + S1 + S2;
+	extern __shared__ char smraw[];
+	T *sp = (T *) smraw;                  // nc*nf
+	const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	T *su = sp + NC * NF + wv * WS, *t1 = su + F3, *t2 = t1 + S1;
+	const int e = blockIdx.x * XW + wv;
+	const bool act = e < nel;
+	const size_t ec = (size_t) (act ? e : 0);
+	static_assert(NC * NF <= 64 * XW, "one matrix entry per lane");
+	// the matrix by uniform (scalar) loads, each lane keeping its entry: its
+	// LDS store then waits for no vector load of the element
+	T pv = T(0);
+#pragma unroll
+	for (int q = 0; q < NC * NF; ++q) {
+		const T m = prol[q];
+		pv = (int) threadIdx.x == q ? m : pv;
+	}
+	T fv[PF];
+#pragma unroll
+	for (int m = 0; m < PF; ++m) {
+		const int q = lane + 64 * m;
+		fv[m] = fine[ec * F3 + (q < F3 ? q : F3 - 1)];
+	}
+	if (threadIdx.x < NC * NF)
+		sp[threadIdx.x] = pv;
+#pragma unroll
+	for (int m = 0; m < PF; ++m) {
+		const int q = lane + 64 * m;
+		if (q < F3)
+			su[q] = fv[m];
+	}
+	lds_sync();
+#pragma unroll
+	for (int m = 0; m < (S1 + 63) / 64; ++m) {
+		const int q = lane + 64 * m;
+		if (q < S1) {
+			const int p = q % NC, kj = q / NC;
+			T s = T(0);
+#pragma unroll
+			for (int c = 0; c < NF; ++c)
+				s += sp[p * NF + c] * su[kj * NF + c];
+			t1[q] = s;
+		}
+	}
+	wave_sync();
+#pragma unroll
+	for (int m = 0; m < (S2 + 63) / 64; ++m) {
+		const int q = lane + 64 * m;
+		if (q < S2) {
+			const int i = q % NC, jq = (q / NC) % NC, k = q / (NC * NC);
+			T s = T(0);
+#pragma unroll
+			for (int l = 0; l < NF; ++l)
+				s += t1[(k * NF + l) * NC + i] * sp[jq * NF + l];
+			t2[q] = s;
+		}
+	}
+	wave_sync();
+#pragma unroll
+	for (int m = 0; m < (C3 + 63) / 64; ++m) {
+		const int q = lane + 64 * m;
+		if (q < C3) {
+			const int ji = q % (NC * NC), k = q / (NC * NC);
+			T s = T(0);
+#pragma unroll
+			for (int l = 0; l < NF; ++l)
+				s += t2[l * NC * NC + ji] * sp[k * NF + l];
+			if (act)
+				coarse[ec * C3 + q] = s;
+		}
+	}
+}
+
+// the wavefront form for (nf, nc) = (n, (n-1)/2 + 1), n = 3..12; false for
+// other pairs (the block-per-element kernels above take them)
+template <typename T>
+static bool launch_transfer_w(hipStream_t st, int restrict_, int nf, int nc,
+		int nel, const void *prol, const void *src, void *dst,
+		int accumulate)
+{
+	if (nf < 3 || nf > 12 || nc != (nf - 1) / 2 + 1)
+		return false;
+	const int grid = (nel + XW - 1) / XW;
+	const size_t ws = restrict_
+		? (size_t) nf * nf * nf + (size_t) nf * nf * nc + (size_t) nf * nc * nc
+		: (size_t) nc * nc * nc + (size_t) nc * nc * nf + (size_t) nc * nf * nf;
+	const size_t sh = sizeof(T) * ((size_t) nc * nf + XW * ws);
+	if (sh > 65536)                 // beyond the default dynamic LDS limit
+		return false;
+	switch (nf) {
+#define XCASE(NF) case NF:                                                     \
+		if (restrict_)                                                 \
+			hipLaunchKernelGGL((k_restrict_w<T, NF, (NF - 1) / 2 + 1>),   \
+					dim3(grid), dim3(64 * XW), sh, st, nel,        \
+					(const T *) prol, (const T *) src, (T *) dst); \
+		else if (accumulate)                                           \
+			hipLaunchKernelGGL((k_prolong_w<T, NF, (NF - 1) / 2 + 1, true>), \
+					dim3(grid), dim3(64 * XW), sh, st, nel,        \
+					(const T *) prol, (const T *) src, (T *) dst); \
+		else                                                           \
+			hipLaunchKernelGGL((k_prolong_w<T, NF, (NF - 1) / 2 + 1, false>), \
+					dim3(grid), dim3(64 * XW), sh, st, nel,        \
+					(const T *) prol, (const T *) src, (T *) dst); \
+		return true;
+	XCASE(3) XCASE(4) XCASE(5) XCASE(6) XCASE(7) XCASE(8) XCASE(9)
+	XCASE(10) XCASE(11) XCASE(12)
+#undef XCASE
+	default:
+		return false;
+	}
+}
+
 template <typename T>
 static int launch_restrict(hipStream_t st, int nf, int nc, int nel,
 		const void *prol, const void *fine, void *coarse)
 {
+	if (nel > 0 && launch_transfer_w<T>(st, 1, nf, nc, nel, prol, fine,
+			coarse, 0)) {
+		LAUNCH_CHECK("k_restrict_w");
+		return SSX_OK;
+	}
 	const size_t sh = sizeof(T) * ((size_t) nc * nf + (size_t) nf * nf * nf
 			+ (size_t) nf * nf * nc + (size_t) nf * nc * nc);
 	hipLaunchKernelGGL(k_restrict<T>, dim3(nel), dim3(256), sh, st, nf, nc,
@@ -1033,6 +1271,11 @@ template <typename T>
 static int launch_prolong(hipStream_t st, int nf, int nc, int nel,
 		const void *prol, const void *coarse, void *fine, int accumulate)
 {
+	if (nel > 0 && launch_transfer_w<T>(st, 0, nf, nc, nel, prol, coarse,
+			fine, accumulate)) {
+		LAUNCH_CHECK("k_prolong_w");
+		return SSX_OK;
+	}
 	const size_t sh = sizeof(T) * ((size_t) nc * nf + (size_t) nc * nc * nc
 			+ (size_t) nc * nc * nf + (size_t) nc * nf * nf);
 	hipLaunchKernelGGL(k_prolong<T>, dim3(nel), dim3(256), sh, st, nf, nc,

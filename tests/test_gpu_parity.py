@@ -932,3 +932,30 @@ def test_assembled_coarse_level(name):
     assert rel(a[3][1], Z["ref_fcg_dirichlet"]) <= 1e-8
     assert st32.converged and st32.iterations <= int(sc[5]) + 3
     assert rel(v32, Z["ref_fcg_dirichlet"]) <= 1e-7
+
+
+@pytest.mark.parametrize("n", range(3, 14))
+def test_transfers_every_level_pair(n):
+    """Restriction (with the DSS average, precon.c:283-291) and prolongation
+    (interpolation.c:136-176) between order n and its p-multigrid level
+    (n-1)/2 + 1 against the oracle, for n = 3..13: the wavefront-per-element
+    kernels (n <= 12) and the block-per-element ones (n = 13).  18 elements:
+    the last block of four elements has two idle.  Bar 1e-13 relative."""
+    mesh = box_mesh(n, (3, 2, 3), deform=1, amp=0.1)
+    rng = np.random.default_rng(100 + n)
+    ctx = make_ctx(mesh)
+    o = P.Oracle(mesh)
+    try:
+        assert ctx.level_n(1) == (n - 1) // 2 + 1
+        fine = rng.standard_normal(ctx.ntot(0))
+        c = torch.zeros(ctx.ntot(1), dtype=torch.float64, device="cuda")
+        ctx.restrict(1, dev(fine), c)
+        ref = o.dss_average(1, o.restrict(1, fine))
+        assert rel(host(c), ref) <= 1e-13
+        coarse = rng.standard_normal(ctx.ntot(1))
+        f = torch.full((ctx.ntot(0),), 7.0, dtype=torch.float64, device="cuda")
+        ctx.prolong(0, dev(coarse), f)
+        assert rel(host(f), o.prolong(0, coarse)) <= 1e-13
+    finally:
+        o.close()
+        ctx.close()
