@@ -100,8 +100,15 @@ __device__ __forceinline__ double ax_team(int eblk, int nel,
 	const int e = eblk * TEAMS + team;
 	const bool act = e < nel;
 
-	for (int q = threadIdx.x; q < 2 * N2; q += blockDim.x)
-		s_d[q] = dmat[q];
+	// D and the SVV matrix (2 N^2 entries) for LDS, stored once the
+	// element's loads are out
+	constexpr int DPER = (2 * N2 + S::BLOCK - 1) / S::BLOCK;
+	T dv[DPER];
+#pragma unroll
+	for (int r = 0; r < DPER; ++r) {
+		const int q = (int) threadIdx.x + r * S::BLOCK;
+		dv[r] = dmat[q < 2 * N2 ? q : 0];
+	}
 	// D_il at [i*N+l]; the SVV matrix enters the xi-forward and the
 	// eta/zeta-transpose products only (helmholtz.c:41-44, deriv.c:29-66)
 	const T *gDn = dmat;
@@ -110,7 +117,8 @@ __device__ __forceinline__ double ax_team(int eblk, int nel,
 	const size_t eoff = (size_t) (act ? e : 0) * N3;
 	T *su = s_u[team], *sa = s_a[team], *sb = s_b[team];
 	// bc[e]: bits 0-5 Dirichlet faces, bits 8-13 faces with a neighbour
-	const int eb = act ? bc[e] : 0;
+	const int ebr = bc[act ? e : 0];
+	const int eb = act ? ebr : 0;
 	const int bits = apply_mask ? eb : 0;
 	const bool mi = ((bits & 1) && i == 0) || ((bits & 2) && i == N - 1)
 		|| ((bits & 4) && j == 0) || ((bits & 8) && j == N - 1);
@@ -119,6 +127,8 @@ __device__ __forceinline__ double ax_team(int eblk, int nel,
 	// no branch, and a branch around each would make the compiler wait for
 	// each load before the next one goes out.  Stores stay behind act.
 	T uc[N], acc[N];
+	const G *g = gf + (size_t) (act ? e : 0) * 6 * N3;
+	G fa[6], fb[6];
 	if (PU >= 1) {
 		const T bb = (T) scal[ib];
 		const T cc = ic >= 0 ? (T) -scal[ic] : T(0);
@@ -132,6 +142,7 @@ __device__ __forceinline__ double ax_team(int eblk, int nel,
 			if (PU == 2)
 				uo[k] = uio[q];
 		}
+		ax_factors<NT, N3>(g, tid, fa);       // plane 0, in flight
 #pragma unroll
 		for (int k = 0; k < N; ++k) {
 			const size_t q = eoff + k * N2 + tid;
@@ -155,6 +166,7 @@ __device__ __forceinline__ double ax_team(int eblk, int nel,
 #pragma unroll
 		for (int k = 0; k < N; ++k)
 			uc[k] = u[eoff + k * N2 + tid];
+		ax_factors<NT, N3>(g, tid, fa);       // plane 0, in flight
 #pragma unroll
 		for (int k = 0; k < N; ++k) {
 			uc[k] = act ? uc[k] : T(0);
@@ -162,8 +174,15 @@ __device__ __forceinline__ double ax_team(int eblk, int nel,
 			acc[k] = T(0);
 		}
 	}
-	const G *g = gf + (size_t) (act ? e : 0) * 6 * N3;
-	__syncthreads();
+#pragma unroll
+	for (int r = 0; r < DPER; ++r) {
+		const int q = (int) threadIdx.x + r * S::BLOCK;
+		if (q < 2 * N2)
+			s_d[q] = dv[r];
+	}
+	// only LDS data crosses the barriers of this function: the factor
+	// loads stay in flight across them
+	lds_sync();
 	const T *Dn = s_d;
 	const T *Ds = s_d + (OP == 1 ? N2 : 0);
 	T dxf[N], dyf[N], dxt[LR ? 1 : N], dyt[LR ? 1 : N];
@@ -198,7 +217,7 @@ __device__ __forceinline__ double ax_team(int eblk, int nel,
 		t2 = act ? t2 : T(0);
 		sa[j * P + i] = t0;
 		sb[j * P + i] = t1;
-		__syncthreads();
+		lds_sync();
 		T r = T(0);
 #pragma unroll
 		for (int l = 0; l < N; ++l) {
@@ -209,15 +228,13 @@ __device__ __forceinline__ double ax_team(int eblk, int nel,
 #pragma unroll
 		for (int kk = 0; kk < N; ++kk)
 			acc[kk] += (kk == k ? r : T(0)) + gDs[k * N + kk] * t2;
-		__syncthreads();
+		lds_sync();
 	};
 	// The six factors of plane k + 1 are loaded before plane k is computed,
 	// so two planes' loads are in flight per lane instead of one.  The loop
 	// takes two planes a turn with two register sets, so no loaded value is
 	// copied (a copy would wait for its load); a load past the last plane
 	// reads the last plane again, unused.
-	G fa[6], fb[6];
-	ax_factors<NT, N3>(g, tid, fa);
 #pragma unroll 1
 	for (int k = 0; k < N; k += 2) {
 		ax_factors<NT, N3>(g, (k + 1 < N ? k + 1 : N - 1) * N2 + tid, fb);

@@ -21,6 +21,23 @@ __device__ __forceinline__ float ldv(bf16_t x)
 	return __uint_as_float(((unsigned) x) << 16);
 }
 
+// barriers among the lanes of one wavefront (LDS written by one lane and
+// read by another), and for LDS data only across a block: loads of global
+// memory still in flight stay in flight across the latter
+__device__ __forceinline__ void wave_sync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void lds_sync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+	__builtin_amdgcn_s_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // A pointer read from device memory (a batch problem's description) is a
 // generic pointer: loads and stores through it compile to flat memory
 // instructions, which also count against the LDS counter and make the

@@ -1024,22 +1024,6 @@ k_prolong(int nf, int nc, const T *__restrict__ prol,
 // so no branch separates them), the three stages are separated by wave
 // barriers only, and a block holds four elements.  Each output is the same
 // sum in the same order as above, so results are identical bit for bit.
-__device__ __forceinline__ void wave_sync()
-{
-	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-	__builtin_amdgcn_wave_barrier();
-	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// a block barrier for LDS data only: loads of global memory still in
-// flight stay in flight across it
-__device__ __forceinline__ void lds_sync()
-{
-	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-	__builtin_amdgcn_s_barrier();
-	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 #define XW 4                            // elements (wavefronts) per block
 
 template <typename T, int NF, int NC, bool ACC>
