@@ -229,6 +229,18 @@ int ssx_set_diag_choice(ssx_ctx *ctx, int choice);
  * reference does.  A coarse operator that is not positive definite (other
  * than the pressure's constant null space) falls back to the CG. */
 #define SSX_OPT_COARSE_DIRECT 10
+/* SSX_OPT_COARSE_SPARSE: m > 0 solves the p-multigrid coarsest level of a
+ * reduced-precision V-cycle (as SSX_OPT_COARSE_DIRECT) by a sparse
+ * Cholesky factor when it has at most m unique nodes and more than the
+ * dense solve takes: nested-dissection order, multifrontal factorisation
+ * on the host at the first solve of each operator (lambda, null space;
+ * about 3 s for 36k nodes), and the two triangular solves on the device, a
+ * few launches per level of the separator tree.  Default
+ * SSX_COARSE_SPARSE_DEFAULT; 0: Jacobi-PCG to the cycle's relative
+ * tolerance.  A coarse operator that is not positive definite falls back
+ * to the CG. */
+#define SSX_OPT_COARSE_SPARSE 11
+#define SSX_COARSE_SPARSE_DEFAULT 65536
 #define SSX_PREC_FP64 0
 #define SSX_PREC_FP32 1
 #define SSX_PREC_BF16 2

@@ -14,7 +14,7 @@ from .native import (  # noqa: F401
     OPT_FUSED_REDUCE, OPT_COARSE_ASSEMBLED, OPT_GATHER_DSS,
     OPT_PERSIST_ABORT_TEST,
     OPT_PARTICLE_WALK,
-    OPT_COARSE_DIRECT,
+    OPT_COARSE_DIRECT, OPT_COARSE_SPARSE,
     PREC_FP64, PREC_FP32, PREC_BF16, NOALLOC_MAX_PARTICLES,
     AX_NVARIANTS,
     DATA_GFAC, DATA_MASS, DATA_DIAG_P, DATA_DIAG_H, DATA_SCALE, DATA_MASK,

@@ -404,6 +404,7 @@ int ssx_create(ssx_ctx **out, const ssx_mesh *m)
 	c->fused = 1;
 	c->persist = 0;
 	c->p_elem_walk = 1;
+	c->coarse_sparse = SSX_COARSE_SPARSE_DEFAULT;
 	c->fused_red = 0;
 	c->coarse_csr = 1;
 	c->gather_upd = 1;
@@ -467,8 +468,15 @@ int ssx_create(ssx_ctx **out, const ssx_mesh *m)
 		}
 		rc = setup_level(c, m, l, nx, ny, nz);
 	}
+	/* the coarsest level keeps its coordinates too (the nested-dissection
+	 * order of its sparse direct solve is taken from them) */
 	if (px && px != c->lev[0].h_x) {
-		free(px); free(py); free(pz);
+		if (rc == SSX_OK && nl > 1 && !c->lev[nl - 1].h_x) {
+			ssx_level *Lc = &c->lev[nl - 1];
+			Lc->h_x = px; Lc->h_y = py; Lc->h_z = pz;
+		} else {
+			free(px); free(py); free(pz);
+		}
 	}
 	/* prolongation matrices */
 	for (int l = 0; l + 1 < nl && rc == SSX_OK; ++l) {
@@ -660,6 +668,14 @@ int ssx_set_option(ssx_ctx *c, int option, int value)
 			return SSX_EINVAL;
 		}
 		c->coarse_direct = value;
+		return SSX_OK;
+	case SSX_OPT_COARSE_SPARSE:
+		if (value < 0) {
+			ssx_set_error("sparse coarse solve: %d unknowns out of range",
+					value);
+			return SSX_EINVAL;
+		}
+		c->coarse_sparse = value;
 		return SSX_OK;
 	case SSX_OPT_PARTICLE_WALK:
 		if (value != 0 && value != 1) {

@@ -332,6 +332,14 @@ static int unull(ssx_ctx *c, double *x)
  * space); a solve is then one dense product over the unique nodes. */
 void ssx_coarse_direct_free(ssx_ctx *c)
 {
+	ssx_sd_free(c->sd);
+	c->sd = NULL;
+	hipFree(c->sd_map);
+	hipFree(c->sd_b);
+	hipFree(c->sd_x);
+	c->sd_map = NULL;
+	c->sd_b = c->sd_x = NULL;
+	c->sd_nu = 0;
 	hipFree(c->cd_inv);
 	c->cd_inv = NULL;
 	c->cd_n = 0;
@@ -432,6 +440,151 @@ static int coarse_direct_solve(ssx_ctx *c, ssx_level *L, int pres,
 	return ssxk_scatter_u(st, L->ntot, cs->cmap, cs->x, x);
 }
 
+/* ---- sparse direct coarsest solve (SSX_OPT_COARSE_SPARSE) ----
+ * The same system as the dense solve above, without its dense inverse:
+ * the masked nodes are left out (their solution is zero), and for a
+ * singular operator one node is pinned to zero instead of the rank-one
+ * term (the zero-mean right-hand side makes the pinned system's solution a
+ * solution of the singular one; the null space is then removed as in the
+ * CG).  The factor (ssx_sdirect.c) is built on the host once per
+ * (operator, lambda, null space) and solved on the device. */
+static int coarse_sparse_build(ssx_ctx *c, int l, int op, int pres)
+{
+	ssx_level *L = &c->lev[l];
+	ssx_coarse *cs = &L->cs;
+	const int nu = cs->nu;
+	if (c->sd_nu == nu && c->sd_op == op && c->sd_pres == pres
+			&& c->sd_lambda == c->lambda && (c->sd || c->sd_bad))
+		return c->sd_bad ? SSX_ENUMERIC : SSX_OK;
+	ssx_sd_free(c->sd);
+	c->sd = NULL;
+	hipFree(c->sd_map);
+	hipFree(c->sd_b);
+	hipFree(c->sd_x);
+	c->sd_map = NULL;
+	c->sd_b = c->sd_x = NULL;
+	c->sd_nu = nu;
+	c->sd_op = op;
+	c->sd_pres = pres;
+	c->sd_lambda = c->lambda;
+	c->sd_bad = 1;                  /* until the factorisation succeeds */
+	hipStream_t st = (hipStream_t) c->stream;
+	HIPQ(hipStreamSynchronize(st));
+	const long long nnz = cs->nnz;
+	int *rp = malloc(sizeof(int) * ((size_t) nu + 1));
+	int *ci = malloc(sizeof(int) * (size_t) nnz);
+	double *av = malloc(sizeof(double) * (size_t) nnz);
+	double *mk = malloc(sizeof(double) * (size_t) nu);
+	int *rep = malloc(sizeof(int) * (size_t) nu);
+	int *rid = malloc(sizeof(int) * (size_t) nu);
+	int *map = malloc(sizeof(int) * (size_t) nu);
+	int *rr = malloc(sizeof(int) * ((size_t) nu + 1));
+	int *rc_ = malloc(sizeof(int) * (size_t) nnz);
+	double *ra = malloc(sizeof(double) * (size_t) nnz);
+	double *xyz = malloc(sizeof(double) * 3 * (size_t) nu);
+	int rc = (rp && ci && av && mk && rep && rid && map && rr && rc_ && ra
+			&& xyz) ? SSX_OK : SSX_ENOMEM;
+	if (rc != SSX_OK)
+		ssx_set_error("out of host memory (sparse coarse factor)");
+	if (rc == SSX_OK && !L->h_x) {
+		ssx_set_error("coarsest level without host coordinates");
+		rc = SSX_EINVAL;
+	}
+	const double *val = op == SSX_OP_HELMHOLTZ ? cs->valh : cs->valp;
+	if (rc == SSX_OK && (hipMemcpy(rp, cs->rowptr, sizeof(int) * (nu + 1),
+				hipMemcpyDeviceToHost) != hipSuccess
+			|| hipMemcpy(ci, cs->col, sizeof(int) * nnz,
+				hipMemcpyDeviceToHost) != hipSuccess
+			|| hipMemcpy(av, val, sizeof(double) * nnz,
+				hipMemcpyDeviceToHost) != hipSuccess
+			|| hipMemcpy(mk, cs->mask, sizeof(double) * nu,
+				hipMemcpyDeviceToHost) != hipSuccess
+			|| hipMemcpy(rep, cs->rep, sizeof(int) * nu,
+				hipMemcpyDeviceToHost) != hipSuccess)) {
+		ssx_set_error("hipMemcpy of the coarse operator failed");
+		rc = SSX_EHIP;
+	}
+	int n = 0, singular = pres;
+	if (rc == SSX_OK) {
+		if (!pres && op == SSX_OP_POISSON) {
+			double msum = 0.0;
+			for (int u = 0; u < nu; ++u)
+				msum += mk[u];
+			singular = msum >= (double) nu - 0.5;
+		}
+		/* included nodes; a singular system pins its last node */
+		for (int u = 0; u < nu; ++u)
+			rid[u] = (pres || mk[u] != 0.0) ? 0 : -1;
+		if (singular)
+			for (int u = nu - 1; u >= 0; --u)
+				if (rid[u] == 0) {
+					rid[u] = -1;
+					break;
+				}
+		for (int u = 0; u < nu; ++u)
+			if (rid[u] == 0) {
+				map[n] = u;
+				rid[u] = n++;
+			}
+		long long m = 0;
+		rr[0] = 0;
+		for (int k = 0; k < n; ++k) {
+			const int u = map[k];
+			for (int q = rp[u]; q < rp[u + 1]; ++q)
+				if (rid[ci[q]] >= 0) {
+					rc_[m] = rid[ci[q]];
+					ra[m] = av[q];
+					++m;
+				}
+			rr[k + 1] = (int) m;
+			const size_t e = (size_t) rep[u];
+			xyz[3 * (size_t) k] = L->h_x[e];
+			xyz[3 * (size_t) k + 1] = L->h_y[e];
+			xyz[3 * (size_t) k + 2] = L->h_z[e];
+		}
+	}
+	c->sd_sing = singular;
+	ssx_sdfactor *f = NULL;
+	if (rc == SSX_OK && n > 0)
+		rc = ssx_sd_factor(n, rr, rc_, ra, xyz, &f);
+	else if (rc == SSX_OK)
+		rc = SSX_ENUMERIC;          /* nothing to factor: keep the CG */
+	if (rc == SSX_OK) {
+		/* the factor's system index -> unique node: gather and scatter
+		 * go straight between unique-node vectors and the factor */
+		for (int i = 0; i < n; ++i)
+			f->perm[i] = map[f->perm[i]];
+		rc = ssx_sd_upload(f);
+	}
+	free(rp); free(ci); free(av); free(mk); free(rep); free(rid);
+	free(map); free(rr); free(rc_); free(ra); free(xyz);
+	if (rc != SSX_OK) {
+		ssx_sd_free(f);
+		return rc;
+	}
+	c->sd = f;
+	c->sd_bad = 0;
+	return SSX_OK;
+}
+
+static int coarse_sparse_solve(ssx_ctx *c, ssx_level *L, int pres,
+		const double *f, double *x)
+{
+	ssx_coarse *cs = &L->cs;
+	void *st = c->stream;
+	const int nu = cs->nu;
+	TRYQ(ssxk_gather_u(st, nu, cs->rep, f, cs->f));
+	if (!pres)
+		TRYQ(ssxk_mul(st, (size_t) nu, cs->f, cs->mask, cs->f));
+	if (c->sd_sing)
+		TRYQ(unull(c, cs->f));
+	TRYQ(ssxk_fill(st, cs->x, 0.0, (size_t) nu));
+	TRYQ(ssx_sd_solve_dev(c->sd, st, cs->f, cs->x));
+	if (c->sd_sing)
+		TRYQ(unull(c, cs->x));
+	return ssxk_scatter_u(st, L->ntot, cs->cmap, cs->x, x);
+}
+
 int ssx_coarse_cg(ssx_ctx *c, int l, int op, int var, const double *f,
 		double *x, double eps, double rtol, int max_iter, int x_zero,
 		int *iters, double *err_out)
@@ -450,6 +603,20 @@ int ssx_coarse_cg(ssx_ctx *c, int l, int op, int var, const double *f,
 		const int rc = coarse_direct_build(c, l, op, pres);
 		if (rc == SSX_OK) {
 			TRYQ(coarse_direct_solve(c, L, pres, f, x));
+			if (iters)
+				*iters = 1;
+			*err_out = 0.0;
+			return SSX_OK;
+		}
+		if (rc != SSX_ENUMERIC)
+			return rc;
+	}
+	if (c->coarse_sparse > 0 && cs->nu <= c->coarse_sparse && rtol > 0.0
+			&& x_zero) {
+		const int pres = var == SSX_PRESSURE;
+		const int rc = coarse_sparse_build(c, l, op, pres);
+		if (rc == SSX_OK) {
+			TRYQ(coarse_sparse_solve(c, &c->lev[l], pres, f, x));
 			if (iters)
 				*iters = 1;
 			*err_out = 0.0;

@@ -17,6 +17,7 @@ OPT_GATHER_DSS = 7
 OPT_PERSIST_ABORT_TEST = 8
 OPT_PARTICLE_WALK = 9
 OPT_COARSE_DIRECT = 10
+OPT_COARSE_SPARSE = 11
 AX_NVARIANTS = 8
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 NOALLOC_MAX_PARTICLES = 131072

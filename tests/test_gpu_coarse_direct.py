@@ -23,6 +23,7 @@ def _solve(mesh, f_host, var, prec, direct, eps):
         ctx.set_diag_choice(-1 if var == S.PRESSURE else 0)
         ctx.set_option(S.OPT_SMOOTHER_PRECISION, prec)
         ctx.set_option(S.OPT_COARSE_DIRECT, 16384 if direct else 0)
+        ctx.set_option(S.OPT_COARSE_SPARSE, 0)       # the CG otherwise
         f = torch.from_numpy(np.ascontiguousarray(f_host)).cuda()
         u = torch.zeros_like(f)
         st = ctx.fcg(S.OP_POISSON, var, S.PC_MG, f, u, eps, 500)
