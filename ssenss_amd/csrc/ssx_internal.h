@@ -158,23 +158,20 @@ struct ssx_ctx {
 };
 
 /* ---- sparse direct solver (ssx_sdirect.c, ssx_sdkernels.hip) ---- */
-/* a level of the separator tree runs one workgroup per supernode when no
- * supernode has more than SD_SMALL_NC columns or SD_SMALL_NU rows below
- * them (the workgroup's LDS holds those vectors); a split level's vectors
- * (at most SD_MAX_VEC entries) are staged in LDS by every workgroup */
-#define SD_SMALL_NC 512
-#define SD_SMALL_NU 2048
+/* the device solve stages a supernode's vectors (at most SD_MAX_VEC
+ * entries: its columns and the rows below them) in LDS */
 #define SD_MAX_VEC 4096
+
 /* one supernode of the factor: columns [c0, c0+nc) of the new order, nu
  * rows below them (their new indices at idx[uoff..]), children ch0/ch1
  * (-1: none) with gather maps at idx[g0..] / idx[g1..] (nc + nu entries:
  * for each row of this front, the child's update entry landing there or
- * -1); M = L11^-1 at val[moff] (nc x nc) and L21 at val[loff] (nu x nc),
- * column-major; the update vector t at T[toff] (nu) */
+ * -1); M = L11^-1 (nc x nc) and K = L21 M (nu x nc), column-major at
+ * val[moff] / val[koff] and row-major at val[mroff] / val[kroff]; the
+ * update vector t at T[toff] (nu) */
 typedef struct {
 	int c0, nc, nu, ch0, ch1, g0, g1, uoff;
-	long long moff, loff, toff;
-	long long mroff, lroff;      /* the same M and L21 row-major          */
+	long long moff, koff, toff, mroff, kroff;
 } ssx_sdnode;
 
 typedef struct {
@@ -186,21 +183,19 @@ typedef struct ssx_sdfactor {
 	ssx_sdnode *node;
 	int *perm;                   /* [n] system index of new index i       */
 	int *lev_off, *lev_sn;       /* supernodes level by level, leaves first */
-	int *lev_big;                /* [nlev] products split over workgroups */
 	long long nval, nidx, nt;
 	double *val;
 	int *idx;
 	double flops;
-	int *wk_off;                 /* [4 nlev + 1] work-table ranges: per
-	                                level, y rows, t rows, g columns, x
-	                                columns                              */
+	int *wk_off;                 /* [2 nlev + 1] work-table ranges: per
+	                                level, forward then backward         */
 	ssx_sdwork *wk;
 	/* device */
 	ssx_sdnode *d_node;
-	int *d_lev_sn, *d_idx, *d_perm;
+	int *d_idx, *d_perm;
 	double *d_val;
 	ssx_sdwork *d_wk;
-	double *d_b, *d_y, *d_x, *d_t, *d_e;
+	double *d_b, *d_y, *d_x, *d_t;
 } ssx_sdfactor;
 
 int ssx_sd_factor(int n, const int *rp, const int *ci, const double *a,
@@ -211,34 +206,18 @@ int ssx_sd_upload(ssx_sdfactor *f);
 int ssx_sd_solve_dev(ssx_sdfactor *f, void *stream, const double *b,
 		double *x);
 void ssx_sd_free(ssx_sdfactor *f);
-/* workgroups of a split level's product: nout outputs of rows of length
- * rowlen */
+/* workgroups of a product: nout outputs of rows of length rowlen */
 int ssxk_sd_blocks(int nout, int rowlen);
 int ssxk_sd_gather(void *stream, int n, const int *perm, const double *src,
 		double *dst);
 int ssxk_sd_scatter(void *stream, int n, const int *perm, const double *src,
 		double *dst);
-int ssxk_sd_fwd_small(void *stream, int nsn, const int *lev_sn,
+int ssxk_sd_fwd(void *stream, int nwk, const ssx_sdwork *wk,
 		const ssx_sdnode *node, const double *val, const int *idx,
 		const double *b, double *y, double *t);
-int ssxk_sd_fwd_beff(void *stream, int nsn, const int *lev_sn,
-		const ssx_sdnode *node, const int *idx, const double *b, double *e,
-		double *t);
-int ssxk_sd_fwd_y(void *stream, int nwk, const ssx_sdwork *wk,
-		const ssx_sdnode *node, const double *val, const double *e,
-		double *y);
-int ssxk_sd_fwd_t(void *stream, int nwk, const ssx_sdwork *wk,
-		const ssx_sdnode *node, const double *val, const double *y,
-		double *t);
-int ssxk_sd_bwd_small(void *stream, int nsn, const int *lev_sn,
+int ssxk_sd_bwd(void *stream, int nwk, const ssx_sdwork *wk,
 		const ssx_sdnode *node, const double *val, const int *idx,
 		const double *y, double *x);
-int ssxk_sd_bwd_g(void *stream, int nwk, const ssx_sdwork *wk,
-		const ssx_sdnode *node, const double *val, const int *idx,
-		const double *y, const double *x, double *e);
-int ssxk_sd_bwd_x(void *stream, int nwk, const ssx_sdwork *wk,
-		const ssx_sdnode *node, const double *val, const double *e,
-		double *x);
 
 /* ---- assembled coarsest level (ssx_coarse.c) ---- */
 int ssx_coarse_build(ssx_ctx *c, int l, const double *gf, const double *mass,

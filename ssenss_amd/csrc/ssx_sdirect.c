@@ -24,10 +24,12 @@
  *              matrices (extend-add), and is factored partially: L11, L21,
  *              the update matrix F22 - L21 L21^T for the parent.  Stored:
  *              M = L11^-1 (so a solve step is a product, not a substitution
- *              chain) and L21, both column-major.
+ *              chain) and K = L21 M, row- and column-major.
  *   solve      forward (leaves to root): b~ = b_C - (children's updates on
- *              C), y_C = M b~, t_s = L21 y_C + (children's updates on U_s);
- *              backward (root to leaves): x_C = M^T (y_C - L21^T x_U).
+ *              C), y_C = M b~, t_s = K b~ + (children's updates on U_s);
+ *              backward (root to leaves): x_C = M^T y_C - K^T x_U (which is
+ *              M^T (y_C - L21^T x_U)).  Both products of a step read the
+ *              same vector, so each level is one launch in each direction.
  *              Every update reaches its destination through one parent
  *              gather with the children in a fixed order: no atomics, the
  *              result is the same on every run.
@@ -41,7 +43,6 @@
 #include "ssx_internal.h"
 
 #define SD_LEAF 64                   /* largest leaf supernode            */
-#define SD_SMALL_WORK (1 << 16)      /* and at most this many entries     */
 
 typedef struct {
 	int n;
@@ -264,12 +265,11 @@ void ssx_sd_free(ssx_sdfactor *f)
 	if (!f)
 		return;
 	free(f->node); free(f->perm); free(f->lev_off); free(f->lev_sn);
-	free(f->lev_big); free(f->val); free(f->idx); free(f->wk_off);
+	free(f->val); free(f->idx); free(f->wk_off);
 	free(f->wk);
-	hipFree(f->d_node); hipFree(f->d_lev_sn); hipFree(f->d_val);
+	hipFree(f->d_node); hipFree(f->d_val);
 	hipFree(f->d_idx); hipFree(f->d_perm); hipFree(f->d_wk);
 	hipFree(f->d_b); hipFree(f->d_y); hipFree(f->d_x); hipFree(f->d_t);
-	hipFree(f->d_e);
 	free(f);
 }
 
@@ -377,11 +377,11 @@ int ssx_sd_factor(int n, const int *rp, const int *ci, const double *a,
 		ssx_sdnode *p = &f->node[s];
 		p->moff = nval;
 		nval += (long long) p->nc * p->nc;
-		p->loff = nval;
+		p->koff = nval;
 		nval += (long long) p->nu * p->nc;
 		p->mroff = nval;
 		nval += (long long) p->nc * p->nc;
-		p->lroff = nval;
+		p->kroff = nval;
 		nval += (long long) p->nu * p->nc;
 		p->toff = nt;
 		nt += p->nu;
@@ -513,18 +513,28 @@ int ssx_sd_factor(int n, const int *rp, const int *ci, const double *a,
 		if (rc == SSX_OK) {
 			double *M = f->val + p->moff;
 			tri_inverse(F, nr, nc, M);
-			double *L21 = f->val + p->loff;
-			for (int j = 0; j < nc; ++j)
-				memcpy(L21 + (size_t) j * nu, F + nc + (size_t) j * nr,
-						sizeof(double) * (size_t) nu);
+			/* K = L21 M (nu x nc): the rows below apply to b~ directly,
+			 * so y = M b~ and t = K b~ are independent products */
+			double *K = f->val + p->koff;
+			memset(K, 0, sizeof(double) * (size_t) nu * nc);
+			for (int j = 0; j < nc; ++j) {
+				double *kj = K + (size_t) j * nu;
+				for (int i = j; i < nc; ++i) {
+					const double m = M[i + (size_t) j * nc];
+					const double *li = F + nc + (size_t) i * nr;
+					for (int r = 0; r < nu; ++r)
+						kj[r] += li[r] * m;
+				}
+			}
+			f->flops += (double) nu * nc * nc;
 			/* row-major copies for the forward products */
-			double *Mr = f->val + p->mroff, *Lr = f->val + p->lroff;
+			double *Mr = f->val + p->mroff, *Kr = f->val + p->kroff;
 			for (int j = 0; j < nc; ++j)
 				for (int i = 0; i < nc; ++i)
 					Mr[(size_t) i * nc + j] = M[i + (size_t) j * nc];
 			for (int j = 0; j < nc; ++j)
 				for (int r = 0; r < nu; ++r)
-					Lr[(size_t) r * nc + j] = L21[r + (size_t) j * nu];
+					Kr[(size_t) r * nc + j] = K[r + (size_t) j * nu];
 			if (nu > 0) {
 				double *Up = malloc(sizeof(double) * (size_t) nu * nu);
 				if (!Up) {
@@ -573,8 +583,7 @@ int ssx_sd_factor(int n, const int *rp, const int *ci, const double *a,
 		f->nlev = nlev;
 		f->lev_off = calloc((size_t) nlev + 1, sizeof(int));
 		f->lev_sn = malloc(sizeof(int) * (size_t) f->nsn);
-		f->lev_big = calloc((size_t) nlev, sizeof(int));
-		if (!f->lev_off || !f->lev_sn || !f->lev_big)
+		if (!f->lev_off || !f->lev_sn)
 			rc = SSX_ENOMEM;
 	}
 	if (rc == SSX_OK) {
@@ -590,26 +599,15 @@ int ssx_sd_factor(int n, const int *rp, const int *ci, const double *a,
 				f->lev_sn[f->lev_off[lev[s]] + fill[lev[s]]++] = s;
 			free(fill);
 		}
-		/* a level runs one workgroup per supernode when all its
-		 * supernodes are small; otherwise its products are split over
-		 * many workgroups (work tables of (supernode, block)) */
-		for (int l = 0; l < nlev && rc == SSX_OK; ++l) {
-			int big = 0;
-			for (int k = f->lev_off[l]; k < f->lev_off[l + 1]; ++k) {
-				const ssx_sdnode *p = &f->node[f->lev_sn[k]];
-				const long long work = (long long) p->nc * p->nc / 2
-					+ (long long) p->nc * p->nu;
-				if (p->nc > SD_SMALL_NC || p->nu > SD_SMALL_NU
-						|| work > SD_SMALL_WORK)
-					big = 1;
-				if (p->nc > SD_MAX_VEC || p->nu > SD_MAX_VEC) {
-					ssx_set_error("sparse direct: supernode of %d x %d "
-							"beyond the solve's limit %d", p->nc, p->nu,
-							SD_MAX_VEC);
-					rc = SSX_ENUMERIC;
-				}
+		/* the device solve stages a supernode's vectors in LDS */
+		for (int s = 0; s < f->nsn && rc == SSX_OK; ++s) {
+			const ssx_sdnode *p = &f->node[s];
+			if (p->nc + p->nu > SD_MAX_VEC) {
+				ssx_set_error("sparse direct: supernode of %d + %d rows "
+						"beyond the solve's limit %d", p->nc, p->nu,
+						SD_MAX_VEC);
+				rc = SSX_ENUMERIC;
 			}
-			f->lev_big[l] = big;
 		}
 	}
 	free(lev);
@@ -623,7 +621,11 @@ int ssx_sd_factor(int n, const int *rp, const int *ci, const double *a,
 
 /* The solve on the host, step for step as on the device (a check of the
  * factorisation and of the device kernels).  b and x in the system's own
- * numbering. */
+ * numbering.
+ *   forward (levels up):  b~ = b_C - (children's t on C),
+ *                         y_C = M b~,  t = K b~ + (children's t on U)
+ *   backward (levels down): x_C = M^T y_C - K^T x_U
+ * (K = L21 M, so M^T (y - L21^T x_U) = M^T y - K^T x_U). */
 int ssx_sd_solve_host(const ssx_sdfactor *f, const double *b, double *x)
 {
 	const int n = f->n;
@@ -643,7 +645,7 @@ int ssx_sd_solve_host(const ssx_sdfactor *f, const double *b, double *x)
 		for (int k = f->lev_off[l]; k < f->lev_off[l + 1]; ++k) {
 			const ssx_sdnode *p = &f->node[f->lev_sn[k]];
 			const int nc = p->nc, nu = p->nu;
-			const double *M = f->val + p->moff, *L = f->val + p->loff;
+			const double *M = f->val + p->moff, *K = f->val + p->koff;
 			const int *g0 = p->g0 >= 0 ? f->idx + p->g0 : NULL;
 			const int *g1 = p->g1 >= 0 ? f->idx + p->g1 : NULL;
 			const double *t0 = p->ch0 >= 0 ? t + f->node[p->ch0].toff : NULL;
@@ -665,7 +667,7 @@ int ssx_sd_solve_host(const ssx_sdfactor *f, const double *b, double *x)
 			for (int r = 0; r < nu; ++r) {
 				double s = 0.0;
 				for (int j = 0; j < nc; ++j)
-					s += L[r + (size_t) j * nu] * y[p->c0 + j];
+					s += K[r + (size_t) j * nu] * w[j];
 				double ps = 0.0;
 				if (g0 && g0[nc + r] >= 0)
 					ps += t0[g0[nc + r]];
@@ -678,20 +680,18 @@ int ssx_sd_solve_host(const ssx_sdfactor *f, const double *b, double *x)
 		for (int k = f->lev_off[l]; k < f->lev_off[l + 1]; ++k) {
 			const ssx_sdnode *p = &f->node[f->lev_sn[k]];
 			const int nc = p->nc, nu = p->nu;
-			const double *M = f->val + p->moff, *L = f->val + p->loff;
+			const double *M = f->val + p->moff, *K = f->val + p->koff;
 			const int *u = f->idx + p->uoff;
 			for (int j = 0; j < nc; ++j) {
-				double s = 0.0;
-				for (int r = 0; r < nu; ++r)
-					s += L[r + (size_t) j * nu] * xn[u[r]];
-				w[j] = y[p->c0 + j] - s;
-			}
-			for (int j = 0; j < nc; ++j) {
-				double s = 0.0;
+				double s = 0.0, q = 0.0;
 				for (int i = j; i < nc; ++i)
-					s += M[i + (size_t) j * nc] * w[i];
-				xn[p->c0 + j] = s;
+					s += M[i + (size_t) j * nc] * y[p->c0 + i];
+				for (int r = 0; r < nu; ++r)
+					q += K[r + (size_t) j * nu] * xn[u[r]];
+				w[j] = s - q;
 			}
+			for (int j = 0; j < nc; ++j)
+				xn[p->c0 + j] = w[j];
 		}
 	for (int i = 0; i < n; ++i)
 		x[f->perm[i]] = xn[i];
@@ -700,7 +700,8 @@ int ssx_sd_solve_host(const ssx_sdfactor *f, const double *b, double *x)
 }
 
 /* test hooks: factor a host system and solve it on the host, reporting
- * the supernode count, levels, factor entries and flops */
+ * the supernode count, levels, factor entries, flops and the largest
+ * supernode (columns, rows below) */
 int ssxh_sd_test(int n, const int *rp, const int *ci, const double *a,
 		const double *xyz, const double *b, double *x, double *info)
 {
@@ -710,20 +711,17 @@ int ssxh_sd_test(int n, const int *rp, const int *ci, const double *a,
 		return rc;
 	const int rs = ssx_sd_solve_host(f, b, x);
 	if (info) {
-		int maxnc = 0, maxnu = 0, nbig = 0;
+		int maxnc = 0, maxnu = 0;
 		for (int s = 0; s < f->nsn; ++s) {
 			maxnc = f->node[s].nc > maxnc ? f->node[s].nc : maxnc;
 			maxnu = f->node[s].nu > maxnu ? f->node[s].nu : maxnu;
 		}
-		for (int l = 0; l < f->nlev; ++l)
-			nbig += f->lev_big[l];
 		info[0] = f->nsn;
 		info[1] = f->nlev;
 		info[2] = (double) f->nval;
 		info[3] = f->flops;
 		info[4] = maxnc;
 		info[5] = maxnu;
-		info[6] = nbig;
 	}
 	ssx_sd_free(f);
 	return rs;
@@ -749,17 +747,16 @@ static int sd_dup(void **dst, const void *src, size_t bytes)
 
 int ssx_sd_upload(ssx_sdfactor *f)
 {
-	/* work tables of the split levels: the outputs of each product (y
-	 * rows, t rows, g and x columns) in blocks sized by their row length */
+	/* work tables: per level, the forward outputs (y rows, then t rows) of
+	 * every supernode and the backward ones (x rows), in blocks sized by
+	 * their row length */
 	long long tot = 0;
-	for (int l = 0; l < f->nlev; ++l)
-		for (int k = f->lev_off[l]; f->lev_big[l] && k < f->lev_off[l + 1];
-				++k) {
-			const ssx_sdnode *p = &f->node[f->lev_sn[k]];
-			tot += ssxk_sd_blocks(p->nc, p->nc) + ssxk_sd_blocks(p->nu, p->nc)
-				+ ssxk_sd_blocks(p->nc, p->nu) + ssxk_sd_blocks(p->nc, p->nc);
-		}
-	f->wk_off = malloc(sizeof(int) * (4 * (size_t) f->nlev + 1));
+	for (int s = 0; s < f->nsn; ++s) {
+		const ssx_sdnode *p = &f->node[s];
+		tot += ssxk_sd_blocks(p->nc + p->nu, p->nc)
+			+ ssxk_sd_blocks(p->nc, p->nc + p->nu);
+	}
+	f->wk_off = malloc(sizeof(int) * (2 * (size_t) f->nlev + 1));
 	f->wk = malloc(sizeof(ssx_sdwork) * (size_t) (tot > 0 ? tot : 1));
 	if (!f->wk_off || !f->wk) {
 		ssx_set_error("out of host memory (sparse direct tables)");
@@ -767,17 +764,13 @@ int ssx_sd_upload(ssx_sdfactor *f)
 	}
 	int m = 0;
 	for (int l = 0; l < f->nlev; ++l)
-		for (int ph = 0; ph < 4; ++ph) {
-			f->wk_off[4 * l + ph] = m;
-			if (!f->lev_big[l])
-				continue;
+		for (int d = 0; d < 2; ++d) {
+			f->wk_off[2 * l + d] = m;
 			for (int k = f->lev_off[l]; k < f->lev_off[l + 1]; ++k) {
 				const int s = f->lev_sn[k];
 				const ssx_sdnode *p = &f->node[s];
-				const int nb = ph == 0 ? ssxk_sd_blocks(p->nc, p->nc)
-					: ph == 1 ? ssxk_sd_blocks(p->nu, p->nc)
-					: ph == 2 ? ssxk_sd_blocks(p->nc, p->nu)
-					: ssxk_sd_blocks(p->nc, p->nc);
+				const int nb = d == 0 ? ssxk_sd_blocks(p->nc + p->nu, p->nc)
+					: ssxk_sd_blocks(p->nc, p->nc + p->nu);
 				for (int b = 0; b < nb; ++b) {
 					f->wk[m].sn = s;
 					f->wk[m].blk = b;
@@ -785,12 +778,10 @@ int ssx_sd_upload(ssx_sdfactor *f)
 				}
 			}
 		}
-	f->wk_off[4 * f->nlev] = m;
+	f->wk_off[2 * f->nlev] = m;
 	const size_t vn = sizeof(double) * (size_t) f->n;
 	int rc = sd_dup((void **) &f->d_node, f->node,
 			sizeof(ssx_sdnode) * (size_t) f->nsn);
-	if (!rc) rc = sd_dup((void **) &f->d_lev_sn, f->lev_sn,
-			sizeof(int) * (size_t) f->nsn);
 	if (!rc) rc = sd_dup((void **) &f->d_idx, f->idx,
 			sizeof(int) * (size_t) f->nidx);
 	if (!rc) rc = sd_dup((void **) &f->d_perm, f->perm,
@@ -802,7 +793,6 @@ int ssx_sd_upload(ssx_sdfactor *f)
 	if (!rc) rc = sd_dup((void **) &f->d_b, NULL, vn);
 	if (!rc) rc = sd_dup((void **) &f->d_y, NULL, vn);
 	if (!rc) rc = sd_dup((void **) &f->d_x, NULL, vn);
-	if (!rc) rc = sd_dup((void **) &f->d_e, NULL, vn);
 	if (!rc) rc = sd_dup((void **) &f->d_t, NULL,
 			sizeof(double) * (size_t) f->nt);
 	/* the host copies of the values are no longer needed */
@@ -813,36 +803,19 @@ int ssx_sd_upload(ssx_sdfactor *f)
 	return rc;
 }
 
+/* one forward and one backward launch per level of the tree */
 int ssx_sd_solve_dev(ssx_sdfactor *f, void *st, const double *b, double *x)
 {
 	int rc = ssxk_sd_gather(st, f->n, f->d_perm, b, f->d_b);
 	for (int l = 0; rc == SSX_OK && l < f->nlev; ++l) {
-		const int k0 = f->lev_off[l], nk = f->lev_off[l + 1] - k0;
-		if (!f->lev_big[l]) {
-			rc = ssxk_sd_fwd_small(st, nk, f->d_lev_sn + k0, f->d_node,
-					f->d_val, f->d_idx, f->d_b, f->d_y, f->d_t);
-			continue;
-		}
-		const int *o = f->wk_off + 4 * l;
-		rc = ssxk_sd_fwd_beff(st, nk, f->d_lev_sn + k0, f->d_node, f->d_idx,
-				f->d_b, f->d_e, f->d_t);
-		if (!rc) rc = ssxk_sd_fwd_y(st, o[1] - o[0], f->d_wk + o[0],
-				f->d_node, f->d_val, f->d_e, f->d_y);
-		if (!rc) rc = ssxk_sd_fwd_t(st, o[2] - o[1], f->d_wk + o[1],
-				f->d_node, f->d_val, f->d_y, f->d_t);
+		const int *o = f->wk_off + 2 * l;
+		rc = ssxk_sd_fwd(st, o[1] - o[0], f->d_wk + o[0], f->d_node,
+				f->d_val, f->d_idx, f->d_b, f->d_y, f->d_t);
 	}
 	for (int l = f->nlev - 1; rc == SSX_OK && l >= 0; --l) {
-		const int k0 = f->lev_off[l], nk = f->lev_off[l + 1] - k0;
-		if (!f->lev_big[l]) {
-			rc = ssxk_sd_bwd_small(st, nk, f->d_lev_sn + k0, f->d_node,
-					f->d_val, f->d_idx, f->d_y, f->d_x);
-			continue;
-		}
-		const int *o = f->wk_off + 4 * l;
-		rc = ssxk_sd_bwd_g(st, o[3] - o[2], f->d_wk + o[2], f->d_node,
-				f->d_val, f->d_idx, f->d_y, f->d_x, f->d_e);
-		if (!rc) rc = ssxk_sd_bwd_x(st, o[4] - o[3], f->d_wk + o[3],
-				f->d_node, f->d_val, f->d_e, f->d_x);
+		const int *o = f->wk_off + 2 * l;
+		rc = ssxk_sd_bwd(st, o[2] - o[1], f->d_wk + o[1], f->d_node,
+				f->d_val, f->d_idx, f->d_y, f->d_x);
 	}
 	if (rc == SSX_OK)
 		rc = ssxk_sd_scatter(st, f->n, f->d_perm, f->d_x, x);

@@ -1,11 +1,10 @@
 // ssx_sdkernels.hip -- the triangular solves of the sparse direct coarse
-// solver (ssx_sdirect.c): one launch per level of the separator tree (one
-// workgroup per supernode), or three (forward) / two (backward) on levels
-// whose supernodes are split over many workgroups.
+// solver (ssx_sdirect.c): one forward and one backward launch per level of
+// the separator tree, each over (supernode, block of outputs) work items.
 //
 // Every product is a set of dot products over contiguous rows: the
-// forward ones read M and L21 row-major, the backward ones (with M^T and
-// L21^T) read them column-major.  A group of G lanes takes one output, G a
+// forward ones read M and K = L21 M row-major, the backward ones (with M^T
+// and K^T) read them column-major.  A group of G lanes takes one output, G a
 // power of two chosen from the row length so that each lane issues at most
 // SD_U loads, all together, before a fixed shuffle tree adds the group:
 // a few memory round trips per product instead of one per entry.  The
@@ -114,184 +113,72 @@ __device__ __forceinline__ double sd_pass(const ssx_sdnode &p,
 	return ps;
 }
 
-// ---- levels of small supernodes: one workgroup of 1024 lanes each ----
+// a workgroup's outputs: block blk of 256 / G outputs
+#define SD_BLOCK_OUTPUTS(G) (256 / (G))
 
-__global__ void __launch_bounds__(1024)
-k_sd_fwd_small(const int *__restrict__ lev_sn,
+// Forward step of one level: a workgroup takes a block of one supernode's
+// outputs, y rows (y = M b~) then t rows (t = K b~ + the children's
+// updates passing through); b~ is gathered into LDS by every workgroup of
+// the supernode.
+__global__ void __launch_bounds__(256)
+k_sd_fwd(const ssx_sdwork *__restrict__ wk,
 		const ssx_sdnode *__restrict__ node, const double *__restrict__ val,
 		const int *__restrict__ idx, const double *__restrict__ b,
 		double *__restrict__ y, double *__restrict__ t)
 {
-	__shared__ double w[SD_SMALL_NC], ys[SD_SMALL_NC], tp[SD_SMALL_NU];
-	const ssx_sdnode p = node[lev_sn[blockIdx.x]];
+	__shared__ double v[SD_MAX_VEC];
+	const ssx_sdwork w = wk[blockIdx.x];
+	const ssx_sdnode p = node[w.sn];
 	const int nc = p.nc, nu = p.nu;
-	for (int i = threadIdx.x; i < nc; i += 1024)
-		w[i] = sd_beff(p, node, idx, b, t, i);
-	for (int r = threadIdx.x; r < nu; r += 1024)
-		tp[r] = sd_pass(p, node, idx, t, r);
+	const int G = sd_group(nc), gl = threadIdx.x & (G - 1);
+	const int o = w.blk * SD_BLOCK_OUTPUTS(G) + threadIdx.x / G;
+	const bool trow = o >= nc && o < nc + nu;
+	// the pass-through loads go out first, with the gathers of b~
+	const double ps = trow && gl == 0 ? sd_pass(p, node, idx, t, o - nc)
+		: 0.0;
+	for (int k = threadIdx.x; k < nc; k += 256)
+		v[k] = sd_beff(p, node, idx, b, t, k);
 	__syncthreads();
-	const int G = sd_group(nc);
-	const int gl = threadIdx.x & (G - 1), grp = threadIdx.x / G;
-	const int per = 1024 / G;
-	const double *M = val + p.mroff;
-	for (int i = grp; i < nc; i += per) {
-		const double s = sd_group_dot(M + (size_t) i * nc, w, 0, i + 1, gl, G);
-		if (gl == 0) {
-			ys[i] = s;
-			y[p.c0 + i] = s;
-		}
-	}
-	__syncthreads();
-	const double *L = val + p.lroff;
-	for (int r = grp; r < nu; r += per) {
-		const double s = sd_group_dot(L + (size_t) r * nc, ys, 0, nc, gl, G);
+	if (o < nc) {
+		const double s = sd_group_dot(val + p.mroff + (size_t) o * nc, v, 0,
+				o + 1, gl, G);
 		if (gl == 0)
-			t[p.toff + r] = s + tp[r];
+			y[p.c0 + o] = s;
+	} else if (trow) {
+		const int r = o - nc;
+		const double s = sd_group_dot(val + p.kroff + (size_t) r * nc, v, 0,
+				nc, gl, G);
+		if (gl == 0)
+			t[p.toff + r] = s + ps;
 	}
 }
 
-__global__ void __launch_bounds__(1024)
-k_sd_bwd_small(const int *__restrict__ lev_sn,
+// Backward step of one level: x_j = (M^T y)_j - (K^T x_U)_j for a block of
+// one supernode's columns j; y of the supernode and x of its rows below
+// (computed by the levels above) staged in LDS.
+__global__ void __launch_bounds__(256)
+k_sd_bwd(const ssx_sdwork *__restrict__ wk,
 		const ssx_sdnode *__restrict__ node, const double *__restrict__ val,
 		const int *__restrict__ idx, const double *__restrict__ y,
 		double *__restrict__ x)
 {
-	__shared__ double xu[SD_SMALL_NU], g[SD_SMALL_NC];
-	const ssx_sdnode p = node[lev_sn[blockIdx.x]];
+	__shared__ double v[SD_MAX_VEC];
+	const ssx_sdwork w = wk[blockIdx.x];
+	const ssx_sdnode p = node[w.sn];
 	const int nc = p.nc, nu = p.nu;
-	for (int r = threadIdx.x; r < nu; r += 1024)
-		xu[r] = x[idx[p.uoff + r]];
+	for (int k = threadIdx.x; k < nc + nu; k += 256)
+		v[k] = k < nc ? y[p.c0 + k] : x[idx[p.uoff + k - nc]];
 	__syncthreads();
-	int G = sd_group(nu);
-	int gl = threadIdx.x & (G - 1), grp = threadIdx.x / G;
-	const double *L = val + p.loff;
-	for (int j = grp; j < nc; j += 1024 / G) {
-		const double s = sd_group_dot(L + (size_t) j * nu, xu, 0, nu, gl, G);
-		if (gl == 0)
-			g[j] = y[p.c0 + j] - s;
-	}
-	__syncthreads();
-	G = sd_group(nc);
-	gl = threadIdx.x & (G - 1);
-	grp = threadIdx.x / G;
-	const double *M = val + p.moff;
-	for (int j = grp; j < nc; j += 1024 / G) {
-		const double s = sd_group_dot(M + (size_t) j * nc, g, j, nc, gl, G);
-		if (gl == 0)
-			x[p.c0 + j] = s;
-	}
-}
-
-// ---- split levels: (supernode, block of outputs) work items ----
-
-// b~ into e and the pass-through part of t, one workgroup per supernode
-__global__ void __launch_bounds__(256)
-k_sd_fwd_beff(const int *__restrict__ lev_sn,
-		const ssx_sdnode *__restrict__ node, const int *__restrict__ idx,
-		const double *__restrict__ b, double *__restrict__ e,
-		double *__restrict__ t)
-{
-	const ssx_sdnode p = node[lev_sn[blockIdx.x]];
-	for (int i = threadIdx.x; i < p.nc; i += 256)
-		e[p.c0 + i] = sd_beff(p, node, idx, b, t, i);
-	for (int r = threadIdx.x; r < p.nu; r += 256)
-		t[p.toff + r] = sd_pass(p, node, idx, t, r);
-}
-
-// the workgroup's outputs: block blk of 256 / G outputs
-#define SD_BLOCK_OUTPUTS(G) (256 / (G))
-
-// y = M b~
-__global__ void __launch_bounds__(256)
-k_sd_fwd_y(const ssx_sdwork *__restrict__ wk,
-		const ssx_sdnode *__restrict__ node, const double *__restrict__ val,
-		const double *__restrict__ e, double *__restrict__ y)
-{
-	__shared__ double v[SD_MAX_VEC];
-	const ssx_sdwork w = wk[blockIdx.x];
-	const ssx_sdnode p = node[w.sn];
-	const int nc = p.nc;
-	for (int k = threadIdx.x; k < nc; k += 256)
-		v[k] = e[p.c0 + k];
-	__syncthreads();
-	const int G = sd_group(nc), gl = threadIdx.x & (G - 1);
-	const int i = w.blk * SD_BLOCK_OUTPUTS(G) + threadIdx.x / G;
-	if (i >= nc)
-		return;
-	const double s = sd_group_dot(val + p.mroff + (size_t) i * nc, v, 0,
-			i + 1, gl, G);
-	if (gl == 0)
-		y[p.c0 + i] = s;
-}
-
-// t += L21 y (t holds the pass-through part)
-__global__ void __launch_bounds__(256)
-k_sd_fwd_t(const ssx_sdwork *__restrict__ wk,
-		const ssx_sdnode *__restrict__ node, const double *__restrict__ val,
-		const double *__restrict__ y, double *__restrict__ t)
-{
-	__shared__ double v[SD_MAX_VEC];
-	const ssx_sdwork w = wk[blockIdx.x];
-	const ssx_sdnode p = node[w.sn];
-	const int nc = p.nc;
-	for (int k = threadIdx.x; k < nc; k += 256)
-		v[k] = y[p.c0 + k];
-	__syncthreads();
-	const int G = sd_group(nc), gl = threadIdx.x & (G - 1);
-	const int r = w.blk * SD_BLOCK_OUTPUTS(G) + threadIdx.x / G;
-	if (r >= p.nu)
-		return;
-	const double s = sd_group_dot(val + p.lroff + (size_t) r * nc, v, 0, nc,
-			gl, G);
-	if (gl == 0)
-		t[p.toff + r] = s + t[p.toff + r];
-}
-
-// e = y - L21^T x_U
-__global__ void __launch_bounds__(256)
-k_sd_bwd_g(const ssx_sdwork *__restrict__ wk,
-		const ssx_sdnode *__restrict__ node, const double *__restrict__ val,
-		const int *__restrict__ idx, const double *__restrict__ y,
-		const double *__restrict__ x, double *__restrict__ e)
-{
-	__shared__ double v[SD_MAX_VEC];
-	const ssx_sdwork w = wk[blockIdx.x];
-	const ssx_sdnode p = node[w.sn];
-	const int nu = p.nu;
-	for (int r = threadIdx.x; r < nu; r += 256)
-		v[r] = x[idx[p.uoff + r]];
-	__syncthreads();
-	const int G = sd_group(nu), gl = threadIdx.x & (G - 1);
-	const int j = w.blk * SD_BLOCK_OUTPUTS(G) + threadIdx.x / G;
-	if (j >= p.nc)
-		return;
-	const double s = sd_group_dot(val + p.loff + (size_t) j * nu, v, 0, nu,
-			gl, G);
-	if (gl == 0)
-		e[p.c0 + j] = y[p.c0 + j] - s;
-}
-
-// x = M^T e
-__global__ void __launch_bounds__(256)
-k_sd_bwd_x(const ssx_sdwork *__restrict__ wk,
-		const ssx_sdnode *__restrict__ node, const double *__restrict__ val,
-		const double *__restrict__ e, double *__restrict__ x)
-{
-	__shared__ double v[SD_MAX_VEC];
-	const ssx_sdwork w = wk[blockIdx.x];
-	const ssx_sdnode p = node[w.sn];
-	const int nc = p.nc;
-	for (int k = threadIdx.x; k < nc; k += 256)
-		v[k] = e[p.c0 + k];
-	__syncthreads();
-	const int G = sd_group(nc), gl = threadIdx.x & (G - 1);
+	const int G = sd_group(nc + nu), gl = threadIdx.x & (G - 1);
 	const int j = w.blk * SD_BLOCK_OUTPUTS(G) + threadIdx.x / G;
 	if (j >= nc)
 		return;
 	const double s = sd_group_dot(val + p.moff + (size_t) j * nc, v, j, nc,
 			gl, G);
+	const double q = sd_group_dot(val + p.koff + (size_t) j * nu, v + nc, 0,
+			nu, gl, G);
 	if (gl == 0)
-		x[p.c0 + j] = s;
+		x[p.c0 + j] = s - q;
 }
 
 extern "C" int ssxk_sd_blocks(int nout, int rowlen)
@@ -322,86 +209,26 @@ extern "C" int ssxk_sd_scatter(void *st, int n, const int *perm,
 	return SSX_OK;
 }
 
-extern "C" int ssxk_sd_fwd_small(void *st, int nsn, const int *lev_sn,
+extern "C" int ssxk_sd_fwd(void *st, int nwk, const ssx_sdwork *wk,
 		const ssx_sdnode *node, const double *val, const int *idx,
 		const double *b, double *y, double *t)
 {
-	if (nsn <= 0)
-		return SSX_OK;
-	hipLaunchKernelGGL(k_sd_fwd_small, dim3(nsn), dim3(1024), 0,
-			(hipStream_t) st, lev_sn, node, val, idx, b, y, t);
-	SD_CHECK("k_sd_fwd_small");
-	return SSX_OK;
-}
-
-extern "C" int ssxk_sd_fwd_beff(void *st, int nsn, const int *lev_sn,
-		const ssx_sdnode *node, const int *idx, const double *b, double *e,
-		double *t)
-{
-	if (nsn <= 0)
-		return SSX_OK;
-	hipLaunchKernelGGL(k_sd_fwd_beff, dim3(nsn), dim3(256), 0,
-			(hipStream_t) st, lev_sn, node, idx, b, e, t);
-	SD_CHECK("k_sd_fwd_beff");
-	return SSX_OK;
-}
-
-extern "C" int ssxk_sd_fwd_y(void *st, int nwk, const ssx_sdwork *wk,
-		const ssx_sdnode *node, const double *val, const double *e,
-		double *y)
-{
 	if (nwk <= 0)
 		return SSX_OK;
-	hipLaunchKernelGGL(k_sd_fwd_y, dim3(nwk), dim3(256), 0,
-			(hipStream_t) st, wk, node, val, e, y);
-	SD_CHECK("k_sd_fwd_y");
+	hipLaunchKernelGGL(k_sd_fwd, dim3(nwk), dim3(256), 0, (hipStream_t) st,
+			wk, node, val, idx, b, y, t);
+	SD_CHECK("k_sd_fwd");
 	return SSX_OK;
 }
 
-extern "C" int ssxk_sd_fwd_t(void *st, int nwk, const ssx_sdwork *wk,
-		const ssx_sdnode *node, const double *val, const double *y,
-		double *t)
-{
-	if (nwk <= 0)
-		return SSX_OK;
-	hipLaunchKernelGGL(k_sd_fwd_t, dim3(nwk), dim3(256), 0,
-			(hipStream_t) st, wk, node, val, y, t);
-	SD_CHECK("k_sd_fwd_t");
-	return SSX_OK;
-}
-
-extern "C" int ssxk_sd_bwd_small(void *st, int nsn, const int *lev_sn,
+extern "C" int ssxk_sd_bwd(void *st, int nwk, const ssx_sdwork *wk,
 		const ssx_sdnode *node, const double *val, const int *idx,
 		const double *y, double *x)
 {
-	if (nsn <= 0)
-		return SSX_OK;
-	hipLaunchKernelGGL(k_sd_bwd_small, dim3(nsn), dim3(1024), 0,
-			(hipStream_t) st, lev_sn, node, val, idx, y, x);
-	SD_CHECK("k_sd_bwd_small");
-	return SSX_OK;
-}
-
-extern "C" int ssxk_sd_bwd_g(void *st, int nwk, const ssx_sdwork *wk,
-		const ssx_sdnode *node, const double *val, const int *idx,
-		const double *y, const double *x, double *e)
-{
 	if (nwk <= 0)
 		return SSX_OK;
-	hipLaunchKernelGGL(k_sd_bwd_g, dim3(nwk), dim3(256), 0,
-			(hipStream_t) st, wk, node, val, idx, y, x, e);
-	SD_CHECK("k_sd_bwd_g");
-	return SSX_OK;
-}
-
-extern "C" int ssxk_sd_bwd_x(void *st, int nwk, const ssx_sdwork *wk,
-		const ssx_sdnode *node, const double *val, const double *e,
-		double *x)
-{
-	if (nwk <= 0)
-		return SSX_OK;
-	hipLaunchKernelGGL(k_sd_bwd_x, dim3(nwk), dim3(256), 0,
-			(hipStream_t) st, wk, node, val, e, x);
-	SD_CHECK("k_sd_bwd_x");
+	hipLaunchKernelGGL(k_sd_bwd, dim3(nwk), dim3(256), 0, (hipStream_t) st,
+			wk, node, val, idx, y, x);
+	SD_CHECK("k_sd_bwd");
 	return SSX_OK;
 }

