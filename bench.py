@@ -325,7 +325,7 @@ def ns_tgv_case(S, torch, stream, steps=10):
 
 
 def ns_channel_case(S, torch, stream, steps=3, ntracers=1_000_000,
-                    prec="fp32", coarse_direct=16384):
+                    prec="fp32", coarse_direct=0):
     """The channel configuration: 24x16x24 hexahedra at N = 9 (n = 10,
     9.2M nodes) on [0, 4 pi] x [-1, 1] x [0, 4 pi / 3], tanh wall
     stretching 2.0 in y, periodic in x and z, visc = 1/180 (Re_tau = 180
@@ -333,11 +333,12 @@ def ns_channel_case(S, torch, stream, steps=3, ntracers=1_000_000,
     balance), dt = 2e-3; Poiseuille profile plus 10% divergence-free noise
     at all three history levels.  Each step: OIFS convection, FCG +
     p-MG pressure solve with the FP32 smoother (its coarsest level,
-    9792 unique nodes, solved directly when coarse_direct >= that:
-    SSX_OPT_COARSE_DIRECT), the three velocity Helmholtz solves as one
-    batch, then the velocity history updated and `ntracers` RK2 tracers
-    advanced over the step.  The first step is a warm-up (graph capture,
-    reduced-precision copies, the coarse inverse)."""
+    9792 unique nodes, solved by the sparse direct factor, or by the dense
+    inverse when coarse_direct >= that: SSX_OPT_COARSE_DIRECT), the three
+    velocity Helmholtz solves as one batch, then the velocity history
+    updated and `ntracers` RK2 tracers advanced over the step.  The first
+    step is a warm-up (graph capture, reduced-precision copies, the coarse
+    factor)."""
     from ssenss_amd.mesh import box_mesh, PERIODIC, NOSLIP
     visc, dt = 1 / 180, 2e-3
     lx, lz = 4 * np.pi, 4 * np.pi / 3
@@ -393,6 +394,8 @@ def ns_channel_case(S, torch, stream, steps=3, ntracers=1_000_000,
                            "Re_tau 180, dt 2e-3", "local_dofs": mesh.ntot,
                "context_setup_s": tset, "pressure_smoother": prec,
                "coarse_direct_max_unknowns": coarse_direct,
+               "coarse_solve": ("dense inverse" if coarse_direct >= 9792
+                                else "sparse factor"),
                "steps": steps, "seconds_per_step_first": ts[0],
                "seconds_per_step": float(np.median(ts[1:])),
                "iterations_per_step": its[-1],

@@ -34,6 +34,7 @@
  *              gather with the children in a fixed order: no atomics, the
  *              result is the same on every run.
  */
+#pragma GCC optimize("O3")
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -195,6 +196,52 @@ static int nd_rec(nd_t *d, int *set, int m)
 
 /* ---- dense kernels of the numeric factorisation (column-major) ---- */
 
+/* F[i][j] -= sum_{k in [kb, ke)} F[i][k] F[j][k] for the lower part of
+ * columns j >= ke: 8 x 4 tiles with the sums in registers (each entry of
+ * the block's columns is loaded once per tile, not once per product);
+ * built for AVX2/FMA where the host has them */
+__attribute__((target_clones("arch=haswell", "default")))
+void ssx_sd_trailing_update(double *F, int nr, int kb, int ke)
+{
+	for (int j0 = ke; j0 < nr; j0 += 4) {
+		const int nj = nr - j0 < 4 ? nr - j0 : 4;
+		for (int i0 = j0; i0 < nr; i0 += 8) {
+			const int ni = nr - i0 < 8 ? nr - i0 : 8;
+			double acc[4][8];
+			for (int jj = 0; jj < 4; ++jj)
+				for (int ii = 0; ii < 8; ++ii)
+					acc[jj][ii] = 0.0;
+			if (ni == 8 && nj == 4) {
+				for (int k = kb; k < ke; ++k) {
+					const double *fk = F + (size_t) k * nr;
+					const double *a = fk + i0;
+					const double b0 = fk[j0], b1 = fk[j0 + 1];
+					const double b2 = fk[j0 + 2], b3 = fk[j0 + 3];
+					for (int ii = 0; ii < 8; ++ii) {
+						acc[0][ii] += a[ii] * b0;
+						acc[1][ii] += a[ii] * b1;
+						acc[2][ii] += a[ii] * b2;
+						acc[3][ii] += a[ii] * b3;
+					}
+				}
+			} else {
+				for (int k = kb; k < ke; ++k) {
+					const double *fk = F + (size_t) k * nr;
+					for (int jj = 0; jj < nj; ++jj)
+						for (int ii = 0; ii < ni; ++ii)
+							acc[jj][ii] += fk[i0 + ii] * fk[j0 + jj];
+				}
+			}
+			for (int jj = 0; jj < nj; ++jj) {
+				double *fj = F + (size_t) (j0 + jj) * nr;
+				for (int ii = 0; ii < ni; ++ii)
+					if (i0 + ii >= j0 + jj)
+						fj[i0 + ii] -= acc[jj][ii];
+			}
+		}
+	}
+}
+
 /* partial Cholesky of the first nc columns of the nr x nr lower front F;
  * F22 becomes the update matrix.  Returns 0, or 1 on a non-positive pivot */
 static int front_factor(double *F, int nr, int nc, double *flops)
@@ -220,18 +267,7 @@ static int front_factor(double *F, int nr, int nc, double *flops)
 			for (int i = j + 1; i < nr; ++i)
 				fj[i] *= ir;
 		}
-		/* trailing update of columns ke..nr-1 by the block */
-		for (int j = ke; j < nr; ++j) {
-			double *fj = F + (size_t) j * nr;
-			for (int k = kb; k < ke; ++k) {
-				const double *fk = F + (size_t) k * nr;
-				const double l = fk[j];
-				if (l == 0.0)
-					continue;
-				for (int i = j; i < nr; ++i)
-					fj[i] -= fk[i] * l;
-			}
-		}
+		ssx_sd_trailing_update(F, nr, kb, ke);
 	}
 	double fl = 0.0;
 	for (int j = 0; j < nc; ++j)
