@@ -334,11 +334,9 @@ void ssx_coarse_direct_free(ssx_ctx *c)
 {
 	ssx_sd_free(c->sd);
 	c->sd = NULL;
-	hipFree(c->sd_map);
-	hipFree(c->sd_b);
-	hipFree(c->sd_x);
-	c->sd_map = NULL;
-	c->sd_b = c->sd_x = NULL;
+	hipFree(c->sd_gmap);
+	hipFree(c->sd_smap);
+	c->sd_gmap = c->sd_smap = NULL;
 	c->sd_nu = 0;
 	hipFree(c->cd_inv);
 	c->cd_inv = NULL;
@@ -458,11 +456,9 @@ static int coarse_sparse_build(ssx_ctx *c, int l, int op, int pres)
 		return c->sd_bad ? SSX_ENUMERIC : SSX_OK;
 	ssx_sd_free(c->sd);
 	c->sd = NULL;
-	hipFree(c->sd_map);
-	hipFree(c->sd_b);
-	hipFree(c->sd_x);
-	c->sd_map = NULL;
-	c->sd_b = c->sd_x = NULL;
+	hipFree(c->sd_gmap);
+	hipFree(c->sd_smap);
+	c->sd_gmap = c->sd_smap = NULL;
 	c->sd_nu = nu;
 	c->sd_op = op;
 	c->sd_pres = pres;
@@ -556,6 +552,45 @@ static int coarse_sparse_build(ssx_ctx *c, int l, int op, int pres)
 			f->perm[i] = map[f->perm[i]];
 		rc = ssx_sd_upload(f);
 	}
+	/* and straight between the level's E-vectors and the factor (a
+	 * non-singular level: the masked nodes' solution is zero) */
+	int *cmap = NULL, *gm = NULL, *sm = NULL;
+	const size_t nt = L->ntot;
+	if (rc == SSX_OK) {
+		cmap = malloc(sizeof(int) * nt);
+		gm = malloc(sizeof(int) * (size_t) n);
+		sm = malloc(sizeof(int) * nt);
+		if (!cmap || !gm || !sm) {
+			ssx_set_error("out of host memory (sparse coarse maps)");
+			rc = SSX_ENOMEM;
+		} else if (hipMemcpy(cmap, cs->cmap, sizeof(int) * nt,
+				hipMemcpyDeviceToHost) != hipSuccess) {
+			ssx_set_error("hipMemcpy of the coarse copy map failed");
+			rc = SSX_EHIP;
+		}
+	}
+	if (rc == SSX_OK) {
+		for (int u = 0; u < nu; ++u)
+			rid[u] = -1;            /* reused: new index of node u */
+		for (int i = 0; i < n; ++i) {
+			gm[i] = rep[f->perm[i]];
+			rid[f->perm[i]] = i;
+		}
+		for (size_t q = 0; q < nt; ++q)
+			sm[q] = rid[cmap[q]];
+		if (hipMalloc((void **) &c->sd_gmap, sizeof(int) * (size_t) n)
+				!= hipSuccess
+				|| hipMalloc((void **) &c->sd_smap, sizeof(int) * nt)
+				!= hipSuccess
+				|| hipMemcpy(c->sd_gmap, gm, sizeof(int) * (size_t) n,
+					hipMemcpyHostToDevice) != hipSuccess
+				|| hipMemcpy(c->sd_smap, sm, sizeof(int) * nt,
+					hipMemcpyHostToDevice) != hipSuccess) {
+			ssx_set_error("device maps of the sparse coarse solve");
+			rc = SSX_ENOMEM;
+		}
+	}
+	free(cmap); free(gm); free(sm);
 	free(rp); free(ci); free(av); free(mk); free(rep); free(rid);
 	free(map); free(rr); free(rc_); free(ra); free(xyz);
 	if (rc != SSX_OK) {
@@ -573,15 +608,16 @@ static int coarse_sparse_solve(ssx_ctx *c, ssx_level *L, int pres,
 	ssx_coarse *cs = &L->cs;
 	void *st = c->stream;
 	const int nu = cs->nu;
+	if (!c->sd_sing)                /* E-vector -> factor -> E-vector */
+		return ssx_sd_solve_dev_map(c->sd, st, f, c->sd_gmap, x,
+				c->sd_smap, (int) L->ntot);
 	TRYQ(ssxk_gather_u(st, nu, cs->rep, f, cs->f));
 	if (!pres)
 		TRYQ(ssxk_mul(st, (size_t) nu, cs->f, cs->mask, cs->f));
-	if (c->sd_sing)
-		TRYQ(unull(c, cs->f));
+	TRYQ(unull(c, cs->f));
 	TRYQ(ssxk_fill(st, cs->x, 0.0, (size_t) nu));
 	TRYQ(ssx_sd_solve_dev(c->sd, st, cs->f, cs->x));
-	if (c->sd_sing)
-		TRYQ(unull(c, cs->x));
+	TRYQ(unull(c, cs->x));
 	return ssxk_scatter_u(st, L->ntot, cs->cmap, cs->x, x);
 }
 

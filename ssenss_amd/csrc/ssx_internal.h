@@ -150,8 +150,8 @@ struct ssx_ctx {
 	struct ssx_sdfactor *sd;     /* sparse factor of the coarsest operator */
 	int sd_op, sd_pres, sd_bad, sd_sing, sd_nu;
 	double sd_lambda;
-	int *sd_map;                 /* [sd->n] unique node of system index   */
-	double *sd_b, *sd_x;         /* [sd->n] system right-hand side, solution */
+	int *sd_gmap, *sd_smap;      /* coarsest E-vector copy of each new index,
+	                                new index of each E-vector copy (-1) */
 	long p_pass_np;              /* particles of the last pass-by-pass step */
 	long long cnt[4];
 	double ax_us[SSX_LMAX][3][SSX_AX_NVARIANTS];  /* autotuner timings  */
@@ -205,6 +205,10 @@ int ssx_sd_upload(ssx_sdfactor *f);
 /* x = A^-1 b on the device, b and x in the system's numbering (n) */
 int ssx_sd_solve_dev(ssx_sdfactor *f, void *stream, const double *b,
 		double *x);
+/* the same with b read as src[gmap[i]] for new index i (gmap: n entries)
+ * and x written to dst[q] = x_new[smap[q]] (0 where smap[q] < 0), q < m */
+int ssx_sd_solve_dev_map(ssx_sdfactor *f, void *stream, const double *src,
+		const int *gmap, double *dst, const int *smap, int m);
 void ssx_sd_free(ssx_sdfactor *f);
 /* workgroups of a product: nout outputs of rows of length rowlen */
 int ssxk_sd_blocks(int nout, int rowlen);
@@ -212,6 +216,9 @@ int ssxk_sd_gather(void *stream, int n, const int *perm, const double *src,
 		double *dst);
 int ssxk_sd_scatter(void *stream, int n, const int *perm, const double *src,
 		double *dst);
+/* dst[q] = src[map[q]] (0 where map[q] < 0), q < m */
+int ssxk_sd_scatter_map(void *stream, int m, const int *map,
+		const double *src, double *dst);
 int ssxk_sd_fwd(void *stream, int nwk, const ssx_sdwork *wk,
 		const ssx_sdnode *node, const double *val, const int *idx,
 		const double *b, double *y, double *t);

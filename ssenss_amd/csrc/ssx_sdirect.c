@@ -839,10 +839,33 @@ int ssx_sd_upload(ssx_sdfactor *f)
 	return rc;
 }
 
-/* one forward and one backward launch per level of the tree */
+static int sd_levels(ssx_sdfactor *f, void *st);
+
+int ssx_sd_solve_dev_map(ssx_sdfactor *f, void *st, const double *src,
+		const int *gmap, double *dst, const int *smap, int m)
+{
+	int rc = ssxk_sd_gather(st, f->n, gmap, src, f->d_b);
+	if (rc == SSX_OK)
+		rc = sd_levels(f, st);
+	if (rc == SSX_OK)
+		rc = ssxk_sd_scatter_map(st, m, smap, f->d_x, dst);
+	return rc;
+}
+
 int ssx_sd_solve_dev(ssx_sdfactor *f, void *st, const double *b, double *x)
 {
 	int rc = ssxk_sd_gather(st, f->n, f->d_perm, b, f->d_b);
+	if (rc == SSX_OK)
+		rc = sd_levels(f, st);
+	if (rc == SSX_OK)
+		rc = ssxk_sd_scatter(st, f->n, f->d_perm, f->d_x, x);
+	return rc;
+}
+
+/* one forward and one backward launch per level of the tree: d_b -> d_x */
+static int sd_levels(ssx_sdfactor *f, void *st)
+{
+	int rc = SSX_OK;
 	for (int l = 0; rc == SSX_OK && l < f->nlev; ++l) {
 		const int *o = f->wk_off + 2 * l;
 		rc = ssxk_sd_fwd(st, o[1] - o[0], f->d_wk + o[0], f->d_node,
@@ -853,7 +876,5 @@ int ssx_sd_solve_dev(ssx_sdfactor *f, void *st, const double *b, double *x)
 		rc = ssxk_sd_bwd(st, o[2] - o[1], f->d_wk + o[1], f->d_node,
 				f->d_val, f->d_idx, f->d_y, f->d_x);
 	}
-	if (rc == SSX_OK)
-		rc = ssxk_sd_scatter(st, f->n, f->d_perm, f->d_x, x);
 	return rc;
 }

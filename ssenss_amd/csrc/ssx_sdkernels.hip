@@ -79,6 +79,18 @@ k_sd_scatter(int n, const int *__restrict__ perm, const double *__restrict__ src
 		dst[perm[i]] = src[i];
 }
 
+// dst[q] = src[map[q]], or 0 where map[q] < 0
+__global__ void __launch_bounds__(256)
+k_sd_scatter_map(int m, const int *__restrict__ map,
+		const double *__restrict__ src, double *__restrict__ dst)
+{
+	const int q = blockIdx.x * 256 + threadIdx.x;
+	if (q < m) {
+		const int i = map[q];
+		dst[q] = i >= 0 ? src[i] : 0.0;
+	}
+}
+
 // b~_i = b_{c0+i} - (child 0's update) - (child 1's update)
 __device__ __forceinline__ double sd_beff(const ssx_sdnode &p,
 		const ssx_sdnode *__restrict__ node, const int *__restrict__ idx,
@@ -206,6 +218,17 @@ extern "C" int ssxk_sd_scatter(void *st, int n, const int *perm,
 	hipLaunchKernelGGL(k_sd_scatter, dim3((n + 255) / 256), dim3(256), 0,
 			(hipStream_t) st, n, perm, src, dst);
 	SD_CHECK("k_sd_scatter");
+	return SSX_OK;
+}
+
+extern "C" int ssxk_sd_scatter_map(void *st, int m, const int *map,
+		const double *src, double *dst)
+{
+	if (m <= 0)
+		return SSX_OK;
+	hipLaunchKernelGGL(k_sd_scatter_map, dim3((m + 255) / 256), dim3(256), 0,
+			(hipStream_t) st, m, map, src, dst);
+	SD_CHECK("k_sd_scatter_map");
 	return SSX_OK;
 }
 
