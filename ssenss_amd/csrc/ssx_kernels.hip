@@ -326,7 +326,7 @@ __device__ __forceinline__ void block_sum4(double *v, double (*sh)[256])
 		sh[c][threadIdx.x] = v[c];
 	__syncthreads();
 #pragma unroll
-	for (int st = 128; st > 0; st >>= 1) {
+	for (int st = 128; st >= 64; st >>= 1) {
 		if ((int) threadIdx.x < st) {
 #pragma unroll
 			for (int c = 0; c < 4; ++c)
@@ -334,6 +334,24 @@ __device__ __forceinline__ void block_sum4(double *v, double (*sh)[256])
 		}
 		__syncthreads();
 	}
+	// steps below 64 in wavefront 0 by shuffles (the same additions in the
+	// same order); the totals go back to sh[c][0]
+	if (threadIdx.x < 64) {
+		double w[4];
+#pragma unroll
+		for (int c = 0; c < 4; ++c)
+			w[c] = sh[c][threadIdx.x];
+#pragma unroll
+		for (int st = 32; st > 0; st >>= 1)
+#pragma unroll
+			for (int c = 0; c < 4; ++c)
+				w[c] += __shfl_down(w[c], st, 64);
+		if (threadIdx.x == 0)
+#pragma unroll
+			for (int c = 0; c < 4; ++c)
+				sh[c][0] = w[c];
+	}
+	__syncthreads();
 }
 
 // u += a p; r -= a ap; z = dinv r; partial <r,z>_w, <r,r>_w and, for the
@@ -777,19 +795,29 @@ k_cg_reduce(int mode, const double *__restrict__ part, long long cnt,
 		sh[c][threadIdx.x] = acc;
 	}
 	__syncthreads();
-	for (int st = 512; st > 0; st >>= 1) {
+	for (int st = 512; st >= 64; st >>= 1) {
 		if ((int) threadIdx.x < st)
 			for (int c = 0; c < 4; ++c)
 				sh[c][threadIdx.x] += sh[c][threadIdx.x + st];
 		__syncthreads();
 	}
+	// the tree's steps below 64 inside wavefront 0, by shuffles: lane t
+	// adds lane t + st's value as the LDS step would, without barriers
+	if (threadIdx.x >= 64)
+		return;
+	double v[4];
+	for (int c = 0; c < 4; ++c)
+		v[c] = sh[c][threadIdx.x];
+	for (int st = 32; st > 0; st >>= 1)
+		for (int c = 0; c < 4; ++c)
+			v[c] += __shfl_down(v[c], st, 64);
 	if (threadIdx.x != 0)
 		return;
 	if (mode == 0)
-		cg_scalars0(sh[0][0], scal, o);
+		cg_scalars0(v[0], scal, o);
 	else
-		cg_scalars1(sh[0][0], sh[1][0], sh[2][0], sh[3][0], scal, o, flags,
-				eps, maxit, nunique, pressure);
+		cg_scalars1(v[0], v[1], v[2], v[3], scal, o, flags, eps, maxit,
+				nunique, pressure);
 }
 
 __global__ void k_cg_tol(double *scal, int dst, int src, double eps,
