@@ -288,6 +288,19 @@ __device__ __forceinline__ void ax_elem_partial(double pd, double *part,
 	constexpr int P2 = N2 <= 4 ? 4 : N2 <= 16 ? 16 : N2 <= 32 ? 32
 		: N2 <= 64 ? 64 : N2 <= 128 ? 128 : 256;
 	static_assert(N2 <= 256, "team larger than the partial tree");
+	if constexpr (N2 == P2 && P2 <= 64) {
+		// a team fills an aligned segment of one wavefront: the same tree
+		// (lane l adds lane l + st) by shuffles, no LDS and no barrier
+		double v = pd;
+#pragma unroll
+		for (int st = P2 / 2; st > 0; st >>= 1)
+			v += __shfl_down(v, st, P2);
+		const int team = threadIdx.x / N2;
+		const int e = eblk * TEAMS + team;
+		if (threadIdx.x - team * N2 == 0 && e < nel)
+			part[e] = v;
+		return;
+	}
 	__shared__ double s_red[TEAMS * P2];
 	const int team = threadIdx.x / N2;
 	const int tid = threadIdx.x - team * N2;
