@@ -233,15 +233,23 @@ __device__ __forceinline__ double ax_team(int eblk, int nel,
 	// The six factors of plane k + 1 are loaded before plane k is computed,
 	// so two planes' loads are in flight per lane instead of one.  The loop
 	// takes two planes a turn with two register sets, so no loaded value is
-	// copied (a copy would wait for its load); a load past the last plane
-	// reads the last plane again, unused.
+	// copied (a copy would wait for its load); the last one or two planes
+	// are taken after it, with no load past the last plane (a second read
+	// of a plane streamed with the non-temporal hint went back to HBM).
+	int k = 0;
 #pragma unroll 1
-	for (int k = 0; k < N; k += 2) {
-		ax_factors<NT, N3>(g, (k + 1 < N ? k + 1 : N - 1) * N2 + tid, fb);
+	for (; k + 2 < N; k += 2) {
+		ax_factors<NT, N3>(g, (k + 1) * N2 + tid, fb);
 		plane_step(k, fa);
-		ax_factors<NT, N3>(g, (k + 2 < N ? k + 2 : N - 1) * N2 + tid, fa);
-		if (k + 1 < N)
-			plane_step(k + 1, fb);
+		ax_factors<NT, N3>(g, (k + 2) * N2 + tid, fa);
+		plane_step(k + 1, fb);
+	}
+	if constexpr (N % 2 == 0) {
+		ax_factors<NT, N3>(g, (N - 1) * N2 + tid, fb);
+		plane_step(N - 2, fa);
+		plane_step(N - 1, fb);
+	} else {
+		plane_step(N - 1, fa);
 	}
 	T ms[OP == 1 ? N : 1];
 	if (OP == 1) {
