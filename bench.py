@@ -432,6 +432,10 @@ def batch_mg_case(S, torch, nprob=64, seed=64, mrange=(8, 24),
         c = S.Context(mesh)
         c.set_stream(torch.cuda.current_stream())
         c.set_option(S.OPT_SMOOTHER_PRECISION, code[prec])
+        # the batch solves every coarsest level in one batched CG pass; the
+        # one-by-one solves take the same coarse CG (not the sparse factor
+        # a lone problem gets by default), so both run one algorithm
+        c.set_option(S.OPT_COARSE_SPARSE, 0)
         # right-hand side: DSS(w detJ f) of a seeded N(0,1) nodal field,
         # drawn on the GPU (the mesh arrays stay on the host only for the
         # context set-up)
