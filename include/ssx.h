@@ -233,14 +233,15 @@ int ssx_set_diag_choice(ssx_ctx *ctx, int choice);
  * reduced-precision V-cycle (as SSX_OPT_COARSE_DIRECT) by a sparse
  * Cholesky factor when it has at most m unique nodes and more than the
  * dense solve takes: nested-dissection order, multifrontal factorisation
- * on the host at the first solve of each operator (lambda, null space;
- * about 3 s for 36k nodes), and the two triangular solves on the device, a
- * few launches per level of the separator tree.  Default
+ * at the first solve of each operator (lambda, null space) on the host,
+ * fronts of 1024 rows and more on the device through rocSOLVER / rocBLAS
+ * (about 0.5 s for 30k nodes, 3 s for 100k), and the two triangular solves
+ * on the device, a few launches per level of the separator tree.  Default
  * SSX_COARSE_SPARSE_DEFAULT; 0: Jacobi-PCG to the cycle's relative
  * tolerance.  A coarse operator that is not positive definite falls back
  * to the CG. */
 #define SSX_OPT_COARSE_SPARSE 11
-#define SSX_COARSE_SPARSE_DEFAULT 65536
+#define SSX_COARSE_SPARSE_DEFAULT 131072
 #define SSX_PREC_FP64 0
 #define SSX_PREC_FP32 1
 #define SSX_PREC_BF16 2

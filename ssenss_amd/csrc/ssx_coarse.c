@@ -541,8 +541,23 @@ static int coarse_sparse_build(ssx_ctx *c, int l, int op, int pres)
 	}
 	c->sd_sing = singular;
 	ssx_sdfactor *f = NULL;
+	/* the large fronts go to the device through rocSOLVER / rocBLAS */
+	if (rc == SSX_OK && !c->cd_blas) {
+		rocblas_handle h;
+		if (rocblas_create_handle(&h) != rocblas_status_success) {
+			ssx_set_error("rocblas_create_handle failed");
+			rc = SSX_EHIP;
+		} else {
+			c->cd_blas = h;
+		}
+	}
+	if (rc == SSX_OK && rocblas_set_stream((rocblas_handle) c->cd_blas,
+			st) != rocblas_status_success) {
+		ssx_set_error("rocblas_set_stream failed");
+		rc = SSX_EHIP;
+	}
 	if (rc == SSX_OK && n > 0)
-		rc = ssx_sd_factor(n, rr, rc_, ra, xyz, &f);
+		rc = ssx_sd_factor(n, rr, rc_, ra, xyz, c->cd_blas, st, &f);
 	else if (rc == SSX_OK)
 		rc = SSX_ENUMERIC;          /* nothing to factor: keep the CG */
 	if (rc == SSX_OK) {

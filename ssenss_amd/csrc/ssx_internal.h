@@ -160,7 +160,7 @@ struct ssx_ctx {
 /* ---- sparse direct solver (ssx_sdirect.c, ssx_sdkernels.hip) ---- */
 /* the device solve stages a supernode's vectors (at most SD_MAX_VEC
  * entries: its columns and the rows below them) in LDS */
-#define SD_MAX_VEC 4096
+#define SD_MAX_VEC 16384
 
 /* one supernode of the factor: columns [c0, c0+nc) of the new order, nu
  * rows below them (their new indices at idx[uoff..]), children ch0/ch1
@@ -189,6 +189,7 @@ typedef struct ssx_sdfactor {
 	double flops;
 	int *wk_off;                 /* [2 nlev + 1] work-table ranges: per
 	                                level, forward then backward         */
+	int *wk_vec;                 /* [2 nlev] LDS entries of those launches */
 	ssx_sdwork *wk;
 	/* device */
 	ssx_sdnode *d_node;
@@ -198,8 +199,10 @@ typedef struct ssx_sdfactor {
 	double *d_b, *d_y, *d_x, *d_t;
 } ssx_sdfactor;
 
+/* blas, stream: a rocblas handle and its stream to factor the large
+ * fronts on the device, or NULL (host only) */
 int ssx_sd_factor(int n, const int *rp, const int *ci, const double *a,
-		const double *xyz, ssx_sdfactor **out);
+		const double *xyz, void *blas, void *stream, ssx_sdfactor **out);
 int ssx_sd_solve_host(const ssx_sdfactor *f, const double *b, double *x);
 int ssx_sd_upload(ssx_sdfactor *f);
 /* x = A^-1 b on the device, b and x in the system's numbering (n) */
@@ -219,12 +222,14 @@ int ssxk_sd_scatter(void *stream, int n, const int *perm, const double *src,
 /* dst[q] = src[map[q]] (0 where map[q] < 0), q < m */
 int ssxk_sd_scatter_map(void *stream, int m, const int *map,
 		const double *src, double *dst);
+/* vec: LDS entries of the launch (the level's largest nc for the forward
+ * step, nc + nu for the backward one) */
 int ssxk_sd_fwd(void *stream, int nwk, const ssx_sdwork *wk,
 		const ssx_sdnode *node, const double *val, const int *idx,
-		const double *b, double *y, double *t);
+		const double *b, double *y, double *t, int vec);
 int ssxk_sd_bwd(void *stream, int nwk, const ssx_sdwork *wk,
 		const ssx_sdnode *node, const double *val, const int *idx,
-		const double *y, double *x);
+		const double *y, double *x, int vec);
 
 /* ---- assembled coarsest level (ssx_coarse.c) ---- */
 int ssx_coarse_build(ssx_ctx *c, int l, const double *gf, const double *mass,

@@ -40,6 +40,8 @@
 #include <string.h>
 
 #include <hip/hip_runtime_api.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include "ssx_internal.h"
 
@@ -296,12 +298,118 @@ static void tri_inverse(const double *L, int ld, int n, double *M)
 	}
 }
 
+/* ---- large fronts on the device (rocSOLVER / rocBLAS) ----
+ * The dense work of a front grows with the cube of its size and the top
+ * separators' fronts hold most of it: those of at least SD_GPU_FRONT rows
+ * are factored on the device: potrf of F11, trsm for L21, syrk for the
+ * update matrix, trtri for M and trmm for K = L21 M; M, K and the update
+ * matrix come back to the host. */
+#define SD_GPU_FRONT 1024
+
+typedef struct {
+	rocblas_handle h;
+	hipStream_t st;
+	double *dF, *dM, *dK;
+	size_t capF, capM, capK;
+	rocblas_int *dinfo;
+} sd_gpu;
+
+static int sd_grow(double **p, size_t *cap, size_t n)
+{
+	if (n <= *cap)
+		return SSX_OK;
+	hipFree(*p);
+	*p = NULL;
+	*cap = 0;
+	if (hipMalloc((void **) p, sizeof(double) * n) != hipSuccess) {
+		*p = NULL;
+		ssx_set_error("hipMalloc of a %zu-entry front", n);
+		return SSX_ENOMEM;
+	}
+	*cap = n;
+	return SSX_OK;
+}
+
+static void sd_gpu_free(sd_gpu *g)
+{
+	hipFree(g->dF); hipFree(g->dM); hipFree(g->dK); hipFree(g->dinfo);
+}
+
+/* F (host, nr x nr column-major, lower part): M = L11^-1 (nc x nc), K
+ * (nu x nc) and the update matrix Up (nu x nu) on return; SSX_ENUMERIC if
+ * F11 is not positive definite */
+static int front_gpu(sd_gpu *g, const double *F, int nr, int nc, double *M,
+		double *K, double *Up)
+{
+	const int nu = nr - nc;
+	const double one = 1.0, mone = -1.0;
+	int rc = sd_grow(&g->dF, &g->capF, (size_t) nr * nr);
+	if (!rc) rc = sd_grow(&g->dM, &g->capM, (size_t) nc * nc);
+	if (!rc && nu > 0) rc = sd_grow(&g->dK, &g->capK, (size_t) nu * nc);
+	if (!rc && !g->dinfo && hipMalloc((void **) &g->dinfo,
+			2 * sizeof(rocblas_int)) != hipSuccess) {
+		g->dinfo = NULL;
+		rc = SSX_ENOMEM;
+	}
+	if (rc)
+		return rc;
+	rocblas_int info[2] = { 0, 0 };
+	int bad = hipMemcpyAsync(g->dF, F, sizeof(double) * (size_t) nr * nr,
+			hipMemcpyHostToDevice, g->st) != hipSuccess;
+	bad |= rocsolver_dpotrf(g->h, rocblas_fill_lower, nc, g->dF, nr,
+			g->dinfo) != rocblas_status_success;
+	if (nu > 0) {
+		bad |= rocblas_dtrsm(g->h, rocblas_side_right, rocblas_fill_lower,
+				rocblas_operation_transpose, rocblas_diagonal_non_unit, nu,
+				nc, &one, g->dF, nr, g->dF + nc, nr)
+				!= rocblas_status_success;
+		bad |= rocblas_dsyrk(g->h, rocblas_fill_lower,
+				rocblas_operation_none, nu, nc, &mone, g->dF + nc, nr, &one,
+				g->dF + nc + (size_t) nc * nr, nr) != rocblas_status_success;
+	}
+	bad |= hipMemcpy2DAsync(g->dM, sizeof(double) * nc, g->dF,
+			sizeof(double) * nr, sizeof(double) * nc, nc,
+			hipMemcpyDeviceToDevice, g->st) != hipSuccess;
+	bad |= rocsolver_dtrtri(g->h, rocblas_fill_lower,
+			rocblas_diagonal_non_unit, nc, g->dM, nc, g->dinfo + 1)
+			!= rocblas_status_success;
+	if (nu > 0)
+		bad |= rocblas_dtrmm(g->h, rocblas_side_right, rocblas_fill_lower,
+				rocblas_operation_none, rocblas_diagonal_non_unit, nu, nc,
+				&one, g->dM, nc, g->dF + nc, nr, g->dK, nu)
+				!= rocblas_status_success;
+	bad |= hipMemcpyAsync(M, g->dM, sizeof(double) * (size_t) nc * nc,
+			hipMemcpyDeviceToHost, g->st) != hipSuccess;
+	if (nu > 0) {
+		bad |= hipMemcpyAsync(K, g->dK, sizeof(double) * (size_t) nu * nc,
+				hipMemcpyDeviceToHost, g->st) != hipSuccess;
+		bad |= hipMemcpy2DAsync(Up, sizeof(double) * nu,
+				g->dF + nc + (size_t) nc * nr, sizeof(double) * nr,
+				sizeof(double) * nu, nu, hipMemcpyDeviceToHost, g->st)
+				!= hipSuccess;
+	}
+	bad |= hipMemcpyAsync(info, g->dinfo, sizeof(info),
+			hipMemcpyDeviceToHost, g->st) != hipSuccess;
+	bad |= hipStreamSynchronize(g->st) != hipSuccess;
+	if (bad) {
+		ssx_set_error("sparse direct: device factorisation of a front of "
+				"%d rows failed", nr);
+		return SSX_EHIP;
+	}
+	if (info[0] != 0 || info[1] != 0)
+		return SSX_ENUMERIC;
+	/* the strict upper part of M: zero, as the host path leaves it */
+	for (int j = 1; j < nc; ++j)
+		memset(M + (size_t) j * nc, 0, sizeof(double) * (size_t) j);
+	return SSX_OK;
+}
+
 void ssx_sd_free(ssx_sdfactor *f)
 {
 	if (!f)
 		return;
 	free(f->node); free(f->perm); free(f->lev_off); free(f->lev_sn);
-	free(f->val); free(f->idx); free(f->wk_off);
+	free(f->val); free(f->idx); free(f->wk_off); free(f->wk_vec);
 	free(f->wk);
 	hipFree(f->d_node); hipFree(f->d_val);
 	hipFree(f->d_idx); hipFree(f->d_perm); hipFree(f->d_wk);
@@ -310,8 +418,10 @@ void ssx_sd_free(ssx_sdfactor *f)
 }
 
 int ssx_sd_factor(int n, const int *rp, const int *ci, const double *a,
-		const double *xyz, ssx_sdfactor **out)
+		const double *xyz, void *blas, void *stream, ssx_sdfactor **out)
 {
+	sd_gpu gpu = { (rocblas_handle) blas, (hipStream_t) stream, NULL, NULL,
+		NULL, 0, 0, 0, NULL };
 	*out = NULL;
 	if (n <= 0) {
 		ssx_set_error("sparse direct: empty system");
@@ -541,12 +651,34 @@ int ssx_sd_factor(int n, const int *rp, const int *ci, const double *a,
 			pos[p->c0 + j] = -1;
 		for (int k = 0; k < nu; ++k)
 			pos[U[s][k]] = -1;
-		if (rc == SSX_OK && front_factor(F, nr, nc, &f->flops)) {
+		const int dev = blas && nr >= SD_GPU_FRONT && nc > 0;
+		if (rc == SSX_OK && dev) {
+			double *Up = nu > 0 ? malloc(sizeof(double) * (size_t) nu * nu)
+				: NULL;
+			if (nu > 0 && !Up)
+				rc = SSX_ENOMEM;
+			if (rc == SSX_OK)
+				rc = front_gpu(&gpu, F, nr, nc, f->val + p->moff,
+						f->val + p->koff, Up);
+			if (rc == SSX_ENUMERIC)
+				ssx_set_error("sparse direct: the coarse operator is not "
+						"positive definite (supernode %d)", s);
+			if (rc == SSX_OK) {
+				upd[s] = Up;
+				double fl = 0.0;
+				for (int j = 0; j < nc; ++j)
+					fl += (double) (nr - j) * (nr - j);
+				f->flops += fl + (double) nu * nc * nc;
+			} else {
+				free(Up);
+			}
+		}
+		if (rc == SSX_OK && !dev && front_factor(F, nr, nc, &f->flops)) {
 			ssx_set_error("sparse direct: the coarse operator is not "
 					"positive definite (supernode %d)", s);
 			rc = SSX_ENUMERIC;
 		}
-		if (rc == SSX_OK) {
+		if (rc == SSX_OK && !dev) {
 			double *M = f->val + p->moff;
 			tri_inverse(F, nr, nc, M);
 			/* K = L21 M (nu x nc): the rows below apply to b~ directly,
@@ -563,14 +695,6 @@ int ssx_sd_factor(int n, const int *rp, const int *ci, const double *a,
 				}
 			}
 			f->flops += (double) nu * nc * nc;
-			/* row-major copies for the forward products */
-			double *Mr = f->val + p->mroff, *Kr = f->val + p->kroff;
-			for (int j = 0; j < nc; ++j)
-				for (int i = 0; i < nc; ++i)
-					Mr[(size_t) i * nc + j] = M[i + (size_t) j * nc];
-			for (int j = 0; j < nc; ++j)
-				for (int r = 0; r < nu; ++r)
-					Kr[(size_t) r * nc + j] = K[r + (size_t) j * nu];
 			if (nu > 0) {
 				double *Up = malloc(sizeof(double) * (size_t) nu * nu);
 				if (!Up) {
@@ -584,6 +708,17 @@ int ssx_sd_factor(int n, const int *rp, const int *ci, const double *a,
 				}
 			}
 		}
+		if (rc == SSX_OK) {
+			const double *M = f->val + p->moff, *K = f->val + p->koff;
+			/* row-major copies for the forward products */
+			double *Mr = f->val + p->mroff, *Kr = f->val + p->kroff;
+			for (int j = 0; j < nc; ++j)
+				for (int i = 0; i < nc; ++i)
+					Mr[(size_t) i * nc + j] = M[i + (size_t) j * nc];
+			for (int j = 0; j < nc; ++j)
+				for (int r = 0; r < nu; ++r)
+					Kr[(size_t) r * nc + j] = K[r + (size_t) j * nu];
+		}
 		free(F);
 	}
 	if (upd) {
@@ -596,6 +731,7 @@ int ssx_sd_factor(int n, const int *rp, const int *ci, const double *a,
 			free(U[s]);
 		free(U);
 	}
+	sd_gpu_free(&gpu);
 	free(inv);
 	free(d.mark);
 	free(d.first);
@@ -742,7 +878,7 @@ int ssxh_sd_test(int n, const int *rp, const int *ci, const double *a,
 		const double *xyz, const double *b, double *x, double *info)
 {
 	ssx_sdfactor *f = NULL;
-	const int rc = ssx_sd_factor(n, rp, ci, a, xyz, &f);
+	const int rc = ssx_sd_factor(n, rp, ci, a, xyz, NULL, NULL, &f);
 	if (rc != SSX_OK)
 		return rc;
 	const int rs = ssx_sd_solve_host(f, b, x);
@@ -793,8 +929,9 @@ int ssx_sd_upload(ssx_sdfactor *f)
 			+ ssxk_sd_blocks(p->nc, p->nc + p->nu);
 	}
 	f->wk_off = malloc(sizeof(int) * (2 * (size_t) f->nlev + 1));
+	f->wk_vec = calloc(2 * (size_t) f->nlev, sizeof(int));
 	f->wk = malloc(sizeof(ssx_sdwork) * (size_t) (tot > 0 ? tot : 1));
-	if (!f->wk_off || !f->wk) {
+	if (!f->wk_off || !f->wk_vec || !f->wk) {
 		ssx_set_error("out of host memory (sparse direct tables)");
 		return SSX_ENOMEM;
 	}
@@ -807,6 +944,9 @@ int ssx_sd_upload(ssx_sdfactor *f)
 				const ssx_sdnode *p = &f->node[s];
 				const int nb = d == 0 ? ssxk_sd_blocks(p->nc + p->nu, p->nc)
 					: ssxk_sd_blocks(p->nc, p->nc + p->nu);
+				const int vec = d == 0 ? p->nc : p->nc + p->nu;
+				if (vec > f->wk_vec[2 * l + d])
+					f->wk_vec[2 * l + d] = vec;
 				for (int b = 0; b < nb; ++b) {
 					f->wk[m].sn = s;
 					f->wk[m].blk = b;
@@ -869,12 +1009,13 @@ static int sd_levels(ssx_sdfactor *f, void *st)
 	for (int l = 0; rc == SSX_OK && l < f->nlev; ++l) {
 		const int *o = f->wk_off + 2 * l;
 		rc = ssxk_sd_fwd(st, o[1] - o[0], f->d_wk + o[0], f->d_node,
-				f->d_val, f->d_idx, f->d_b, f->d_y, f->d_t);
+				f->d_val, f->d_idx, f->d_b, f->d_y, f->d_t,
+				f->wk_vec[2 * l]);
 	}
 	for (int l = f->nlev - 1; rc == SSX_OK && l >= 0; --l) {
 		const int *o = f->wk_off + 2 * l;
 		rc = ssxk_sd_bwd(st, o[2] - o[1], f->d_wk + o[1], f->d_node,
-				f->d_val, f->d_idx, f->d_y, f->d_x);
+				f->d_val, f->d_idx, f->d_y, f->d_x, f->wk_vec[2 * l + 1]);
 	}
 	return rc;
 }

@@ -138,7 +138,7 @@ k_sd_fwd(const ssx_sdwork *__restrict__ wk,
 		const int *__restrict__ idx, const double *__restrict__ b,
 		double *__restrict__ y, double *__restrict__ t)
 {
-	__shared__ double v[SD_MAX_VEC];
+	extern __shared__ double v[];         // nc entries (the level's largest)
 	const ssx_sdwork w = wk[blockIdx.x];
 	const ssx_sdnode p = node[w.sn];
 	const int nc = p.nc, nu = p.nu;
@@ -174,7 +174,7 @@ k_sd_bwd(const ssx_sdwork *__restrict__ wk,
 		const int *__restrict__ idx, const double *__restrict__ y,
 		double *__restrict__ x)
 {
-	__shared__ double v[SD_MAX_VEC];
+	extern __shared__ double v[];         // nc + nu entries
 	const ssx_sdwork w = wk[blockIdx.x];
 	const ssx_sdnode p = node[w.sn];
 	const int nc = p.nc, nu = p.nu;
@@ -232,13 +232,37 @@ extern "C" int ssxk_sd_scatter_map(void *st, int m, const int *map,
 	return SSX_OK;
 }
 
+// dynamic LDS beyond 64 KB needs the kernel's attribute raised once
+static int sd_lds_attr(const void *fn, int *done)
+{
+	if (*done)
+		return SSX_OK;
+	if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+			(int) (sizeof(double) * SD_MAX_VEC)) != hipSuccess) {
+		ssx_set_error("sparse direct: %zu bytes of LDS refused",
+				sizeof(double) * SD_MAX_VEC);
+		return SSX_EHIP;
+	}
+	*done = 1;
+	return SSX_OK;
+}
+
 extern "C" int ssxk_sd_fwd(void *st, int nwk, const ssx_sdwork *wk,
 		const ssx_sdnode *node, const double *val, const int *idx,
-		const double *b, double *y, double *t)
+		const double *b, double *y, double *t, int vec)
 {
+	static int attr = 0;
 	if (nwk <= 0)
 		return SSX_OK;
-	hipLaunchKernelGGL(k_sd_fwd, dim3(nwk), dim3(256), 0, (hipStream_t) st,
+	if (vec > SD_MAX_VEC) {
+		ssx_set_error("sparse direct: %d-entry vector beyond %d", vec,
+				SD_MAX_VEC);
+		return SSX_EINVAL;
+	}
+	if (sd_lds_attr((const void *) k_sd_fwd, &attr))
+		return SSX_EHIP;
+	hipLaunchKernelGGL(k_sd_fwd, dim3(nwk), dim3(256),
+			sizeof(double) * (size_t) (vec > 0 ? vec : 1), (hipStream_t) st,
 			wk, node, val, idx, b, y, t);
 	SD_CHECK("k_sd_fwd");
 	return SSX_OK;
@@ -246,11 +270,20 @@ extern "C" int ssxk_sd_fwd(void *st, int nwk, const ssx_sdwork *wk,
 
 extern "C" int ssxk_sd_bwd(void *st, int nwk, const ssx_sdwork *wk,
 		const ssx_sdnode *node, const double *val, const int *idx,
-		const double *y, double *x)
+		const double *y, double *x, int vec)
 {
+	static int attr = 0;
 	if (nwk <= 0)
 		return SSX_OK;
-	hipLaunchKernelGGL(k_sd_bwd, dim3(nwk), dim3(256), 0, (hipStream_t) st,
+	if (vec > SD_MAX_VEC) {
+		ssx_set_error("sparse direct: %d-entry vector beyond %d", vec,
+				SD_MAX_VEC);
+		return SSX_EINVAL;
+	}
+	if (sd_lds_attr((const void *) k_sd_bwd, &attr))
+		return SSX_EHIP;
+	hipLaunchKernelGGL(k_sd_bwd, dim3(nwk), dim3(256),
+			sizeof(double) * (size_t) (vec > 0 ? vec : 1), (hipStream_t) st,
 			wk, node, val, idx, y, x);
 	SD_CHECK("k_sd_bwd");
 	return SSX_OK;
