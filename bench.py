@@ -608,10 +608,12 @@ def main():
                     "device copy (known bytes, to calibrate FETCH_SIZE/"
                     "WRITE_SIZE)")
     ap.add_argument("--profile-case", default="cube",
-                    choices=("cube", "solve32", "particles", "ns"),
+                    choices=("cube", "solve32", "solve32_bf16", "particles",
+                             "ns"),
                     help="with --profile: the workload whose kernels run "
                     "(cube: the solver kernels above; solve32: one whole "
                     "FCG + p-MG solve of the cube with the FP32 smoother; "
+                    "solve32_bf16: the same with BF16 operator factors; "
                     "particles: the 1M-particle channel step and RK2 "
                     "tracers; ns: two channel Navier-Stokes steps), each "
                     "followed by the 1 GiB calibration copy")
@@ -633,7 +635,8 @@ def main():
         dst.copy_(src)                                   # 1 GiB each way
         torch.cuda.synchronize()
 
-    if args.profile and args.profile_case not in ("cube", "solve32"):
+    if args.profile and args.profile_case not in ("cube", "solve32",
+                                                  "solve32_bf16"):
         stream = torch.cuda.current_stream()
         if args.profile_case == "particles":
             r = particle_case(S, torch, stream)
@@ -688,8 +691,9 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / 1e3 / steps
 
-    if args.profile and args.profile_case == "solve32":
-        solve_case(S, torch, ctx, mesh, args.max_iter, precs=("fp32",),
+    if args.profile and args.profile_case.startswith("solve32"):
+        prec = "bf16" if args.profile_case.endswith("bf16") else "fp32"
+        solve_case(S, torch, ctx, mesh, args.max_iter, precs=(prec,),
                    log_prefix="profile ")
         calibration_copy()
         log("[bench] profile mode done")

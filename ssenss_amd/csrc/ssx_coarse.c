@@ -346,6 +346,33 @@ void ssx_coarse_direct_free(ssx_ctx *c)
 	c->cd_blas = NULL;
 }
 
+/* the context's rocBLAS handle on stream st, created at first use with
+ * atomics disallowed: rocBLAS may otherwise sum partial results with
+ * atomics in an order that depends on timing, and the factors must come
+ * out the same bits every run */
+static int coarse_blas(ssx_ctx *c, hipStream_t st)
+{
+	if (!c->cd_blas) {
+		rocblas_handle h;
+		if (rocblas_create_handle(&h) != rocblas_status_success) {
+			ssx_set_error("rocblas_create_handle failed");
+			return SSX_EHIP;
+		}
+		c->cd_blas = h;
+		if (rocblas_set_atomics_mode(h, rocblas_atomics_not_allowed)
+				!= rocblas_status_success) {
+			ssx_set_error("rocblas_set_atomics_mode failed");
+			return SSX_EHIP;
+		}
+	}
+	if (rocblas_set_stream((rocblas_handle) c->cd_blas, st)
+			!= rocblas_status_success) {
+		ssx_set_error("rocblas_set_stream failed");
+		return SSX_EHIP;
+	}
+	return SSX_OK;
+}
+
 static int coarse_direct_build(ssx_ctx *c, int l, int op, int pres)
 {
 	ssx_coarse *cs = &c->lev[l].cs;
@@ -366,19 +393,8 @@ static int coarse_direct_build(ssx_ctx *c, int l, int op, int pres)
 		}
 		c->cd_n = nu;
 	}
-	if (!c->cd_blas) {
-		rocblas_handle h;
-		if (rocblas_create_handle(&h) != rocblas_status_success) {
-			ssx_set_error("rocblas_create_handle failed");
-			return SSX_EHIP;
-		}
-		c->cd_blas = h;
-	}
+	TRYQ(coarse_blas(c, st));
 	rocblas_handle h = (rocblas_handle) c->cd_blas;
-	if (rocblas_set_stream(h, st) != rocblas_status_success) {
-		ssx_set_error("rocblas_set_stream failed");
-		return SSX_EHIP;
-	}
 	c->cd_op = op;
 	c->cd_pres = pres;
 	c->cd_lambda = c->lambda;
@@ -542,20 +558,8 @@ static int coarse_sparse_build(ssx_ctx *c, int l, int op, int pres)
 	c->sd_sing = singular;
 	ssx_sdfactor *f = NULL;
 	/* the large fronts go to the device through rocSOLVER / rocBLAS */
-	if (rc == SSX_OK && !c->cd_blas) {
-		rocblas_handle h;
-		if (rocblas_create_handle(&h) != rocblas_status_success) {
-			ssx_set_error("rocblas_create_handle failed");
-			rc = SSX_EHIP;
-		} else {
-			c->cd_blas = h;
-		}
-	}
-	if (rc == SSX_OK && rocblas_set_stream((rocblas_handle) c->cd_blas,
-			st) != rocblas_status_success) {
-		ssx_set_error("rocblas_set_stream failed");
-		rc = SSX_EHIP;
-	}
+	if (rc == SSX_OK)
+		rc = coarse_blas(c, st);
 	if (rc == SSX_OK && n > 0)
 		rc = ssx_sd_factor(n, rr, rc_, ra, xyz, c->cd_blas, st, &f);
 	else if (rc == SSX_OK)

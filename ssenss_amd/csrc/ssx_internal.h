@@ -187,6 +187,7 @@ typedef struct ssx_sdfactor {
 	double *val;
 	int *idx;
 	double flops;
+	int ndev;                    /* fronts factored on the device */
 	int *wk_off;                 /* [2 nlev + 1] work-table ranges: per
 	                                level, forward then backward         */
 	int *wk_vec;                 /* [2 nlev] LDS entries of those launches */

@@ -354,8 +354,13 @@ static int front_gpu(sd_gpu *g, const double *F, int nr, int nc, double *M,
 	if (rc)
 		return rc;
 	rocblas_int info[2] = { 0, 0 };
-	int bad = hipMemcpyAsync(g->dF, F, sizeof(double) * (size_t) nr * nr,
-			hipMemcpyHostToDevice, g->st) != hipSuccess;
+	/* blocking copies and a device-wide wait on either side of the
+	 * library calls: the front and its results live in pageable host
+	 * memory, and nothing here may depend on how the runtime orders an
+	 * asynchronous pageable copy against library kernels */
+	int bad = hipStreamSynchronize(g->st) != hipSuccess;
+	bad |= hipMemcpy(g->dF, F, sizeof(double) * (size_t) nr * nr,
+			hipMemcpyHostToDevice) != hipSuccess;
 	bad |= rocsolver_dpotrf(g->h, rocblas_fill_lower, nc, g->dF, nr,
 			g->dinfo) != rocblas_status_success;
 	if (nu > 0) {
@@ -367,9 +372,10 @@ static int front_gpu(sd_gpu *g, const double *F, int nr, int nc, double *M,
 				rocblas_operation_none, nu, nc, &mone, g->dF + nc, nr, &one,
 				g->dF + nc + (size_t) nc * nr, nr) != rocblas_status_success;
 	}
-	bad |= hipMemcpy2DAsync(g->dM, sizeof(double) * nc, g->dF,
+	bad |= hipDeviceSynchronize() != hipSuccess;
+	bad |= hipMemcpy2D(g->dM, sizeof(double) * nc, g->dF,
 			sizeof(double) * nr, sizeof(double) * nc, nc,
-			hipMemcpyDeviceToDevice, g->st) != hipSuccess;
+			hipMemcpyDeviceToDevice) != hipSuccess;
 	bad |= rocsolver_dtrtri(g->h, rocblas_fill_lower,
 			rocblas_diagonal_non_unit, nc, g->dM, nc, g->dinfo + 1)
 			!= rocblas_status_success;
@@ -378,19 +384,20 @@ static int front_gpu(sd_gpu *g, const double *F, int nr, int nc, double *M,
 				rocblas_operation_none, rocblas_diagonal_non_unit, nu, nc,
 				&one, g->dM, nc, g->dF + nc, nr, g->dK, nu)
 				!= rocblas_status_success;
-	bad |= hipMemcpyAsync(M, g->dM, sizeof(double) * (size_t) nc * nc,
-			hipMemcpyDeviceToHost, g->st) != hipSuccess;
+	bad |= hipStreamSynchronize(g->st) != hipSuccess;
+	bad |= hipDeviceSynchronize() != hipSuccess;
+	bad |= hipMemcpy(M, g->dM, sizeof(double) * (size_t) nc * nc,
+			hipMemcpyDeviceToHost) != hipSuccess;
 	if (nu > 0) {
-		bad |= hipMemcpyAsync(K, g->dK, sizeof(double) * (size_t) nu * nc,
-				hipMemcpyDeviceToHost, g->st) != hipSuccess;
-		bad |= hipMemcpy2DAsync(Up, sizeof(double) * nu,
+		bad |= hipMemcpy(K, g->dK, sizeof(double) * (size_t) nu * nc,
+				hipMemcpyDeviceToHost) != hipSuccess;
+		bad |= hipMemcpy2D(Up, sizeof(double) * nu,
 				g->dF + nc + (size_t) nc * nr, sizeof(double) * nr,
-				sizeof(double) * nu, nu, hipMemcpyDeviceToHost, g->st)
+				sizeof(double) * nu, nu, hipMemcpyDeviceToHost)
 				!= hipSuccess;
 	}
-	bad |= hipMemcpyAsync(info, g->dinfo, sizeof(info),
-			hipMemcpyDeviceToHost, g->st) != hipSuccess;
-	bad |= hipStreamSynchronize(g->st) != hipSuccess;
+	bad |= hipMemcpy(info, g->dinfo, sizeof(info), hipMemcpyDeviceToHost)
+			!= hipSuccess;
 	if (bad) {
 		ssx_set_error("sparse direct: device factorisation of a front of "
 				"%d rows failed", nr);
@@ -417,8 +424,9 @@ void ssx_sd_free(ssx_sdfactor *f)
 	free(f);
 }
 
-int ssx_sd_factor(int n, const int *rp, const int *ci, const double *a,
-		const double *xyz, void *blas, void *stream, ssx_sdfactor **out)
+static int sd_factor_run(int n, const int *rp, const int *ci,
+		const double *a, const double *xyz, void *blas, void *stream,
+		ssx_sdfactor **out)
 {
 	sd_gpu gpu = { (rocblas_handle) blas, (hipStream_t) stream, NULL, NULL,
 		NULL, 0, 0, 0, NULL };
@@ -665,6 +673,7 @@ int ssx_sd_factor(int n, const int *rp, const int *ci, const double *a,
 						"positive definite (supernode %d)", s);
 			if (rc == SSX_OK) {
 				upd[s] = Up;
+				f->ndev++;
 				double fl = 0.0;
 				for (int j = 0; j < nc; ++j)
 					fl += (double) (nr - j) * (nr - j);
@@ -791,6 +800,66 @@ int ssx_sd_factor(int n, const int *rp, const int *ci, const double *a,
 	return SSX_OK;
 }
 
+/* |x - x*|_max / |x*|_max for the host solve of A x = A x*, x* a fixed
+ * vector in [1, 2): about 1e-13 for a sound factor of these operators */
+static int sd_check(const ssx_sdfactor *f, const int *rp, const int *ci,
+		const double *a, double *err)
+{
+	const int n = f->n;
+	double *xs = malloc(sizeof(double) * (size_t) n);
+	double *b = malloc(sizeof(double) * (size_t) n);
+	double *x = malloc(sizeof(double) * (size_t) n);
+	int rc = xs && b && x ? SSX_OK : SSX_ENOMEM;
+	if (rc == SSX_OK) {
+		for (int i = 0; i < n; ++i)
+			xs[i] = 1.0 + (double) (((unsigned) i * 2654435761u) >> 22)
+				/ 1024.0;
+		for (int i = 0; i < n; ++i) {
+			double s = 0.0;
+			for (int q = rp[i]; q < rp[i + 1]; ++q)
+				s += a[q] * xs[ci[q]];
+			b[i] = s;
+		}
+		rc = ssx_sd_solve_host(f, b, x);
+	}
+	if (rc == SSX_OK) {
+		double e = 0.0, m = 0.0;
+		for (int i = 0; i < n; ++i) {
+			const double d = fabs(x[i] - xs[i]);
+			e = d > e || d != d ? d : e;
+			m = fabs(xs[i]) > m ? fabs(xs[i]) : m;
+		}
+		*err = e / m;
+	}
+	free(xs); free(b); free(x);
+	return rc;
+}
+
+/* A factor with fronts from rocSOLVER / rocBLAS is checked before use:
+ * the host solve of A x = A x* must return x* to 1e-9, or the factor is
+ * computed again on the host alone.  A device front reported as not
+ * positive definite is also settled by the host factorisation: under a
+ * kernel profiler the device fronts have failed that way on operators the
+ * host factors soundly. */
+int ssx_sd_factor(int n, const int *rp, const int *ci, const double *a,
+		const double *xyz, void *blas, void *stream, ssx_sdfactor **out)
+{
+	int rc = sd_factor_run(n, rp, ci, a, xyz, blas, stream, out);
+	if (rc == SSX_ENUMERIC && blas)      /* the host decides definiteness */
+		return sd_factor_run(n, rp, ci, a, xyz, NULL, NULL, out);
+	if (rc != SSX_OK || (*out)->ndev == 0)
+		return rc;
+	double err = 0.0;
+	rc = sd_check(*out, rp, ci, a, &err);
+	if (rc == SSX_OK && err <= 1e-9)
+		return SSX_OK;
+	ssx_sd_free(*out);
+	*out = NULL;
+	if (rc != SSX_OK)
+		return rc;
+	return sd_factor_run(n, rp, ci, a, xyz, NULL, NULL, out);
+}
+
 /* The solve on the host, step for step as on the device (a check of the
  * factorisation and of the device kernels).  b and x in the system's own
  * numbering.
@@ -872,8 +941,8 @@ int ssx_sd_solve_host(const ssx_sdfactor *f, const double *b, double *x)
 }
 
 /* test hooks: factor a host system and solve it on the host, reporting
- * the supernode count, levels, factor entries, flops and the largest
- * supernode (columns, rows below) */
+ * the supernode count, levels, factor entries, flops, the largest
+ * supernode (columns, rows below) and the factor check's error */
 int ssxh_sd_test(int n, const int *rp, const int *ci, const double *a,
 		const double *xyz, const double *b, double *x, double *info)
 {
@@ -894,6 +963,10 @@ int ssxh_sd_test(int n, const int *rp, const int *ci, const double *a,
 		info[3] = f->flops;
 		info[4] = maxnc;
 		info[5] = maxnu;
+		double err = -1.0;
+		if (sd_check(f, rp, ci, a, &err) != SSX_OK)
+			err = -1.0;
+		info[6] = err;
 	}
 	ssx_sd_free(f);
 	return rs;

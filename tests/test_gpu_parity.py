@@ -577,16 +577,16 @@ def test_operator_launch_variants_bit_identical(name):
         assert rel(outs[(S.OP_POISSON, 0)], Z["ref_ax_poisson"]) <= 1e-12
         assert rel(outs[(S.OP_HELMHOLTZ, 0)], Z["ref_ax_helmholtz"]) <= 1e-12
         # the reduced-precision copies are tuned when first built; forcing
-        # each shape leaves the FP64 and the mixed solves unchanged bit for
-        # bit (one operator partial per element, whatever the shape), so a
+        # each shape leaves the FP64 and the mixed solves (FP32 and BF16
+        # factors) unchanged bit for bit (one operator partial per element, whatever the shape), so a
         # solve does not depend on what the autotuner timed
         ctx.set_diag_choice(0)
-        for prec in (S.PREC_FP64, S.PREC_FP32):
+        for prec in (S.PREC_FP64, S.PREC_FP32, S.PREC_BF16):
             ctx.set_option(S.OPT_SMOOTHER_PRECISION, prec)
             its = []
             for v in range(S.AX_NVARIANTS):
                 for l in range(ctx.nlevels - 1):
-                    ctx.set_ax_variant(l, S.PREC_FP32, v)
+                    ctx.set_ax_variant(l, prec, v)
                     ctx.set_ax_variant(l, S.PREC_FP64, v)
                 f = dev(Z["in_f"])
                 x = torch.zeros_like(f)
