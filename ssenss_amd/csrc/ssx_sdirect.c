@@ -807,7 +807,7 @@ static int sd_check(const ssx_sdfactor *f, const int *rp, const int *ci,
 {
 	const int n = f->n;
 	double *xs = malloc(sizeof(double) * (size_t) n);
-	double *b = malloc(sizeof(double) * (size_t) n);
+	double *b = calloc((size_t) n, sizeof(double));
 	double *x = malloc(sizeof(double) * (size_t) n);
 	int rc = xs && b && x ? SSX_OK : SSX_ENOMEM;
 	if (rc == SSX_OK) {
