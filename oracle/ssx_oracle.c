@@ -1655,9 +1655,23 @@ int orc_tracers_step(orc_ctx *c, orc_particle *ps, long n, double t_now,
 			const int nb = c->neigh_copy[p->el * 6 + k];
 			x[k / 2] = (k % 2) ? 1.0 : -1.0;
 			if (nb >= 0) {
+				/* the entry face: the face of nb that leads back to
+				 * the element left, the opposite one first; the other
+				 * two coordinates keep their order */
+				int kin = k ^ 1;
+				if (c->neigh_copy[nb * 6 + kin] != p->el)
+					for (int i = 0; i < 6; ++i)
+						if (c->neigh_copy[nb * 6 + i] == p->el)
+							kin = i;
+				const int d0 = k / 2, d = kin / 2;
+				const double xa = d0 == 0 ? x[1] : x[0];
+				const double xb = d0 == 2 ? x[1] : x[2];
+				for (int i = 0, j = 0; i < 3; ++i)
+					if (i != d)
+						x[i] = j++ == 0 ? xa : xb;
+				x[d] = (kin % 2) ? 1.0 : -1.0;
 				p->old_el = p->el;
 				p->el = nb;
-				x[k / 2] = (k % 2) ? -1.0 : 1.0;
 			} else {                        /* wall: stay on it */
 				p->t = t_now;
 				break;

@@ -849,8 +849,9 @@ k_particles_coop(const PField F, int pass, long coop_max,
 // reference coordinates of the owning element; a stage that leaves
 // [-1,1]^3 is replaced by a straight move to the boundary along the slope
 // that left (k1 if the predictor left, else (k1 + k2)/2), a hop across the
-// closest face with the crossed coordinate set to the opposite face, and
-// a new step over the remaining time.  At a wall the tracer stays on the
+// closest face onto the face of the neighbour that leads back (the
+// opposite face for aligned elements), and a new step over the remaining
+// time.  At a wall the tracer stays on the
 // wall for the rest of the step; after more than 10 passes it restarts at
 // the element centre (flagged), as the inertial walk does.
 // ---------------------------------------------------------------------
@@ -912,9 +913,26 @@ __device__ __forceinline__ bool tracer_iter(const PField &F,
 	const int nb = F.neigh[e * 6 + k];
 	put3(xs, k / 2, (k % 2) ? 1.0 : -1.0);
 	if (nb >= 0) {
+		// the entry face: the face of nb that leads back to e, the
+		// opposite one first (aligned elements; two elements periodic in
+		// one direction lead back across both faces), else found as
+		// new_location finds it; the other two coordinates keep their order
+		int kin = k ^ 1;
+		if (F.neigh[nb * 6 + kin] != e)
+			for (int i = 0; i < 6; ++i)
+				if (F.neigh[nb * 6 + i] == e)
+					kin = i;
+		const int d0 = k / 2, d = kin / 2;
+		const double xa = d0 == 0 ? xs[1] : xs[0];
+		const double xb = d0 == 2 ? xs[1] : xs[2];
+		int jj = 0;
+#pragma unroll
+		for (int i = 0; i < 3; ++i)
+			if (i != d)
+				put3(xs, i, jj++ == 0 ? xa : xb);
+		put3(xs, d, (kin % 2) ? 1.0 : -1.0);
 		oe = e;
 		e = nb;
-		put3(xs, k / 2, (k % 2) ? -1.0 : 1.0);
 		return false;
 	}
 	tt = F.T;                               // wall: stay on it
